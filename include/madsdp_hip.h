@@ -1,0 +1,173 @@
+/*
+ * madsdp_hip.h -- C ABI of the MI355X (gfx950) implementation of MadSDP's
+ * interior-point hot path.
+ *
+ * Two layers:
+ *
+ *  1. Solver layer (msdp_solver_*): one opaque solver owns a problem
+ *     (LowRankOpt.Model fields) and its iterate, resident in device memory,
+ *     and exposes the algorithmic-component seam of MadSDP: the driver steps
+ *     of src/solver.jl and the component methods of src/components.jl.
+ *     A Julia component binds these with `ccall`.
+ *
+ *  2. Kernel layer (msdp_k_*): the device building blocks on caller-owned
+ *     device pointers, on a caller-given HIP stream -- the replacements of
+ *     the reference's LAPACK/BLAS calls, exported for unit tests and for
+ *     callers that keep their own state.
+ *
+ * Conventions: matrices are column-major FP64 with explicit leading
+ * dimension (Julia's Matrix{Float64}).  Return value 0 = success; < 0 =
+ * API/HIP error (msdp_last_error() has the message); > 0 = numerical failure
+ * (the reference's `false`, e.g. the 1-based first non-positive pivot).
+ * Status codes are MadNLP.Status values (src/moi.jl:16-27).
+ */
+#ifndef MADSDP_HIP_H
+#define MADSDP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct msdp_solver msdp_solver;
+
+/* component selectors (src/components.jl) */
+enum {
+  MSDP_ALG_PRIMAL_DUAL = 0,     /* PrimalDual      components.jl:30-36 */
+  MSDP_ALG_DUAL_CENTERING = 1,  /* DualCentering   components.jl:57-65 */
+  MSDP_DIR_NT = 0,              /* NT              components.jl:96 */
+  MSDP_DIR_HKM = 1,             /* HKM             components.jl:110 */
+  MSDP_STEP_EIGEN = 0,          /* EigenStep       components.jl:132-135 */
+  MSDP_STEP_BACKTRACK = 1,      /* CholeskyBacktrack components.jl:146-150 */
+  MSDP_CENTER_MEHROTRA = 0,     /* Mehrotra        components.jl:169-172 */
+  MSDP_CENTER_ADAPTIVE = 1,     /* AdaptiveMu      components.jl:188-192 */
+  MSDP_INIT_SCALED_IDENTITY = 0,/* ScaledIdentity  components.jl:208-217 */
+  MSDP_INIT_POINT = 1,          /* InitialPoint    components.jl:243-251 */
+  MSDP_INIT_DUAL_SHIFT = 2,     /* DualShift       components.jl:228-232 */
+  MSDP_SCHUR_DENSE_CHOLESKY = 0,/* DenseCholesky   components.jl:273 */
+  MSDP_SCHUR_QR = 1             /* QRSchur         components.jl:300 */
+};
+
+/* MadSDPOptions (src/options.jl:3-15) and the component parameters. */
+typedef struct msdp_options {
+  double tol;              /* 1e-7 */
+  int32_t max_iter;        /* 100 */
+  double max_wall_time;    /* 3600 */
+  double mu_min;           /* 1e-14 */
+  double regularize_schur; /* 1e-12 */
+  int32_t max_schur_reg;   /* 8 */
+  int32_t algorithm, direction, step, centering, initial_point, schur;
+  double tau;              /* EigenStep tau 0.95 */
+  double shrink, boundary; /* CholeskyBacktrack 0.8, 0.95 */
+  double sigma_exponent;   /* Mehrotra 3 */
+  double initial_ratio, ratio; /* AdaptiveMu 0.25, 0.5 */
+  double identity_scale;   /* ScaledIdentity: <= 0 means automatic */
+  double initial_shift, max_shift; /* DualShift 1, 1e10 */
+  double dc_beta, dc_delta, dc_A;  /* DualCentering 0.2, 1, 2 (dc_A < 0: theoretical) */
+  int32_t dc_centering_max_iter;   /* 50 */
+} msdp_options;
+
+/* iteration data of MadSDPSolver (src/solver.jl:37-48) */
+typedef struct msdp_info {
+  double mu, sigma, alpha, beta, inf_pr, inf_du, inf_compl, obj_val;
+  double pobj, dobj;
+  int32_t iter, status, n_newton, n_predictor;
+  double dc_t, dc_lambda;
+} msdp_info;
+
+/* which iterate array msdp_solver_get/set moves */
+enum {
+  MSDP_X = 0, MSDP_S = 1, MSDP_Y = 2, MSDP_XLIN = 3, MSDP_ZLIN = 4,
+  MSDP_DELX = 5, MSDP_DELS = 6, MSDP_DELY = 7, MSDP_DELXLIN = 8, MSDP_DELZLIN = 9,
+  MSDP_RD = 10, MSDP_RP = 11, MSDP_RDLIN = 12, MSDP_W = 13, MSDP_G = 14, MSDP_SI = 15,
+  MSDP_CORR = 16, MSDP_LS = 17, MSDP_LX = 18, MSDP_H = 19, MSDP_D = 20
+};
+
+const char* msdp_last_error(void);
+void msdp_default_options(msdp_options* opt);
+
+/* ---- problem: LRO.Model(C, A, b, d_lin, C_lin, dims) (test/runtests.jl:16-31) ---- */
+/* replaces MadSDPSolver(model; ...) construction, src/solver.jl:87-160 */
+int msdp_solver_create(msdp_solver** out, int device, int32_t nblocks, const int64_t* dims,
+                       int64_t m, int64_t n_scalar);
+int msdp_solver_destroy(msdp_solver* s);
+/* LRO.grad(model, MatrixIndex(i)): C_i, n_i x n_i */
+int msdp_solver_set_cost_block(msdp_solver* s, int32_t block, const double* C);
+/* A_ij as COO entries of both triangles (LRO.jac sparse); the one-hot diagonal
+ * case (max-cut) is detected and given the Hadamard Schur kernel */
+int msdp_solver_set_constraints_coo(msdp_solver* s, int32_t block, int64_t nnz,
+                                    const int64_t* con, const int64_t* row, const int64_t* col,
+                                    const double* val);
+/* A_ij dense for the m_i constraints con_idx (sorted) touching the block;
+ * mats is m_i consecutive n_i x n_i matrices (BufferedModelForSchur dense path) */
+int msdp_solver_set_constraints_dense(msdp_solver* s, int32_t block, int64_t m_i,
+                                      const int64_t* con_idx, const double* mats);
+/* LRO.cons_constant (b), d_lin, C_lin (m x n_s column-major) */
+int msdp_solver_set_rhs(msdp_solver* s, const double* b);
+int msdp_solver_set_scalar_data(msdp_solver* s, const double* d_lin, const double* C_lin);
+int msdp_solver_configure(msdp_solver* s, const msdp_options* opt);
+/* InitialPoint(X, y; x): X blocks concatenated, y (m), x (n_s) or NULL */
+int msdp_solver_set_initial_point(msdp_solver* s, const double* X, const double* y,
+                                  const double* x);
+
+/* ---- driver (src/solver.jl:286-396) ---- */
+int msdp_solver_initialize(msdp_solver* s);          /* initialize!        solver.jl:286 */
+int msdp_solver_iterate(msdp_solver* s);             /* iterate!           solver.jl:316 */
+int msdp_solver_update_termination(msdp_solver* s);  /* update_termination! solver.jl:349 */
+int msdp_solver_solve(msdp_solver* s);               /* solve!             solver.jl:385 */
+int msdp_solver_get_info(const msdp_solver* s, msdp_info* out);
+/* number of iteration records and the per-iteration trace (pobj, dobj,
+ * inf_pr, inf_du, gap, mu, alpha, beta, sigma) */
+int msdp_solver_trace(const msdp_solver* s, int32_t maxrec, double* out9, int32_t* nrec);
+
+/* ---- component seam (src/primal_dual.jl, src/step.jl, src/directions.jl) ---- */
+int msdp_solver_update_residuals(msdp_solver* s);                 /* step.jl:147-182 */
+int msdp_solver_prepare(msdp_solver* s, int32_t* ok);             /* primal_dual.jl:29-37 */
+int msdp_solver_factorize_schur(msdp_solver* s, int32_t* ok);     /* solver.jl:299-314 */
+int msdp_solver_solve_directions(msdp_solver* s, double mu, int32_t corrected); /* primal_dual.jl:44-54 */
+int msdp_solver_correction(msdp_solver* s);                       /* step.jl:263-271 */
+int msdp_solver_step_lengths(msdp_solver* s, double* alpha, double* beta); /* step.jl:186-228 */
+int msdp_solver_affine_complementarity(msdp_solver* s, double alpha, double beta,
+                                       double* out);              /* step.jl:231-243 */
+int msdp_solver_take_step(msdp_solver* s, double alpha, double beta); /* primal_dual.jl:56-72 */
+int msdp_solver_schur_solve(msdp_solver* s, double* rhs_host);    /* schur_solvers.jl:21 */
+/* copy an iterate array (block index for matrix arrays, ignored otherwise) */
+int msdp_solver_get(const msdp_solver* s, int32_t which, int32_t block, double* host);
+int msdp_solver_set(msdp_solver* s, int32_t which, int32_t block, const double* host);
+/* wall-clock seconds spent on the device stream by the last solve */
+int msdp_solver_synchronize(msdp_solver* s);
+
+/* ---- kernel layer: device pointers, caller's stream ---- */
+/* C = alpha op(A) op(B) + beta C (BLAS dgemm; directions.jl, step.jl products) */
+int msdp_k_gemm(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
+                double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                double beta, double* C, int64_t ldc);
+/* lower Cholesky in place; *info_dev = 0 or first failing pivot (LAPACK potrf) */
+int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info_dev);
+/* _cholesky_factor! (blocks.jl:94-113): L = chol((M+M')/2) + flush; returns the
+ * info through *info_dev */
+int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm, double* L,
+                           int64_t ldl, int32_t* info_dev, double* scratch);
+size_t msdp_k_cholesky_factor_scratch(int32_t n);
+/* Linv = L^-1 for lower-triangular L (LAPACK trtri) */
+int msdp_k_trtri(void* stream, int32_t n, const double* L, int64_t ldl, double* Linv,
+                 int64_t ldi, double* work);
+size_t msdp_k_trtri_work(int32_t n);
+/* eigenvalues ascending (+ vectors) of a symmetric matrix (LAPACK syevd / LRO.eigen!) */
+int msdp_k_syevd(void* stream, int32_t n, const double* A, int64_t lda, double* w, double* V,
+                 int64_t ldv, int32_t vectors, double* work, int32_t* iwork);
+size_t msdp_k_syevd_work(int32_t n);
+size_t msdp_k_syevd_iwork(int32_t n);
+/* smallest eigenvalue of a symmetric matrix into *out_dev (LRO.lanczos_eigmin!) */
+int msdp_k_eig_min(void* stream, int32_t n, const double* A, int64_t lda, double* out_dev,
+                   double* work);
+/* Hadamard Schur complement of one-hot constraints: H[i,j] += L[i,j] R[j,i]
+ * for i >= j (max-cut H = W o W, src/components.jl:102-104) */
+int msdp_k_schur_onehot(void* stream, int32_t m, const double* L, int64_t ldl, const double* R,
+                        int64_t ldr, double* H, int64_t ldh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+// Shared declarations of the MadSDP gfx950 library: device helpers and the
+// host-side launchers of every kernel.  All matrices are column-major FP64
+// (Julia's `Matrix{Float64}` layout), leading dimension given explicitly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+#include <string>
+
+namespace msdp {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int WAVE = 64;
+
+// --------------------------------------------------------------------------
+// error plumbing: C-ABI functions return 0 on success, < 0 on API/HIP error,
+// > 0 on a numerical failure (first failing pivot).
+// --------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error();
+int fail_hip(hipError_t e, const char* what, const char* file, int line);
+#define MSDP_HIP(x)                                                      \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) return ::msdp::fail_hip(e_, #x, __FILE__, __LINE__); \
+  } while (0)
+#define MSDP_TRY(x)              \
+  do {                           \
+    int r_ = (x);                \
+    if (r_ < 0) return r_;       \
+  } while (0)
+
+// --------------------------------------------------------------------------
+// device helpers
+// --------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// Block-wide reduction for blockDim.x a multiple of 64 (<= 1024).
+template <int OP>  // 0 sum, 1 max, 2 min
+__device__ inline double block_reduce(double v, double* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = OP == 0 ? wave_sum(v) : (OP == 1 ? wave_max(v) : wave_min(v));
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int i = 1; i < nw; ++i) r = OP == 0 ? r + sh[i] : (OP == 1 ? fmax(r, sh[i]) : fmin(r, sh[i]));
+  __syncthreads();
+  return r;
+}
+
+// --------------------------------------------------------------------------
+// dense FP64 kernels (dense.hip)
+// --------------------------------------------------------------------------
+// C = alpha op(A) op(B) + beta C for a batch of `batch` problems separated by
+// the given strides; `lower` restricts the write (and the work) to C's lower
+// triangle, i.e. a SYRK-shaped product.
+int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+         double* C, int64_t ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0,
+         int64_t sC = 0, bool lower = false);
+// Lower Cholesky in place (reads the lower triangle).  *info_dev receives 0 or
+// the 1-based index of the first non-positive pivot; kernels after a failure
+// return immediately.
+int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info_dev, double* work);
+size_t potrf_work_elems(int n);
+// Batched Cholesky of small matrices (n <= 64), one workgroup each.
+int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t stride,
+                        int batch, int* info_dev);
+// Inverse of a lower-triangular matrix: Linv = L^-1 (lower, upper zeroed).
+int trtri_lower(hipStream_t s, int n, const double* L, int64_t ldl, double* Linv,
+                int64_t ldi, double* work);
+size_t trtri_work_elems(int n);
+// Cholesky flush of the factor: zero the upper triangle, zero entries below
+// eps^2 * max |L_ii| (src/blocks.jl:101-111).
+int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch);
+// elementwise helpers
+int symmetrize(hipStream_t s, int n, double* A, int64_t lda);
+int mirror_lower(hipStream_t s, int n, double* A, int64_t lda);  // upper := lower'
+int copy_mat(hipStream_t s, int rows, int cols, const double* A, int64_t lda, double* B,
+             int64_t ldb);
+int transpose(hipStream_t s, int rows, int cols, const double* A, int64_t lda, double* B,
+              int64_t ldb);
+// Z = a*X + b*Y (+ c*V if V), elementwise over an n x n matrix; Z may alias.
+int axpby(hipStream_t s, int64_t count, double a, const double* X, double b, const double* Y,
+          double* Z);
+int axpbypcz(hipStream_t s, int64_t count, double a, const double* X, double b,
+             const double* Y, double c, const double* V, double* Z);
+int set_identity(hipStream_t s, int n, double* A, int64_t lda, double scale);
+int fill(hipStream_t s, int64_t count, double v, double* X);
+int add_diag(hipStream_t s, int n, double* A, int64_t lda, const double* dev_val, double val);
+int scale_cols(hipStream_t s, int rows, int cols, double* A, int64_t lda, const double* d,
+               int power_half_sign);  // A[:,k] *= d[k]^(p/2), p in {-1,+1}
+int scale_rows(hipStream_t s, int rows, int cols, double* A, int64_t lda, const double* d,
+               int power_half_sign);
+// Reductions into a device scalar (deterministic two-pass).
+int dot(hipStream_t s, int64_t count, const double* X, const double* Y, double* out_dev,
+        double* scratch);
+int max_abs(hipStream_t s, int64_t count, const double* X, double* out_dev, double* scratch);
+size_t reduce_scratch_elems(int64_t count);
+
+// --------------------------------------------------------------------------
+// symmetric eigensolvers (eigen.hip)
+// --------------------------------------------------------------------------
+// Eigenvalues (ascending) and optionally eigenvectors (columns of V) of the
+// symmetric matrix A (full storage, read only).
+int syevd(hipStream_t s, int n, const double* A, int64_t lda, double* w, double* V,
+          int64_t ldv, bool vectors, double* work, int* iwork);
+size_t syevd_work_elems(int n);
+size_t syevd_iwork_elems(int n);
+// Smallest eigenvalue of the symmetric matrix A into *out_dev.
+int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out_dev,
+            double* work, int* iwork);
+// Batched Jacobi for many small matrices (n <= 64).
+int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t strideA,
+                   double* w, int64_t stridew, double* V, int64_t ldv, int64_t strideV,
+                   int batch, bool vectors);
+
+}  // namespace msdp
