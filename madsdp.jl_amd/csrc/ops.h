@@ -1,0 +1,69 @@
+// Constraint operators, Schur assembly, scalar-block and small solver
+// kernels (ops.hip).
+#pragma once
+#include "common.h"
+
+namespace msdp {
+
+// Device view of the constraint matrices touching one PSD block.
+struct BlockOps {
+  enum Kind { EMPTY = 0, COO = 1, DENSE = 2 };
+  int kind = EMPTY;
+  int n = 0;
+  bool one_hot = false;       // every constraint is v e_r e_r'
+  int ncon = 0;               // constraints touching the block
+  int* con_list = nullptr;    // [ncon] sorted constraint indices
+  // COO: entries sorted by constraint, con_ptr[ncon+1] ranges
+  int64_t nnz = 0;
+  int* con_ptr = nullptr;
+  int* con = nullptr;
+  int* row = nullptr;
+  int* col = nullptr;
+  double* val = nullptr;
+  // A*(y) by position: pos_lin[npos] linear index, pos_ptr[npos+1], pos_ent
+  int npos = 0;
+  int64_t* pos_lin = nullptr;
+  int* pos_ptr = nullptr;
+  int* pos_ent = nullptr;
+  // DENSE: ncon consecutive n x n matrices
+  double* mats = nullptr;
+  bool dcon_identity = false;  // con_list == 0..m-1
+  double* Bbuf = nullptr;      // svec/vec panel of the scaled constraints
+  double* Hbuf = nullptr;      // ncon x ncon block of H before the scatter
+};
+
+// (L, R) of H_jk = <A_j, L A_k R> and, for dense blocks, the factored form
+// P_j = op(Lm) A_j op(F2) with H_jk = <P_j, P_k>.
+struct SchurScaling {
+  const double* L = nullptr;
+  const double* R = nullptr;
+  const double* Lm = nullptr;
+  bool transL = false;
+  const double* F2 = nullptr;
+  bool transF2 = false;
+  bool full_vec = false;  // HKM: P_j is not symmetric
+};
+
+int jprod(hipStream_t st, const BlockOps& b, const double* X, double* out);
+int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out);
+int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
+                double* work, size_t work_elems);
+
+enum { SOP_ZINV = 0, SOP_RHS = 1, SOP_DIRS = 2, SOP_CORR = 3, SOP_XZ = 4 };
+int scalar_op(hipStream_t st, int op, int ns, double mu, const double* x, const double* z,
+              const double* zinv, const double* a, const double* b, double* out, double* out2,
+              bool corrected);
+int scalar_min_step(hipStream_t st, int ns, const double* x, const double* dx, double* out);
+int scalar_affine(hipStream_t st, int ns, const double* x, const double* dx, const double* z,
+                  const double* dz, double a, double b, double* out);
+int nt_rnt(hipStream_t st, int n, const double* M, const double* D, double* R);
+int sqrt_min(hipStream_t st, int n, const double* lam, double* D, double* minlam);
+int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, double* out);
+int potrs(hipStream_t st, int m, const double* L, int64_t ld, double* b);
+int scale_cols_vec(hipStream_t st, int rows, int cols, double* A, int64_t lda, const double* d);
+int vec_min(hipStream_t st, int n, const double* v, double* out);
+int schur_onehot_identity(hipStream_t st, int m, const double* L, int64_t ldl, const double* R,
+                          int64_t ldr, double* H, int64_t ldh);
+int dc_solve_alpha(hipStream_t st, int total, const double* mu, double A, double* out);
+
+}  // namespace msdp

@@ -1,0 +1,464 @@
+// Device kernels of the model callbacks the reference takes from LowRankOpt
+// and of the scalar (LP) block:
+//   A(X)_j    = sum_i <A_ij, X_i>               LRO.add_jprod!     (src/step.jl:19,75)
+//   A*(y)_i   = sum_j y_j A_ij                  LRO.unsafe_jtprod  (src/step.jl:30,97)
+//   H_jk     += <A_ij, L A_ik R>                LRO.schur_complement! (src/solver.jl:304)
+// Constraint storage per block: COO entries sorted by constraint (with a
+// by-position index for A*), or dense matrices of the touching constraints.
+#include "ops.h"
+
+#include <algorithm>
+
+namespace msdp {
+
+static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------- A(X) ----
+__global__ void jprod_coo_kernel(int ncon, const int* con_list, const int* con_ptr,
+                                 const int* row, const int* col, const double* val,
+                                 const double* X, int64_t ldx, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncon) return;
+  double s = 0.0;
+  for (int e = con_ptr[t]; e < con_ptr[t + 1]; ++e) s += val[e] * X[row[e] + (int64_t)col[e] * ldx];
+  out[con_list[t]] += s;
+}
+
+__global__ __launch_bounds__(256) void jprod_dense_kernel(int64_t n2, const int* dcon,
+                                                          const double* mats, const double* X,
+                                                          double* out) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  const double* Aj = mats + j * n2;
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < n2; e += 256) s += Aj[e] * X[e];
+  s = block_reduce<0>(s, red);
+  if (threadIdx.x == 0) out[dcon[j]] += s;
+}
+
+int jprod(hipStream_t st, const BlockOps& b, const double* X, double* out) {
+  if (b.kind == BlockOps::EMPTY) return 0;
+  if (b.kind == BlockOps::COO) {
+    jprod_coo_kernel<<<cdiv(b.ncon, 256), 256, 0, st>>>(b.ncon, b.con_list, b.con_ptr, b.row,
+                                                         b.col, b.val, X, b.n, out);
+  } else {
+    jprod_dense_kernel<<<b.ncon, 256, 0, st>>>((int64_t)b.n * b.n, b.con_list, b.mats, X, out);
+  }
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- A*(y) ---
+__global__ void jtprod_coo_kernel(int npos, const int64_t* pos_lin, const int* pos_ptr,
+                                  const int* pos_ent, const int* con, const double* val,
+                                  const double* y, double* out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npos) return;
+  double s = 0.0;
+  for (int k = pos_ptr[p]; k < pos_ptr[p + 1]; ++k) {
+    const int e = pos_ent[k];
+    s += y[con[e]] * val[e];
+  }
+  out[pos_lin[p]] = s;
+}
+
+__global__ void jtprod_dense_kernel(int64_t n2, int mi, const int* dcon, const double* mats,
+                                    const double* y, double* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n2) return;
+  double s = 0.0;
+  for (int j = 0; j < mi; ++j) s += y[dcon[j]] * mats[(int64_t)j * n2 + e];
+  out[e] = s;
+}
+
+int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out) {
+  const int64_t n2 = (int64_t)b.n * b.n;
+  if (b.kind == BlockOps::EMPTY) return fill(st, n2, 0.0, out);
+  if (b.kind == BlockOps::COO) {
+    MSDP_TRY(fill(st, n2, 0.0, out));
+    jtprod_coo_kernel<<<cdiv(b.npos, 256), 256, 0, st>>>(b.npos, b.pos_lin, b.pos_ptr, b.pos_ent,
+                                                          b.con, b.val, y, out);
+  } else {
+    jtprod_dense_kernel<<<cdiv(n2, 256), 256, 0, st>>>(n2, b.ncon, b.con_list, b.mats, y, out);
+  }
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------ Schur assembly ----
+// One-hot constraints A_j = v_j e_{r_j} e_{r_j}':  H[c_a, c_b] += v_a v_b L[r_a, r_b] R[r_b, r_a]
+__global__ void schur_onehot_kernel(int cnt, const int* con, const int* row, const double* val,
+                                    const double* L, int64_t ldl, const double* R, int64_t ldr,
+                                    double* H, int64_t ldh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (a >= cnt || a < b) return;
+  // L and R are symmetric (W, S^-1 or X), so R[r_b, r_a] is read as R[r_a, r_b]
+  // to keep the loads of a wavefront contiguous.
+  const int ra = row[a], rb = row[b];
+  H[con[a] + (int64_t)con[b] * ldh] += val[a] * val[b] * L[ra + (int64_t)rb * ldl] * R[ra + (int64_t)rb * ldr];
+}
+
+// General COO: H[j,k] += sum_{e in j, f in k} v_e v_f L[c_e, r_f] R[c_f, r_e]
+__global__ void schur_coo_kernel(int ncon, const int* con_list, const int* con_ptr,
+                                 const int* row, const int* col, const double* val,
+                                 const double* L, int64_t ldl, const double* R, int64_t ldr,
+                                 double* H, int64_t ldh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (a >= ncon || a < b) return;
+  double s = 0.0;
+  for (int e = con_ptr[a]; e < con_ptr[a + 1]; ++e)
+    for (int f = con_ptr[b]; f < con_ptr[b + 1]; ++f)
+      s += val[e] * val[f] * L[col[e] + (int64_t)row[f] * ldl] * R[col[f] + (int64_t)row[e] * ldr];
+  H[con_list[a] + (int64_t)con_list[b] * ldh] += s;
+}
+
+// svec with sqrt(2) off the diagonal, column-major upper triangle order:
+// index c(c+1)/2 + r for r <= c.
+__global__ void svec_pack_kernel(int n, int cb, const double* P, double* B, int64_t ldb) {
+  const int64_t d = (int64_t)n * (n + 1) / 2;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (k >= d || j >= cb) return;
+  // invert k -> (r, c)
+  int c = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(c + 1) * (c + 2) / 2 <= k) ++c;
+  while ((int64_t)c * (c + 1) / 2 > k) --c;
+  const int r = (int)(k - (int64_t)c * (c + 1) / 2);
+  const double v = P[(int64_t)j * n * n + r + (int64_t)c * n];
+  B[k + (int64_t)j * ldb] = r == c ? v : 1.4142135623730951 * v;
+}
+
+__global__ void scatter_lower_kernel(int mi, const int* dcon, const double* Hb, double* H,
+                                     int64_t ldh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (a >= mi || a < b) return;
+  H[dcon[a] + (int64_t)dcon[b] * ldh] += Hb[a + (int64_t)b * mi];
+}
+
+int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
+                double* work, size_t work_elems) {
+  const int n = b.n;
+  if (b.kind == BlockOps::EMPTY || b.ncon == 0) return 0;
+  if (b.kind == BlockOps::COO) {
+    dim3 g(cdiv(b.ncon, 128), b.ncon);
+    if (b.one_hot)
+      schur_onehot_kernel<<<g, 128, 0, st>>>(b.ncon, b.con, b.row, b.val, sc.L, n, sc.R, n, H, m);
+    else
+      schur_coo_kernel<<<g, 128, 0, st>>>(b.ncon, b.con_list, b.con_ptr, b.row, b.col, b.val,
+                                          sc.L, n, sc.R, n, H, m);
+    MSDP_HIP(hipGetLastError());
+    return 0;
+  }
+  // dense: P_j = op(Lm) (A_j op(F2)); B[:, j] = svec(P_j) or vec(P_j); Hb = B'B
+  const int64_t n2 = (int64_t)n * n;
+  const int64_t drow = sc.full_vec ? n2 : (int64_t)n * (n + 1) / 2;
+  const int mi = b.ncon;
+  double* Bm = b.Bbuf;  // drow x mi
+  // per-chunk buffers for T_j and P_j
+  const int64_t per = 2 * n2;
+  int cb = (int)std::max<int64_t>(1, std::min<int64_t>(mi, (int64_t)(work_elems / per)));
+  double* T = work;
+  double* P = work + (int64_t)cb * n2;
+  for (int j0 = 0; j0 < mi; j0 += cb) {
+    const int c = std::min(cb, mi - j0);
+    MSDP_TRY(gemm(st, false, sc.transF2, n, n, n, 1.0, b.mats + (int64_t)j0 * n2, n, sc.F2, n, 0.0,
+                  T, n, c, n2, 0, n2));
+    double* dst = sc.full_vec ? Bm + (int64_t)j0 * n2 : P;
+    MSDP_TRY(gemm(st, sc.transL, false, n, n, n, 1.0, sc.Lm, n, T, n, 0.0, dst, n, c, 0, n2, n2));
+    if (!sc.full_vec) {
+      dim3 g(cdiv(drow, 256), c);
+      svec_pack_kernel<<<g, 256, 0, st>>>(n, c, P, Bm + (int64_t)j0 * drow, drow);
+      MSDP_HIP(hipGetLastError());
+    }
+  }
+  if (b.dcon_identity) {
+    MSDP_TRY(gemm(st, true, false, mi, mi, (int)drow, 1.0, Bm, drow, Bm, drow, 1.0, H, m, 1, 0, 0,
+                  0, true));
+  } else {
+    double* Hb = b.Hbuf;
+    MSDP_TRY(gemm(st, true, false, mi, mi, (int)drow, 1.0, Bm, drow, Bm, drow, 0.0, Hb, mi, 1, 0,
+                  0, 0, true));
+    dim3 g(cdiv(mi, 128), mi);
+    scatter_lower_kernel<<<g, 128, 0, st>>>(mi, b.con_list, Hb, H, m);
+    MSDP_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// ------------------------------------------------------ scalar block ------
+__global__ void scalar_kernel(int op, int ns, double mu, const double* x, const double* z,
+                              const double* zinv, const double* a, const double* b,
+                              double* out, double* out2, int corrected) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  switch (op) {
+    case SOP_ZINV: out[k] = 1.0 / z[k]; break;
+    case SOP_RHS: {  // v = x - mu zinv + x zinv Rd_lin (- corr_lin)    a = Rd_lin, b = corr_lin
+      double v = x[k] - mu * zinv[k] + x[k] * zinv[k] * a[k];
+      if (corrected) v -= b[k];
+      out[k] = v;
+      break;
+    }
+    case SOP_DIRS: {  // out = delz = Rd_lin - CtDy (a = Rd_lin, b = CtDy) ; out2 = delx
+      const double dz = a[k] - b[k];
+      out[k] = dz;
+      double dx = mu * zinv[k] - x[k] - x[k] * zinv[k] * dz;
+      if (corrected) dx += out2[k];  // out2 holds corr_lin on entry
+      out2[k] = dx;
+      break;
+    }
+    case SOP_CORR:  // out = -delx delz zinv   (a = delx, b = delz)
+      out[k] = -a[k] * b[k] * zinv[k];
+      break;
+    case SOP_XZ:  // out = x .* zinv (Schur scalar scaling)
+      out[k] = x[k] * zinv[k];
+      break;
+  }
+}
+
+int scalar_op(hipStream_t st, int op, int ns, double mu, const double* x, const double* z,
+              const double* zinv, const double* a, const double* b, double* out, double* out2,
+              bool corrected) {
+  if (ns <= 0) return 0;
+  scalar_kernel<<<cdiv(ns, 256), 256, 0, st>>>(op, ns, mu, x, z, zinv, a, b, out, out2,
+                                                corrected ? 1 : 0);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// min over dx < 0 of -x/dx (+inf if none), and sum (x + a dx)(z + b dz)
+__global__ __launch_bounds__(256) void scalar_reduce_kernel(int op, int ns, const double* x,
+                                                            const double* dx, const double* z,
+                                                            const double* dz, double a,
+                                                            double bb, double* out) {
+  __shared__ double red[4];
+  if (op == 0) {
+    double v = INFINITY;
+    for (int k = threadIdx.x; k < ns; k += 256)
+      if (dx[k] < 0) v = fmin(v, -x[k] / dx[k]);
+    v = block_reduce<2>(v, red);
+    if (threadIdx.x == 0) *out = v;
+  } else {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < ns; k += 256) v += (x[k] + a * dx[k]) * (z[k] + bb * dz[k]);
+    v = block_reduce<0>(v, red);
+    if (threadIdx.x == 0) *out = v;
+  }
+}
+
+int scalar_min_step(hipStream_t st, int ns, const double* x, const double* dx, double* out) {
+  scalar_reduce_kernel<<<1, 256, 0, st>>>(0, ns, x, dx, nullptr, nullptr, 0, 0, out);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+int scalar_affine(hipStream_t st, int ns, const double* x, const double* dx, const double* z,
+                  const double* dz, double a, double b, double* out) {
+  scalar_reduce_kernel<<<1, 256, 0, st>>>(1, ns, x, dx, z, dz, a, b, out);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------- NT correction / E ------
+// RNT_pq = -(M_pq + M_qp) / (D_p + D_q)   (src/directions.jl:97-99)
+__global__ void nt_rnt_kernel(int n, const double* M, const double* D, double* R) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+  if (p >= n) return;
+  R[p + (int64_t)q * n] = -(M[p + (int64_t)q * n] + M[q + (int64_t)p * n]) / (D[p] + D[q]);
+}
+
+int nt_rnt(hipStream_t st, int n, const double* M, const double* D, double* R) {
+  dim3 g(cdiv(n, 256), n);
+  nt_rnt_kernel<<<g, 256, 0, st>>>(n, M, D, R);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// D = sqrt(lambda); *minlam for the positivity check
+__global__ __launch_bounds__(256) void sqrt_min_kernel(int n, const double* lam, double* D,
+                                                       double* minlam) {
+  __shared__ double red[4];
+  double v = INFINITY;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    v = fmin(v, lam[k]);
+    D[k] = sqrt(lam[k]);
+  }
+  v = block_reduce<2>(v, red);
+  if (threadIdx.x == 0) *minlam = v;
+}
+
+int sqrt_min(hipStream_t st, int n, const double* lam, double* D, double* minlam) {
+  sqrt_min_kernel<<<1, 256, 0, st>>>(n, lam, D, minlam);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// out = W diag(a) : column scaling by the diagonal of M (one-hot A*(y))
+__global__ void scale_by_diag_kernel(int n, const double* W, const double* M, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  out[i + (int64_t)j * n] = W[i + (int64_t)j * n] * M[j + (int64_t)j * n];
+}
+
+int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, double* out) {
+  dim3 g(cdiv(n, 256), n);
+  scale_by_diag_kernel<<<g, 256, 0, st>>>(n, W, M, out);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------- triangular solves ------
+// Solve L y = b (trans = 0) or L' y = b (trans = 1) for one diagonal block
+// [k0, k0+kb) in place on b; one workgroup of 64 threads.
+__global__ __launch_bounds__(64) void trsv_diag_kernel(int kb, const double* L, int64_t ld,
+                                                       int k0, double* b, int trans) {
+  __shared__ double Ls[64][65];
+  __shared__ double x[64];
+  const int t = threadIdx.x;
+  for (int e = t; e < kb * kb; e += 64) {
+    const int i = e % kb, j = e / kb;
+    Ls[i][j] = i >= j ? L[(k0 + i) + (int64_t)(k0 + j) * ld] : 0.0;
+  }
+  if (t < kb) x[t] = b[k0 + t];
+  __syncthreads();
+  if (!trans) {
+    for (int i = 0; i < kb; ++i) {
+      if (t == 0) x[i] /= Ls[i][i];
+      __syncthreads();
+      if (t > i && t < kb) x[t] -= Ls[t][i] * x[i];
+      __syncthreads();
+    }
+  } else {
+    for (int i = kb - 1; i >= 0; --i) {
+      if (t == 0) x[i] /= Ls[i][i];
+      __syncthreads();
+      if (t < i) x[t] -= Ls[i][t] * x[i];
+      __syncthreads();
+    }
+  }
+  if (t < kb) b[k0 + t] = x[t];
+}
+
+int potrs(hipStream_t st, int m, const double* L, int64_t ld, double* b) {
+  // forward: L y = b
+  for (int k0 = 0; k0 < m; k0 += 64) {
+    const int kb = std::min(64, m - k0);
+    trsv_diag_kernel<<<1, 64, 0, st>>>(kb, L, ld, k0, b, 0);
+    const int rest = m - k0 - kb;
+    if (rest > 0)
+      MSDP_TRY(gemm(st, false, false, rest, 1, kb, -1.0, L + (k0 + kb) + (int64_t)k0 * ld, ld,
+                    b + k0, m, 1.0, b + k0 + kb, m));
+  }
+  // backward: L' x = y
+  const int nblk = (m + 63) / 64;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int k0 = bi * 64;
+    const int kb = std::min(64, m - k0);
+    trsv_diag_kernel<<<1, 64, 0, st>>>(kb, L, ld, k0, b, 1);
+    if (k0 > 0)
+      MSDP_TRY(gemm(st, true, false, k0, 1, kb, -1.0, L + k0, ld, b + k0, m, 1.0, b, m));
+  }
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace msdp
+
+namespace msdp {
+
+// A[:, j] *= d[j]
+__global__ void scale_cols_vec_kernel(int rows, double* A, int64_t lda, const double* d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < rows) A[i + (int64_t)j * lda] *= d[j];
+}
+
+int scale_cols_vec(hipStream_t st, int rows, int cols, double* A, int64_t lda, const double* d) {
+  if (rows <= 0 || cols <= 0) return 0;
+  dim3 g(cdiv(rows, 256), cols);
+  scale_cols_vec_kernel<<<g, 256, 0, st>>>(rows, A, lda, d);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void vec_min_kernel(int n, const double* v, double* out) {
+  __shared__ double red[4];
+  double r = INFINITY;
+  for (int k = threadIdx.x; k < n; k += 256) r = fmin(r, v[k]);
+  r = block_reduce<2>(r, red);
+  if (threadIdx.x == 0) *out = r;
+}
+
+int vec_min(hipStream_t st, int n, const double* v, double* out) {
+  vec_min_kernel<<<1, 256, 0, st>>>(n, v, out);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// H[i, j] += L[i, j] R[i, j] for i >= j (one-hot constraints A_j = e_j e_j',
+// symmetric L, R): the Hadamard Schur complement of max-cut.
+__global__ void schur_onehot_identity_kernel(int m, const double* L, int64_t ldl, const double* R,
+                                             int64_t ldr, double* H, int64_t ldh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= m || i < j) return;
+  H[i + (int64_t)j * ldh] += L[i + (int64_t)j * ldl] * R[i + (int64_t)j * ldr];
+}
+
+int schur_onehot_identity(hipStream_t st, int m, const double* L, int64_t ldl, const double* R,
+                          int64_t ldr, double* H, int64_t ldh) {
+  if (m <= 0) return 0;
+  dim3 g(cdiv(m, 256), m);
+  schur_onehot_identity_kernel<<<g, 256, 0, st>>>(m, L, ldl, R, ldr, H, ldh);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// DualCentering step length (src/dual_centering.jl:146-201): alpha_max from
+// the proximity eigenvalues, then bisection on xi(alpha) = A; one workgroup,
+// every xi evaluation a block reduction.
+__global__ __launch_bounds__(1024) void dc_alpha_kernel(int total, const double* mu, double A,
+                                                        double* out) {
+  __shared__ double red[16];
+  const double eps = 2.220446049250313e-16;
+  double am = 1.0;
+  for (int k = threadIdx.x; k < total; k += blockDim.x) {
+    const double v = mu[k];
+    if (v > 0) am = fmin(am, 1.0 / v);
+    if (v < 0) am = fmin(am, 1.0 / (1.0 - v));
+  }
+  am = block_reduce<2>(am, red);
+  auto xi = [&](double a) {
+    const double b2 = a / (1 - a);
+    double s = 0.0, bad = 0.0;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+      const double u = 1 - a * mu[k], v = 1 + b2 * mu[k];
+      if (u > 0 && v > 0) s -= log(u) + log(v);
+      else bad = 1.0;
+    }
+    s = block_reduce<0>(s, red);
+    bad = block_reduce<1>(bad, red);
+    return bad > 0 ? INFINITY : s;
+  };
+  double hi = am * 0.99999;
+  const double xh = xi(hi);
+  double res;
+  if (!isfinite(xh) || xh < A) {
+    res = hi;
+  } else {
+    double lo = 0.0;
+    for (int it = 0; it < 80; ++it) {
+      const double mid = (lo + hi) / 2;
+      if (xi(mid) < A) lo = mid; else hi = mid;
+      if (hi - lo <= eps * fmax(1.0, hi)) break;
+    }
+    res = (lo + hi) / 2;
+  }
+  if (threadIdx.x == 0) *out = res;
+}
+
+int dc_solve_alpha(hipStream_t st, int total, const double* mu, double A, double* out) {
+  dc_alpha_kernel<<<1, 1024, 0, st>>>(total, mu, A, out);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace msdp

@@ -1,0 +1,1321 @@
+// MadSDP interior-point driver on one gfx950 device.
+//
+// The state of `MadSDPSolver` (reference src/solver.jl:7-49) lives in device
+// memory; this file sequences the device kernels exactly as the reference's
+// driver and components do (src/solver.jl, src/primal_dual.jl, src/step.jl,
+// src/directions.jl, src/blocks.jl, src/schur_solvers.jl,
+// src/dual_centering.jl).  Only scalars cross to the host: the step lengths,
+// mu, the termination quantities and the factorisation statuses, i.e. the
+// values the reference's control flow branches on.
+#include "ops.h"
+#include "madsdp_hip.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+using namespace msdp;
+
+namespace {
+
+constexpr double EPS = 2.220446049250313e-16;
+
+enum Status {
+  SOLVE_SUCCEEDED = 1,
+  SEARCH_DIRECTION_BECOMES_TOO_SMALL = 3,
+  MAXIMUM_ITERATIONS_EXCEEDED = 6,
+  MAXIMUM_WALLTIME_EXCEEDED = 7,
+  INITIAL = 11,
+  REGULAR = 12,
+  ERROR_IN_STEP_COMPUTATION = -3,
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Per-PSD-block device state: SDPBlock + PrimalDualBlock + NTWorkspace
+// (src/blocks.jl:10-63, src/directions.jl:16-43) and the DualCentering block
+// workspace (src/dual_centering.jl:48-55).
+struct Blk {
+  int n = 0;
+  BlockOps ops;
+  double* C = nullptr;
+  double cnorm = 0.0;
+  double *X, *S, *Rd, *Ls, *Si, *Lsinv, *delX, *delS, *Lx, *Lxinv, *corr, *buf, *buf2, *tmp,
+      *tmp2, *W, *G, *Gi, *D, *RNT, *Mm, *Nm, *Vv, *lam, *Shat, *Dc, *Lh;
+  bool rd_nonzero = true;
+  // host copies of the constraint data until finalize
+  std::vector<int> h_con, h_row, h_col;
+  std::vector<double> h_val;
+  std::vector<int> h_dcon;
+  std::vector<double> h_mats;
+  bool has_coo = false, has_dense = false;
+};
+
+}  // namespace
+
+struct msdp_solver {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int nb = 0;
+  int64_t m = 0, ns = 0;
+  std::vector<Blk> blk;
+  msdp_options opt;
+  bool finalized = false;
+  // device vectors
+  double *b = nullptr, *dlin = nullptr, *Clin = nullptr, *y = nullptr, *dely = nullptr,
+         *Rp = nullptr, *x = nullptr, *z = nullptr, *delx = nullptr, *delz = nullptr,
+         *Rdlin = nullptr, *zinv = nullptr, *corrlin = nullptr, *wm = nullptr, *ws1 = nullptr,
+         *ws2 = nullptr, *Cs = nullptr;
+  double *H = nullptr, *Hs = nullptr;
+  int* info = nullptr;    // per-block factorisation infos (4 per block) + 8
+  double* sc = nullptr;   // scalar slots
+  double* red = nullptr;  // reduction scratch
+  double* ework = nullptr;
+  int* eiwork = nullptr;
+  double* trwork = nullptr;
+  double* swork = nullptr;
+  size_t swork_elems = 0;
+  double* eigall = nullptr;  // DualCentering proximity eigenvalues, all blocks
+  // DualCentering vectors
+  double *dc_d = nullptr, *dc_g = nullptr, *dc_dy = nullptr, *dc_yhat = nullptr;
+  // host scalars (src/solver.jl:37-48)
+  double mu = 0, sigma = 0, alpha = 0, beta = 0, inf_pr = 0, inf_du = 0, inf_compl = 0,
+         obj_val = 0, pobj = 0, dobj = 0;
+  int iter = 0, status = INITIAL;
+  double start_time = 0;
+  double bnorm = 0, cnorm_max = 0, dnorm = 0;
+  int dc_nu = 0, n_newton = 0, n_predictor = 0;
+  double dc_A = 0, dc_t = 0, dc_lambda = 0;
+  std::vector<double> ip_X, ip_y, ip_x;
+  std::vector<void*> allocs;
+  std::vector<std::array<double, 9>> trace;
+  int nslots = 0;
+  std::vector<double> hsc;
+
+  ~msdp_solver() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  template <class T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    MSDP_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    allocs.push_back(q);
+    *p = reinterpret_cast<T*>(q);
+    return 0;
+  }
+
+  double* slot(int k) { return sc + k; }
+  // slots: 0..31 global, 32 + 8*i per block
+  int bslot(int i, int k) const { return 32 + 8 * i + k; }
+  int read_slots() {
+    hsc.resize(nslots);
+    MSDP_HIP(hipMemcpyAsync(hsc.data(), sc, sizeof(double) * nslots, hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int read_infos(std::vector<int>& out) {
+    out.resize(4 * nb + 8);
+    MSDP_HIP(hipMemcpyAsync(out.data(), info, sizeof(int) * out.size(), hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int* binfo(int i, int k) { return info + 8 + 4 * i + k; }
+
+  bool nt() const { return opt.direction == MSDP_DIR_NT; }
+  bool dual_centering() const { return opt.algorithm == MSDP_ALG_DUAL_CENTERING; }
+  int barrier_dim() const {
+    int64_t s = ns;
+    for (const auto& b : blk) s += b.n;
+    return (int)s;
+  }
+
+  int finalize();
+  // block kernels
+  int cholesky_factor(Blk& b, const double* M, double* L, int* inf);
+  int prepare_block(Blk& b, int i);
+  int set_dual_slacks(const double* yv);
+  int slack(Blk& b, const double* yv, double* out);
+  // PrimalDual
+  int update_residuals();
+  int prepare(bool& ok);
+  int factorize_schur(bool& ok);
+  int apply_E(Blk& b, const double* M, bool diag, double* out);
+  int solve_directions(double mu_, bool corrected);
+  int correction();
+  int step_lengths(double& a, double& bt);
+  int affine_complementarity(double a, double bt, double& out);
+  int take_step(double a, double bt);
+  int total_complementarity(double& out);
+  int iterate();
+  int update_termination();
+  int initialize();
+  int initial_point();
+  int solve();
+  // DualCentering
+  int refresh_slacks(bool& ok);
+  int newton_direction(bool& ok);
+  int dc_initialize();
+  int dc_iterate();
+  int predictor_step(bool& ok);
+};
+
+static int as_err(const char* msg) {
+  set_error(msg);
+  return -2;
+}
+
+// ==========================================================================
+// set-up
+// ==========================================================================
+int msdp_solver::finalize() {
+  if (finalized) return 0;
+  MSDP_HIP(hipSetDevice(device));
+  int nmax = 1;
+  for (auto& b : blk) nmax = std::max(nmax, b.n);
+  const size_t nm2 = (size_t)nmax * nmax;
+  for (int i = 0; i < nb; ++i) {
+    Blk& b = blk[i];
+    const size_t n2 = (size_t)b.n * b.n;
+    double** mats[] = {&b.X, &b.S, &b.Rd, &b.Ls, &b.Si, &b.Lsinv, &b.delX, &b.delS, &b.Lx,
+                       &b.Lxinv, &b.corr, &b.buf, &b.buf2, &b.tmp, &b.tmp2, &b.W, &b.G, &b.Gi,
+                       &b.RNT, &b.Mm, &b.Nm, &b.Vv, &b.Shat, &b.Dc, &b.Lh};
+    for (double** p : mats) {
+      MSDP_TRY(alloc(p, n2));
+      MSDP_HIP(hipMemsetAsync(*p, 0, sizeof(double) * n2, st));
+    }
+    MSDP_TRY(alloc(&b.D, b.n));
+    MSDP_TRY(alloc(&b.lam, b.n));
+    if (!b.C) {
+      MSDP_TRY(alloc(&b.C, n2));
+      MSDP_HIP(hipMemsetAsync(b.C, 0, sizeof(double) * n2, st));
+    }
+    // constraints
+    BlockOps& o = b.ops;
+    o.n = b.n;
+    if (b.has_dense) {
+      const int mi = (int)b.h_dcon.size();
+      o.kind = mi > 0 ? BlockOps::DENSE : BlockOps::EMPTY;
+      o.ncon = mi;
+      MSDP_TRY(alloc(&o.con_list, mi));
+      MSDP_TRY(alloc(&o.mats, (size_t)mi * n2));
+      MSDP_HIP(hipMemcpy(o.con_list, b.h_dcon.data(), sizeof(int) * mi, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.mats, b.h_mats.data(), sizeof(double) * mi * n2, hipMemcpyHostToDevice));
+      bool ident = mi == m;
+      for (int j = 0; ident && j < mi; ++j) ident = b.h_dcon[j] == j;
+      o.dcon_identity = ident;
+      const size_t drow = n2;  // vec (HKM) is the larger of vec / svec
+      MSDP_TRY(alloc(&o.Bbuf, drow * mi));
+      if (!ident) MSDP_TRY(alloc(&o.Hbuf, (size_t)mi * mi));
+      std::vector<double>().swap(b.h_mats);
+    } else if (b.has_coo && !b.h_con.empty()) {
+      // sort entries by constraint (stable), then build the ranges
+      const int64_t nnz = (int64_t)b.h_con.size();
+      std::vector<int64_t> perm(nnz);
+      for (int64_t e = 0; e < nnz; ++e) perm[e] = e;
+      std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t c) { return b.h_con[a] < b.h_con[c]; });
+      std::vector<int> con(nnz), row(nnz), col(nnz);
+      std::vector<double> val(nnz);
+      for (int64_t e = 0; e < nnz; ++e) {
+        con[e] = b.h_con[perm[e]]; row[e] = b.h_row[perm[e]]; col[e] = b.h_col[perm[e]]; val[e] = b.h_val[perm[e]];
+      }
+      std::vector<int> clist, cptr;
+      for (int64_t e = 0; e < nnz; ++e) {
+        if (clist.empty() || clist.back() != con[e]) { clist.push_back(con[e]); cptr.push_back((int)e); }
+      }
+      cptr.push_back((int)nnz);
+      bool onehot = (int64_t)clist.size() == nnz;
+      for (int64_t e = 0; onehot && e < nnz; ++e) onehot = row[e] == col[e];
+      // positions for A*(y)
+      std::vector<int64_t> lin(nnz), pp(nnz);
+      for (int64_t e = 0; e < nnz; ++e) { lin[e] = (int64_t)row[e] + (int64_t)col[e] * b.n; pp[e] = e; }
+      std::stable_sort(pp.begin(), pp.end(), [&](int64_t a, int64_t c) { return lin[a] < lin[c]; });
+      std::vector<int64_t> plin;
+      std::vector<int> pptr, pent(nnz);
+      for (int64_t k = 0; k < nnz; ++k) {
+        const int64_t e = pp[k];
+        pent[k] = (int)e;
+        if (plin.empty() || plin.back() != lin[e]) { plin.push_back(lin[e]); pptr.push_back((int)k); }
+      }
+      pptr.push_back((int)nnz);
+      o.kind = BlockOps::COO;
+      o.one_hot = onehot;
+      o.nnz = nnz;
+      o.ncon = onehot ? (int)nnz : (int)clist.size();
+      o.npos = (int)plin.size();
+      MSDP_TRY(alloc(&o.con, nnz));
+      MSDP_TRY(alloc(&o.row, nnz));
+      MSDP_TRY(alloc(&o.col, nnz));
+      MSDP_TRY(alloc(&o.val, nnz));
+      MSDP_TRY(alloc(&o.con_list, clist.size()));
+      MSDP_TRY(alloc(&o.con_ptr, cptr.size()));
+      MSDP_TRY(alloc(&o.pos_lin, plin.size()));
+      MSDP_TRY(alloc(&o.pos_ptr, pptr.size()));
+      MSDP_TRY(alloc(&o.pos_ent, pent.size()));
+      MSDP_HIP(hipMemcpy(o.con, con.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.row, row.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.con_list, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.con_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.pos_lin, plin.data(), sizeof(int64_t) * plin.size(), hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.pos_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(o.pos_ent, pent.data(), sizeof(int) * pent.size(), hipMemcpyHostToDevice));
+    } else {
+      o.kind = BlockOps::EMPTY;
+    }
+  }
+  // vectors
+  const size_t mm = std::max<int64_t>(m, 1), nss = std::max<int64_t>(ns, 1);
+  double** mv[] = {&y, &dely, &Rp, &wm, &dc_d, &dc_g, &dc_dy, &dc_yhat};
+  for (double** p : mv) { MSDP_TRY(alloc(p, mm)); MSDP_HIP(hipMemsetAsync(*p, 0, sizeof(double) * mm, st)); }
+  if (!b) { MSDP_TRY(alloc(&b, mm)); MSDP_HIP(hipMemsetAsync(b, 0, sizeof(double) * mm, st)); }
+  double** sv[] = {&x, &z, &delx, &delz, &Rdlin, &zinv, &corrlin, &ws1, &ws2};
+  for (double** p : sv) { MSDP_TRY(alloc(p, nss)); MSDP_HIP(hipMemsetAsync(*p, 0, sizeof(double) * nss, st)); }
+  if (!dlin) { MSDP_TRY(alloc(&dlin, nss)); MSDP_HIP(hipMemsetAsync(dlin, 0, sizeof(double) * nss, st)); }
+  if (!Clin) { MSDP_TRY(alloc(&Clin, mm * nss)); MSDP_HIP(hipMemsetAsync(Clin, 0, sizeof(double) * mm * nss, st)); }
+  MSDP_TRY(alloc(&Cs, mm * nss));
+  MSDP_TRY(alloc(&H, mm * mm));
+  MSDP_TRY(alloc(&Hs, mm * mm));
+  MSDP_TRY(alloc(&info, 4 * nb + 8));
+  nslots = 32 + 8 * nb;
+  MSDP_TRY(alloc(&sc, nslots));
+  MSDP_TRY(alloc(&red, reduce_scratch_elems(0) + 64));
+  MSDP_TRY(alloc(&ework, syevd_work_elems(nmax)));
+  MSDP_TRY(alloc(&eiwork, syevd_iwork_elems(nmax)));
+  MSDP_TRY(alloc(&trwork, std::max(trtri_work_elems(nmax), (size_t)m + 64)));
+  int64_t tot = 0;
+  for (auto& bb : blk) tot += bb.n;
+  MSDP_TRY(alloc(&eigall, std::max<int64_t>(tot, 1)));
+  // chunk buffers of the dense Schur assembly: up to 64 constraints or 256 MiB
+  size_t want = 0;
+  for (auto& bb : blk)
+    if (bb.ops.kind == BlockOps::DENSE)
+      want = std::max(want, 2 * (size_t)bb.n * bb.n * (size_t)std::min(bb.ops.ncon, 64));
+  want = std::min(want, (size_t)(256u << 20) / 8);
+  for (auto& bb : blk)
+    if (bb.ops.kind == BlockOps::DENSE) want = std::max(want, 2 * (size_t)bb.n * bb.n);
+  swork_elems = want;
+  if (want) MSDP_TRY(alloc(&swork, want));
+  // norms of the data (src/solver.jl:119-120, 217-221, 322, 330)
+  for (int i = 0; i < nb; ++i) MSDP_TRY(max_abs(st, (int64_t)blk[i].n * blk[i].n, blk[i].C, slot(bslot(i, 0)), red));
+  if (m > 0) MSDP_TRY(max_abs(st, m, b, slot(0), red));
+  else MSDP_TRY(fill(st, 1, 0.0, slot(0)));
+  if (ns > 0) MSDP_TRY(max_abs(st, ns, dlin, slot(1), red));
+  else MSDP_TRY(fill(st, 1, 0.0, slot(1)));
+  MSDP_TRY(read_slots());
+  bnorm = hsc[0];
+  dnorm = hsc[1];
+  cnorm_max = 0;
+  for (int i = 0; i < nb; ++i) {
+    blk[i].cnorm = hsc[bslot(i, 0)];
+    cnorm_max = std::max(cnorm_max, blk[i].cnorm);
+  }
+  finalized = true;
+  return 0;
+}
+
+// ==========================================================================
+// block kernels (src/blocks.jl)
+// ==========================================================================
+// _cholesky_factor! (blocks.jl:94-113)
+int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) {
+  const int n = bk.n;
+  MSDP_TRY(copy_mat(st, n, n, M, n, L, n));
+  MSDP_TRY(symmetrize(st, n, L, n));
+  MSDP_TRY(potrf(st, n, L, n, inf, nullptr));
+  MSDP_TRY(chol_flush(st, n, L, n, red));
+  return 0;
+}
+
+// prepare_block! (blocks.jl:126-130): Ls = chol(S), Si = S^-1 (potri)
+int msdp_solver::prepare_block(Blk& bk, int i) {
+  const int n = bk.n;
+  MSDP_TRY(cholesky_factor(bk, bk.S, bk.Ls, binfo(i, 0)));
+  MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));
+  MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Lsinv, n, bk.Lsinv, n, 0.0, bk.Si, n));
+  MSDP_TRY(mirror_lower(st, n, bk.Si, n));
+  return 0;
+}
+
+// out = C_i - A*_i(y), symmetrised (solver.jl:193-198)
+int msdp_solver::slack(Blk& bk, const double* yv, double* out) {
+  const int64_t n2 = (int64_t)bk.n * bk.n;
+  MSDP_TRY(jtprod(st, bk.ops, yv, bk.tmp2));
+  MSDP_TRY(axpby(st, n2, 1.0, bk.C, -1.0, bk.tmp2, out));
+  MSDP_TRY(symmetrize(st, bk.n, out, bk.n));
+  return 0;
+}
+
+// _set_dual_slacks! (solver.jl:201-211)
+int msdp_solver::set_dual_slacks(const double* yv) {
+  for (auto& bk : blk) MSDP_TRY(slack(bk, yv, bk.S));
+  if (ns > 0) {
+    MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, yv, m, 0.0, ws1, ns));
+    MSDP_TRY(axpby(st, ns, 1.0, dlin, -1.0, ws1, z));
+  }
+  return 0;
+}
+
+// ==========================================================================
+// PrimalDual components
+// ==========================================================================
+// update_residuals! (step.jl:147-182)
+int msdp_solver::update_residuals() {
+  if (m > 0) {
+    MSDP_TRY(fill(st, m, 0.0, Rp));
+    for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.X, Rp));
+    if (ns > 0) MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, x, ns, 1.0, Rp, m));
+    MSDP_TRY(axpby(st, m, 1.0, b, -1.0, Rp, Rp));
+  }
+  for (int i = 0; i < nb; ++i) {
+    Blk& bk = blk[i];
+    const int64_t n2 = (int64_t)bk.n * bk.n;
+    MSDP_TRY(jtprod(st, bk.ops, y, bk.tmp2));
+    MSDP_TRY(axpbypcz(st, n2, 1.0, bk.C, -1.0, bk.S, -1.0, bk.tmp2, bk.Rd));
+    MSDP_TRY(max_abs(st, n2, bk.Rd, slot(bslot(i, 1)), red));
+  }
+  if (ns > 0) {
+    MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, y, m, 0.0, ws1, ns));
+    MSDP_TRY(axpbypcz(st, ns, 1.0, dlin, -1.0, z, -1.0, ws1, Rdlin));
+  }
+  MSDP_TRY(read_slots());
+  for (int i = 0; i < nb; ++i) {
+    const double thr = 100 * EPS * std::max(1.0, blk[i].cnorm);
+    blk[i].rd_nonzero = hsc[bslot(i, 1)] > thr;
+  }
+  return 0;
+}
+
+// _prepare! (primal_dual.jl:29-37): prepare_block!, prepare_direction!,
+// prepare_step! for every block, then zinv.
+int msdp_solver::prepare(bool& ok) {
+  MSDP_TRY(fill(st, 1, 0.0, slot(2)));
+  MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
+  for (int i = 0; i < nb; ++i) {
+    Blk& bk = blk[i];
+    const int n = bk.n;
+    MSDP_TRY(prepare_block(bk, i));
+    const bool need_lx = nt() || opt.step == MSDP_STEP_EIGEN;
+    if (need_lx) {
+      MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
+      MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));
+    }
+    if (nt()) {  // prepare_direction!(::NTWorkspace) directions.jl:45-75
+      MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n));
+      MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Mm, n, bk.Mm, n, 0.0, bk.Nm, n));
+      MSDP_TRY(symmetrize(st, n, bk.Nm, n));
+      MSDP_TRY(syevd(st, n, bk.Nm, n, bk.lam, bk.Vv, n, true, ework, eiwork));
+      MSDP_TRY(sqrt_min(st, n, bk.lam, bk.D, slot(bslot(i, 2))));
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Vv, n, 0.0, bk.G, n));
+      MSDP_TRY(scale_cols(st, n, n, bk.G, n, bk.D, -1));
+      MSDP_TRY(transpose(st, n, n, bk.Vv, n, bk.tmp, n));
+      MSDP_TRY(scale_rows(st, n, n, bk.tmp, n, bk.D, +1));
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, bk.Lxinv, n, 0.0, bk.Gi, n));
+      MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.G, n, bk.G, n, 0.0, bk.W, n));
+      MSDP_TRY(symmetrize(st, n, bk.W, n));
+    }
+  }
+  if (ns > 0) MSDP_TRY(scalar_op(st, SOP_ZINV, (int)ns, 0, nullptr, z, nullptr, nullptr, nullptr, zinv, nullptr, false));
+  std::vector<int> inf;
+  MSDP_TRY(read_infos(inf));
+  MSDP_TRY(read_slots());
+  ok = true;
+  for (int i = 0; i < nb && ok; ++i) {
+    if (inf[8 + 4 * i] != 0 || inf[8 + 4 * i + 1] != 0) ok = false;
+    if (ok && nt() && !(hsc[bslot(i, 2)] > 0.0)) ok = false;  // directions.jl:55-58
+  }
+  return 0;
+}
+
+// factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
+int msdp_solver::factorize_schur(bool& ok) {
+  ok = false;
+  if (m == 0) { ok = true; return 0; }
+  MSDP_TRY(fill(st, m * m, 0.0, Hs));
+  for (auto& bk : blk) {
+    SchurScaling sc;
+    const int n = bk.n;
+    if (dual_centering()) {
+      sc.L = bk.Si; sc.R = bk.Si;
+      sc.Lm = bk.Lsinv; sc.transL = false; sc.F2 = bk.Lsinv; sc.transF2 = true; sc.full_vec = false;
+    } else if (nt()) {
+      sc.L = bk.W; sc.R = bk.W;
+      sc.Lm = bk.G; sc.transL = true; sc.F2 = bk.G; sc.transF2 = false; sc.full_vec = false;
+    } else {
+      sc.L = bk.Si; sc.R = bk.X;
+      sc.Lm = bk.Lsinv; sc.transL = false; sc.F2 = bk.Lx; sc.transF2 = false; sc.full_vec = true;
+      if (bk.ops.kind == BlockOps::DENSE && opt.step != MSDP_STEP_EIGEN) {
+        // HKM needs chol(X) for the factored dense form
+        MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(0, 3)));
+      }
+    }
+    (void)n;
+    MSDP_TRY(schur_block(st, bk.ops, sc, Hs, m, swork, swork_elems));
+  }
+  if (ns > 0) {
+    // H += C_lin diag(x ./ z) C_lin'
+    MSDP_TRY(scalar_op(st, SOP_XZ, (int)ns, 0, x, nullptr, zinv, nullptr, nullptr, ws2, nullptr, false));
+    MSDP_TRY(copy_mat(st, (int)m, (int)ns, Clin, m, Cs, m));
+    MSDP_TRY(scale_cols_vec(st, (int)m, (int)ns, Cs, m, ws2));
+    MSDP_TRY(gemm(st, false, true, (int)m, (int)m, (int)ns, 1.0, Cs, m, Clin, m, 1.0, Hs, m, 1, 0, 0, 0, true));
+  }
+  double reg = opt.regularize_schur;
+  for (int t = 0; t < opt.max_schur_reg; ++t) {
+    MSDP_TRY(copy_mat(st, (int)m, (int)m, Hs, m, H, m));
+    if (reg > 0) MSDP_TRY(add_diag(st, (int)m, H, m, nullptr, reg));
+    MSDP_TRY(potrf(st, (int)m, H, m, info, nullptr));
+    int hinfo = 0;
+    MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipStreamSynchronize(st));
+    if (hinfo == 0) { ok = true; return 0; }
+    reg = reg == 0 ? 1e-12 : 10 * reg;
+  }
+  return 0;
+}
+
+// apply_E! (directions.jl:80-84 NT, 116-120 HKM); `diag` marks a diagonal M
+// (one-hot A*(y)) for which the first product is a column scaling.
+int msdp_solver::apply_E(Blk& bk, const double* M, bool diag, double* out) {
+  const int n = bk.n;
+  const double* P = nt() ? bk.W : bk.X;
+  const double* Q = nt() ? bk.W : bk.Si;
+  if (diag) MSDP_TRY(scale_by_diag(st, n, P, M, bk.tmp));
+  else MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, P, n, M, n, 0.0, bk.tmp, n));
+  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, Q, n, 0.0, out, n));
+  MSDP_TRY(symmetrize(st, n, out, n));
+  return 0;
+}
+
+// solve_directions! (primal_dual.jl:44-54) = rhs! (step.jl:198-224),
+// schur_solve! (schur_solvers.jl:21), directions! (step.jl:229-259)
+int msdp_solver::solve_directions(double mu_, bool corrected) {
+  // rhs
+  if (m > 0) {
+    MSDP_HIP(hipMemcpyAsync(dely, Rp, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    for (auto& bk : blk) {
+      const int64_t n2 = (int64_t)bk.n * bk.n;
+      MSDP_TRY(axpby(st, n2, 1.0, bk.X, -mu_, bk.Si, bk.buf));
+      if (bk.rd_nonzero) {
+        MSDP_TRY(apply_E(bk, bk.Rd, false, bk.buf2));
+        MSDP_TRY(axpby(st, n2, 1.0, bk.buf, 1.0, bk.buf2, bk.buf));
+      }
+      if (corrected) MSDP_TRY(axpby(st, n2, 1.0, bk.buf, -1.0, bk.corr, bk.buf));
+      MSDP_TRY(jprod(st, bk.ops, bk.buf, dely));
+    }
+    if (ns > 0) {
+      MSDP_TRY(scalar_op(st, SOP_RHS, (int)ns, mu_, x, z, zinv, Rdlin, corrlin, ws1, nullptr, corrected));
+      MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, ws1, ns, 1.0, dely, m));
+    }
+    MSDP_TRY(potrs(st, (int)m, H, m, dely));
+  }
+  // directions
+  for (auto& bk : blk) {
+    const int n = bk.n;
+    const int64_t n2 = (int64_t)n * n;
+    MSDP_TRY(jtprod(st, bk.ops, dely, bk.tmp2));
+    MSDP_TRY(axpby(st, n2, 1.0, bk.Rd, -1.0, bk.tmp2, bk.delS));
+    MSDP_TRY(symmetrize(st, n, bk.delS, n));
+    if (bk.ops.kind == BlockOps::EMPTY) MSDP_TRY(fill(st, n2, 0.0, bk.buf2));
+    else MSDP_TRY(apply_E(bk, bk.tmp2, bk.ops.one_hot, bk.buf2));
+    MSDP_TRY(axpbypcz(st, n2, mu_, bk.Si, -1.0, bk.X, 1.0, bk.buf2, bk.delX));
+    if (bk.rd_nonzero) {
+      MSDP_TRY(apply_E(bk, bk.Rd, false, bk.buf2));
+      MSDP_TRY(axpby(st, n2, 1.0, bk.delX, -1.0, bk.buf2, bk.delX));
+    }
+    if (corrected) MSDP_TRY(axpby(st, n2, 1.0, bk.delX, 1.0, bk.corr, bk.delX));
+    MSDP_TRY(symmetrize(st, n, bk.delX, n));
+  }
+  if (ns > 0) {
+    MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, dely, m, 0.0, ws1, ns));
+    if (corrected) MSDP_HIP(hipMemcpyAsync(delx, corrlin, sizeof(double) * ns, hipMemcpyDeviceToDevice, st));
+    MSDP_TRY(scalar_op(st, SOP_DIRS, (int)ns, mu_, x, z, zinv, Rdlin, ws1, delz, delx, corrected));
+  }
+  return 0;
+}
+
+// correction! (step.jl:263-271; directions.jl:90-100 NT, 123-127 HKM)
+int msdp_solver::correction() {
+  for (auto& bk : blk) {
+    const int n = bk.n;
+    if (nt()) {
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Gi, n, bk.delX, n, 0.0, bk.tmp, n));
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, bk.delS, n, 0.0, bk.RNT, n));
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.RNT, n, bk.G, n, 0.0, bk.Mm, n));
+      MSDP_TRY(nt_rnt(st, n, bk.Mm, bk.D, bk.RNT));
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.G, n, bk.RNT, n, 0.0, bk.tmp, n));
+      MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.G, n, 0.0, bk.corr, n));
+    } else {
+      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.delX, n, bk.delS, n, 0.0, bk.tmp, n));
+      MSDP_TRY(gemm(st, false, false, n, n, n, -1.0, bk.tmp, n, bk.Si, n, 0.0, bk.corr, n));
+    }
+    MSDP_TRY(symmetrize(st, n, bk.corr, n));
+  }
+  if (ns > 0) MSDP_TRY(scalar_op(st, SOP_CORR, (int)ns, 0, nullptr, nullptr, zinv, delx, delz, corrlin, nullptr, false));
+  return 0;
+}
+
+// step_lengths (step.jl:186-198 EigenStep, 200-228 CholeskyBacktrack)
+int msdp_solver::step_lengths(double& a, double& bt) {
+  if (opt.step == MSDP_STEP_EIGEN) {
+    for (int i = 0; i < nb; ++i) {
+      Blk& bk = blk[i];
+      const int n = bk.n;
+      const double* Ls[2] = {bk.Lxinv, bk.Lsinv};
+      const double* Dl[2] = {bk.delX, bk.delS};
+      for (int k = 0; k < 2; ++k) {
+        MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
+        MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, bk.tmp2, n));
+        MSDP_TRY(symmetrize(st, n, bk.tmp2, n));
+        MSDP_TRY(eig_min(st, n, bk.tmp2, n, slot(bslot(i, 3 + k)), ework, eiwork));
+      }
+    }
+    if (ns > 0) {
+      MSDP_TRY(scalar_min_step(st, (int)ns, x, delx, slot(3)));
+      MSDP_TRY(scalar_min_step(st, (int)ns, z, delz, slot(4)));
+    }
+    MSDP_TRY(read_slots());
+    const double tau = opt.tau;
+    auto step = [&](double lam) { return lam > -std::sqrt(EPS) ? 1.0 : std::min(1.0, -tau / lam); };
+    a = bt = 1.0;
+    for (int i = 0; i < nb; ++i) {
+      a = std::min(a, step(hsc[bslot(i, 3)]));
+      bt = std::min(bt, step(hsc[bslot(i, 4)]));
+    }
+    if (ns > 0) {
+      a = std::min(a, tau * hsc[3]);
+      bt = std::min(bt, tau * hsc[4]);
+    }
+    return 0;
+  }
+  // CholeskyBacktrack: _backtrack for X (alpha) and S (beta)
+  auto backtrack = [&](bool primal, double& out) -> int {
+    double al = 1.0;
+    for (int i = 0; i < nb; ++i) {
+      Blk& bk = blk[i];
+      const int n = bk.n;
+      const double* Xm = primal ? bk.X : bk.S;
+      const double* dX = primal ? bk.delX : bk.delS;
+      for (;;) {
+        MSDP_TRY(axpby(st, (int64_t)n * n, 1.0, Xm, al, dX, bk.buf));
+        MSDP_TRY(potrf(st, n, bk.buf, n, info, nullptr));
+        int hinfo = 0;
+        MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+        MSDP_HIP(hipStreamSynchronize(st));
+        if (hinfo == 0) break;
+        al *= opt.shrink;
+        if (al < EPS) { out = 0.0; return 0; }
+      }
+    }
+    if (al != 1.0) al *= opt.boundary;
+    if (ns > 0) {
+      MSDP_TRY(scalar_min_step(st, (int)ns, primal ? x : z, primal ? delx : delz, slot(3)));
+      MSDP_TRY(read_slots());
+      al = std::min(al, opt.boundary * hsc[3]);
+    }
+    out = al;
+    return 0;
+  };
+  MSDP_TRY(backtrack(true, a));
+  MSDP_TRY(backtrack(false, bt));
+  return 0;
+}
+
+// affine_complementarity (step.jl:231-243)
+int msdp_solver::affine_complementarity(double a, double bt, double& out) {
+  for (int i = 0; i < nb; ++i) {
+    Blk& bk = blk[i];
+    const int64_t n2 = (int64_t)bk.n * bk.n;
+    MSDP_TRY(dot(st, n2, bk.X, bk.S, slot(bslot(i, 3)), red));
+    MSDP_TRY(dot(st, n2, bk.X, bk.delS, slot(bslot(i, 4)), red));
+    MSDP_TRY(dot(st, n2, bk.delX, bk.S, slot(bslot(i, 5)), red));
+    MSDP_TRY(dot(st, n2, bk.delX, bk.delS, slot(bslot(i, 6)), red));
+  }
+  if (ns > 0) MSDP_TRY(scalar_affine(st, (int)ns, x, delx, z, delz, a, bt, slot(5)));
+  MSDP_TRY(read_slots());
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    s += hsc[bslot(i, 3)];
+    s += bt * hsc[bslot(i, 4)];
+    s += a * hsc[bslot(i, 5)];
+    s += a * bt * hsc[bslot(i, 6)];
+  }
+  if (ns > 0) s += hsc[5];
+  out = s / std::max(barrier_dim(), 1);
+  return 0;
+}
+
+int msdp_solver::total_complementarity(double& out) {
+  for (int i = 0; i < nb; ++i)
+    MSDP_TRY(dot(st, (int64_t)blk[i].n * blk[i].n, blk[i].X, blk[i].S, slot(bslot(i, 3)), red));
+  if (ns > 0) MSDP_TRY(dot(st, ns, x, z, slot(5), red));
+  MSDP_TRY(read_slots());
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += hsc[bslot(i, 3)];
+  if (ns > 0) s += hsc[5];
+  out = s;
+  return 0;
+}
+
+// _take_step! (primal_dual.jl:56-72)
+int msdp_solver::take_step(double a, double bt) {
+  alpha = a;
+  beta = bt;
+  for (auto& bk : blk) {
+    const int64_t n2 = (int64_t)bk.n * bk.n;
+    MSDP_TRY(axpby(st, n2, 1.0, bk.X, a, bk.delX, bk.X));
+    MSDP_TRY(symmetrize(st, bk.n, bk.X, bk.n));
+    MSDP_TRY(axpby(st, n2, 1.0, bk.S, bt, bk.delS, bk.S));
+    MSDP_TRY(symmetrize(st, bk.n, bk.S, bk.n));
+  }
+  if (m > 0) MSDP_TRY(axpby(st, m, 1.0, y, bt, dely, y));
+  if (ns > 0) {
+    MSDP_TRY(axpby(st, ns, 1.0, x, a, delx, x));
+    MSDP_TRY(axpby(st, ns, 1.0, z, bt, delz, z));
+  }
+  iter += 1;
+  return update_residuals();
+}
+
+// iterate! (primal_dual.jl:85-121)
+int msdp_solver::iterate() {
+  if (dual_centering()) return dc_iterate();
+  bool ok1 = false, ok2 = false;
+  MSDP_TRY(prepare(ok1));
+  if (ok1) MSDP_TRY(factorize_schur(ok2));
+  if (!ok1 || !ok2) { status = ERROR_IN_STEP_COMPUTATION; return 0; }
+  const double nu = std::max(barrier_dim(), 1);
+  if (opt.centering == MSDP_CENTER_MEHROTRA) {
+    double compl_ = 0;
+    MSDP_TRY(total_complementarity(compl_));
+    const double mu_ = compl_ / nu;
+    mu = mu_;
+    MSDP_TRY(solve_directions(0.0, false));
+    double aa, ba;
+    MSDP_TRY(step_lengths(aa, ba));
+    double mu_aff;
+    MSDP_TRY(affine_complementarity(aa, ba, mu_aff));
+    const double sg = std::min(std::max(std::pow(mu_aff / std::max(mu_, EPS), opt.sigma_exponent), 1e-8), 1.0);
+    sigma = sg;
+    MSDP_TRY(correction());
+    MSDP_TRY(solve_directions(sg * mu_, true));
+    double a, bt;
+    MSDP_TRY(step_lengths(a, bt));
+    return take_step(a, bt);
+  }
+  MSDP_TRY(solve_directions(mu, false));
+  double a, bt;
+  MSDP_TRY(step_lengths(a, bt));
+  MSDP_TRY(take_step(a, bt));
+  double compl_ = 0;
+  MSDP_TRY(total_complementarity(compl_));
+  double mu_ = opt.ratio * compl_ / nu;
+  if (std::min(a, bt) < 0.5) mu_ *= 1.5;
+  if (a + bt > 1.6) mu_ *= 0.75;
+  if (a + bt > 1.9) mu_ /= 1 + 0.1 * iter;
+  mu = mu_;
+  return 0;
+}
+
+// update_termination! (primal_dual.jl:123-135, dual_centering.jl:293-303)
+int msdp_solver::update_termination() {
+  if (dual_centering()) MSDP_TRY(update_residuals());
+  for (int i = 0; i < nb; ++i)
+    MSDP_TRY(dot(st, (int64_t)blk[i].n * blk[i].n, blk[i].C, blk[i].X, slot(bslot(i, 7)), red));
+  if (ns > 0) {
+    MSDP_TRY(dot(st, ns, dlin, x, slot(6), red));
+    MSDP_TRY(max_abs(st, ns, Rdlin, slot(7), red));
+  }
+  if (m > 0) {
+    MSDP_TRY(dot(st, m, b, y, slot(8), red));
+    MSDP_TRY(max_abs(st, m, Rp, slot(9), red));
+  }
+  MSDP_TRY(read_slots());
+  double p = 0.0;
+  for (int i = 0; i < nb; ++i) p += hsc[bslot(i, 7)];
+  if (ns > 0) p += hsc[6];
+  const double d = m > 0 ? hsc[8] : 0.0;
+  pobj = p;
+  dobj = d;
+  obj_val = p;
+  inf_pr = (m > 0 ? hsc[9] : 0.0) / std::max(1.0, bnorm);
+  double rd = 0.0;
+  for (int i = 0; i < nb; ++i) rd = std::max(rd, hsc[bslot(i, 1)]);
+  double cn = cnorm_max;
+  if (ns > 0) {
+    rd = std::max(rd, hsc[7]);
+    cn = std::max(cn, dnorm);
+  }
+  inf_du = rd / std::max(1.0, cn);
+  bool converged;
+  if (dual_centering()) {
+    inf_compl = dc_nu / std::max(dc_t, EPS) / std::max(1.0, std::fabs(p) + std::fabs(d));
+    converged = inf_compl <= opt.tol;
+  } else {
+    inf_compl = std::fabs(p - d) / std::max(1.0, std::fabs(p) + std::fabs(d));
+    converged = std::max(inf_pr, std::max(inf_du, inf_compl)) <= opt.tol;
+  }
+  if (converged) status = SOLVE_SUCCEEDED;
+  else if (iter >= opt.max_iter) status = MAXIMUM_ITERATIONS_EXCEEDED;
+  else if (now_s() - start_time >= opt.max_wall_time) status = MAXIMUM_WALLTIME_EXCEEDED;
+  if (!dual_centering() && status == REGULAR && mu < opt.mu_min)
+    status = SEARCH_DIRECTION_BECOMES_TOO_SMALL;
+  trace.push_back({p, d, inf_pr, inf_du, inf_compl, mu, alpha, beta, sigma});
+  return 0;
+}
+
+// initial_point! (solver.jl:214-284)
+int msdp_solver::initial_point() {
+  if (opt.initial_point == MSDP_INIT_SCALED_IDENTITY) {
+    double cscale = cnorm_max;
+    if (ns > 0) cscale = std::max(cscale, dnorm);
+    double beta0 = opt.identity_scale;
+    if (!(beta0 > 0))
+      beta0 = std::max(10.0, std::sqrt(std::max(bnorm, 1.0) * std::max(cscale, 1.0)) *
+                                 std::sqrt((double)barrier_dim()));
+    for (auto& bk : blk) {
+      MSDP_TRY(set_identity(st, bk.n, bk.X, bk.n, beta0));
+      MSDP_TRY(set_identity(st, bk.n, bk.S, bk.n, beta0));
+    }
+    if (m > 0) MSDP_TRY(fill(st, m, 0.0, y));
+    if (ns > 0) {
+      MSDP_TRY(fill(st, ns, beta0, x));
+      MSDP_TRY(fill(st, ns, beta0, z));
+    }
+    return 0;
+  }
+  if (opt.initial_point == MSDP_INIT_POINT) {
+    if ((int64_t)ip_y.size() != m) return as_err("InitialPoint not set");
+    if (m > 0) MSDP_HIP(hipMemcpyAsync(y, ip_y.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+    size_t off = 0;
+    for (auto& bk : blk) {
+      const size_t n2 = (size_t)bk.n * bk.n;
+      MSDP_HIP(hipMemcpyAsync(bk.X, ip_X.data() + off, sizeof(double) * n2, hipMemcpyHostToDevice, st));
+      off += n2;
+    }
+    if (ns > 0) {
+      if (ip_x.empty()) MSDP_TRY(fill(st, ns, 1.0, x));
+      else MSDP_HIP(hipMemcpyAsync(x, ip_x.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
+    }
+    MSDP_TRY(set_dual_slacks(y));
+    MSDP_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  // DualShift
+  for (auto& bk : blk) MSDP_TRY(set_identity(st, bk.n, bk.X, bk.n, 1.0));
+  if (ns > 0) MSDP_TRY(fill(st, ns, 1.0, x));
+  double rho = 0.0;
+  while (rho <= opt.max_shift) {
+    if (m > 0) MSDP_TRY(fill(st, m, -rho, y));
+    MSDP_TRY(set_dual_slacks(y));
+    // _dual_interior (solver.jl:255-258)
+    MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
+    for (int i = 0; i < nb; ++i) MSDP_TRY(cholesky_factor(blk[i], blk[i].S, blk[i].Ls, binfo(i, 0)));
+    std::vector<int> inf;
+    MSDP_TRY(read_infos(inf));
+    bool interior = true;
+    for (int i = 0; i < nb; ++i) interior = interior && inf[8 + 4 * i] == 0;
+    if (interior && ns > 0) {  // all(z .> 0)
+      MSDP_TRY(vec_min(st, (int)ns, z, slot(10)));
+      MSDP_TRY(read_slots());
+      interior = hsc[10] > 0;
+    }
+    if (interior) {
+      if (rho != 0.0) {
+        if (m > 0) MSDP_TRY(fill(st, m, -2 * rho, y));
+        MSDP_TRY(set_dual_slacks(y));
+      }
+      return 0;
+    }
+    rho = rho == 0.0 ? opt.initial_shift : 2 * rho;
+  }
+  set_error("MadSDP: could not find a dual interior point `y = -rho 1`; use `InitialPoint` to give one.");
+  return -3;
+}
+
+// initialize! (solver.jl:286-293) + initialize_algorithm!
+int msdp_solver::initialize() {
+  MSDP_TRY(finalize());
+  MSDP_TRY(initial_point());
+  iter = 0;
+  status = REGULAR;
+  start_time = now_s();
+  trace.clear();
+  alpha = beta = sigma = 0.0;
+  if (dual_centering()) return dc_initialize();
+  MSDP_TRY(update_residuals());
+  double compl_ = 0;
+  MSDP_TRY(total_complementarity(compl_));
+  const double nu = std::max(barrier_dim(), 1);
+  mu = opt.centering == MSDP_CENTER_MEHROTRA ? compl_ / nu : opt.initial_ratio * compl_ / nu;
+  return 0;
+}
+
+int msdp_solver::solve() {
+  MSDP_TRY(initialize());
+  MSDP_TRY(update_termination());
+  while (status == REGULAR) {
+    MSDP_TRY(iterate());
+    if (status != REGULAR) break;
+    MSDP_TRY(update_termination());
+  }
+  MSDP_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ==========================================================================
+// DualCentering (src/dual_centering.jl)
+// ==========================================================================
+int msdp_solver::refresh_slacks(bool& ok) {  // :83-86
+  MSDP_TRY(set_dual_slacks(y));
+  MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
+  for (int i = 0; i < nb; ++i) MSDP_TRY(prepare_block(blk[i], i));
+  std::vector<int> inf;
+  MSDP_TRY(read_infos(inf));
+  ok = true;
+  for (int i = 0; i < nb; ++i) ok = ok && inf[8 + 4 * i] == 0;
+  return 0;
+}
+
+int msdp_solver::newton_direction(bool& ok) {  // :89-110
+  MSDP_TRY(fill(st, m, 0.0, wm));
+  for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.Si, wm));
+  MSDP_TRY(axpby(st, m, 1.0, wm, -dc_t, b, dc_g));
+  MSDP_TRY(factorize_schur(ok));
+  if (!ok) return 0;
+  MSDP_HIP(hipMemcpyAsync(dc_d, dc_g, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+  MSDP_TRY(potrs(st, (int)m, H, m, dc_d));
+  MSDP_TRY(dot(st, m, dc_g, dc_d, slot(11), red));
+  MSDP_TRY(read_slots());
+  dc_lambda = std::sqrt(std::max(hsc[11], 0.0));
+  return 0;
+}
+
+int msdp_solver::dc_initialize() {  // :112-143
+  dc_nu = 0;
+  for (auto& bk : blk) dc_nu += bk.n;
+  if (opt.dc_A < 0) {
+    const double tau = 2 * opt.dc_beta / ((1 - opt.dc_beta) * (1 - opt.dc_beta));
+    dc_A = opt.dc_delta + (-tau - std::log1p(-tau));
+  } else {
+    dc_A = opt.dc_A;
+  }
+  n_predictor = n_newton = 0;
+  bool ok = false;
+  MSDP_TRY(refresh_slacks(ok));
+  if (!ok) { set_error("MadSDP: the initial `y` does not give a dual slack `S > 0`."); return -3; }
+  dc_t = 0.0;
+  MSDP_TRY(newton_direction(ok));
+  if (!ok) { set_error("MadSDP: failed to factorize the dual Hessian at y0."); return -3; }
+  MSDP_HIP(hipMemcpyAsync(dc_dy, b, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+  MSDP_TRY(potrs(st, (int)m, H, m, dc_dy));
+  MSDP_TRY(dot(st, m, b, dc_dy, slot(12), red));
+  MSDP_TRY(dot(st, m, b, dc_d, slot(13), red));
+  MSDP_TRY(read_slots());
+  const double bBb = std::max(hsc[12], EPS);
+  const double bBg = hsc[13];
+  dc_t = std::max(bBg / bBb, opt.dc_beta / std::sqrt(bBb));
+  for (int k = 0; k < opt.dc_centering_max_iter; ++k) {
+    MSDP_TRY(newton_direction(ok));
+    if (!ok) { set_error("MadSDP: failed to factorize the dual Hessian during the initial centering."); return -3; }
+    if (dc_lambda <= opt.dc_beta / 2) break;
+    MSDP_TRY(axpby(st, m, 1.0, y, -1.0 / (1 + dc_lambda), dc_d, y));
+    iter += 1;
+    n_newton += 1;
+    MSDP_TRY(refresh_slacks(ok));
+    if (!ok) { set_error("MadSDP: damped Newton left the interior during the initial centering."); return -3; }
+  }
+  mu = 1.0 / dc_t;
+  return 0;
+}
+
+int msdp_solver::predictor_step(bool& ok) {  // :232-264
+  MSDP_TRY(axpby(st, m, 1.0, y, 1.0, dc_d, dc_yhat));
+  MSDP_HIP(hipMemcpyAsync(dc_dy, b, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+  MSDP_TRY(potrs(st, (int)m, H, m, dc_dy));
+  MSDP_TRY(axpby(st, m, dc_t, dc_dy, 0.0, dc_dy, dc_dy));
+  MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
+  int64_t off = 0;
+  for (int i = 0; i < nb; ++i) {
+    Blk& bk = blk[i];
+    const int n = bk.n;
+    const int64_t n2 = (int64_t)n * n;
+    MSDP_TRY(slack(bk, dc_yhat, bk.Shat));
+    MSDP_TRY(jtprod(st, bk.ops, dc_dy, bk.Dc));
+    MSDP_TRY(symmetrize(st, n, bk.Dc, n));
+    // proximity eigenvalues (:205-215)
+    MSDP_TRY(cholesky_factor(bk, bk.Shat, bk.Lh, binfo(i, 2)));
+    MSDP_TRY(trtri_lower(st, n, bk.Lh, n, bk.Lx, n, trwork));  // Lx reused for Lhat^-1
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Dc, n, 0.0, bk.tmp, n));
+    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lx, n, 0.0, bk.tmp2, n));
+    MSDP_TRY(symmetrize(st, n, bk.tmp2, n));
+    MSDP_TRY(syevd(st, n, bk.tmp2, n, eigall + off, bk.Vv, n, false, ework, eiwork));
+    // ||Ls^-1 Shat Ls^-T||_F^2 (:219-229)
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lsinv, n, bk.Shat, n, 0.0, bk.tmp, n));
+    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lsinv, n, 0.0, bk.tmp2, n));
+    MSDP_TRY(dot(st, n2, bk.tmp2, bk.tmp2, slot(bslot(i, 6)), red));
+    off += n;
+  }
+  std::vector<int> inf;
+  MSDP_TRY(read_infos(inf));
+  for (int i = 0; i < nb; ++i)
+    if (inf[8 + 4 * i + 2] != 0) { ok = false; return 0; }
+  MSDP_TRY(dc_solve_alpha(st, (int)off, eigall, dc_A, slot(14)));
+  MSDP_TRY(read_slots());
+  double shat_n2 = 0.0;
+  for (int i = 0; i < nb; ++i) shat_n2 += hsc[bslot(i, 6)];
+  const double a = hsc[14];
+  alpha = beta = a;
+  for (auto& bk : blk) {
+    const int n = bk.n;
+    const int64_t n2 = (int64_t)n * n;
+    MSDP_TRY(axpby(st, n2, 1 - a, bk.Shat, a, bk.Dc, bk.buf));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Si, n, bk.buf, n, 0.0, bk.buf2, n));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0 / dc_t, bk.buf2, n, bk.Si, n, 0.0, bk.X, n));
+    MSDP_TRY(symmetrize(st, n, bk.X, n));
+  }
+  MSDP_TRY(axpby(st, m, 1.0, dc_yhat, a, dc_dy, y));
+  dc_t = dc_nu * dc_t / ((1 - a) * shat_n2);
+  n_predictor += 1;
+  ok = true;
+  return 0;
+}
+
+int msdp_solver::dc_iterate() {  // :266-289
+  bool ok = false;
+  MSDP_TRY(newton_direction(ok));
+  if (!ok) { status = ERROR_IN_STEP_COMPUTATION; return 0; }
+  if (dc_lambda > opt.dc_beta) {
+    MSDP_TRY(axpby(st, m, 1.0, y, -1.0 / (1 + dc_lambda), dc_d, y));
+    alpha = beta = 1.0 / (1 + dc_lambda);
+    n_newton += 1;
+  } else {
+    MSDP_TRY(predictor_step(ok));
+    if (!ok) { status = ERROR_IN_STEP_COMPUTATION; return 0; }
+  }
+  iter += 1;
+  MSDP_TRY(refresh_slacks(ok));
+  if (!ok) { status = ERROR_IN_STEP_COMPUTATION; return 0; }
+  mu = 1.0 / dc_t;
+  return 0;
+}
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+#define GUARD(s) \
+  if (!(s)) return as_err("null solver");
+
+extern "C" {
+
+const char* msdp_last_error(void) { return last_error(); }
+
+void msdp_default_options(msdp_options* o) {
+  std::memset(o, 0, sizeof *o);
+  o->tol = 1e-7;
+  o->max_iter = 100;
+  o->max_wall_time = 3600;
+  o->mu_min = 1e-14;
+  o->regularize_schur = 1e-12;
+  o->max_schur_reg = 8;
+  o->algorithm = MSDP_ALG_PRIMAL_DUAL;
+  o->direction = MSDP_DIR_NT;
+  o->step = MSDP_STEP_EIGEN;
+  o->centering = MSDP_CENTER_MEHROTRA;
+  o->initial_point = MSDP_INIT_SCALED_IDENTITY;
+  o->schur = MSDP_SCHUR_DENSE_CHOLESKY;
+  o->tau = 0.95;
+  o->shrink = 0.8;
+  o->boundary = 0.95;
+  o->sigma_exponent = 3.0;
+  o->initial_ratio = 0.25;
+  o->ratio = 0.5;
+  o->identity_scale = 0.0;
+  o->initial_shift = 1.0;
+  o->max_shift = 1e10;
+  o->dc_beta = 0.2;
+  o->dc_delta = 1.0;
+  o->dc_A = 2.0;
+  o->dc_centering_max_iter = 50;
+}
+
+int msdp_solver_create(msdp_solver** out, int device, int32_t nblocks, const int64_t* dims,
+                       int64_t m, int64_t n_scalar) {
+  if (!out) return as_err("null out");
+  *out = nullptr;
+  if (nblocks < 0 || m < 0 || n_scalar < 0) return as_err("negative size");
+  if (nblocks == 0 && n_scalar == 0) return as_err("MadSDP: model has no variables.");
+  int ndev = 0;
+  MSDP_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return as_err("no such HIP device");
+  MSDP_HIP(hipSetDevice(device));
+  auto* s = new msdp_solver();
+  s->device = device;
+  s->nb = nblocks;
+  s->m = m;
+  s->ns = n_scalar;
+  s->blk.resize(nblocks);
+  for (int i = 0; i < nblocks; ++i) {
+    if (dims[i] <= 0) { delete s; return as_err("block dimension must be positive"); }
+    s->blk[i].n = (int)dims[i];
+  }
+  msdp_default_options(&s->opt);
+  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete s; return fail_hip(e, "hipStreamCreate", __FILE__, __LINE__); }
+  *out = s;
+  return 0;
+}
+
+int msdp_solver_destroy(msdp_solver* s) {
+  delete s;
+  return 0;
+}
+
+int msdp_solver_set_cost_block(msdp_solver* s, int32_t block, const double* C) {
+  GUARD(s);
+  if (block < 0 || block >= s->nb) return as_err("block out of range");
+  MSDP_HIP(hipSetDevice(s->device));
+  Blk& b = s->blk[block];
+  const size_t n2 = (size_t)b.n * b.n;
+  if (!b.C) MSDP_TRY(s->alloc(&b.C, n2));
+  MSDP_HIP(hipMemcpy(b.C, C, sizeof(double) * n2, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int msdp_solver_set_constraints_coo(msdp_solver* s, int32_t block, int64_t nnz, const int64_t* con,
+                                    const int64_t* row, const int64_t* col, const double* val) {
+  GUARD(s);
+  if (block < 0 || block >= s->nb) return as_err("block out of range");
+  if (s->finalized) return as_err("problem already finalized");
+  Blk& b = s->blk[block];
+  if (b.has_dense) return as_err("block already has dense constraints");
+  for (int64_t e = 0; e < nnz; ++e) {
+    if (con[e] < 0 || con[e] >= s->m || row[e] < 0 || row[e] >= b.n || col[e] < 0 || col[e] >= b.n)
+      return as_err("constraint entry out of range");
+    if (val[e] == 0.0) continue;
+    b.h_con.push_back((int)con[e]);
+    b.h_row.push_back((int)row[e]);
+    b.h_col.push_back((int)col[e]);
+    b.h_val.push_back(val[e]);
+  }
+  b.has_coo = true;
+  return 0;
+}
+
+int msdp_solver_set_constraints_dense(msdp_solver* s, int32_t block, int64_t m_i,
+                                      const int64_t* con_idx, const double* mats) {
+  GUARD(s);
+  if (block < 0 || block >= s->nb) return as_err("block out of range");
+  if (s->finalized) return as_err("problem already finalized");
+  Blk& b = s->blk[block];
+  if (b.has_coo) return as_err("block already has coo constraints");
+  for (int64_t j = 0; j < m_i; ++j) {
+    if (con_idx[j] < 0 || con_idx[j] >= s->m) return as_err("constraint index out of range");
+    if (j > 0 && con_idx[j] <= con_idx[j - 1]) return as_err("con_idx must be strictly increasing");
+  }
+  b.h_dcon.assign(con_idx, con_idx + m_i);
+  b.h_mats.assign(mats, mats + (size_t)m_i * b.n * b.n);
+  b.has_dense = true;
+  return 0;
+}
+
+int msdp_solver_set_rhs(msdp_solver* s, const double* bv) {
+  GUARD(s);
+  MSDP_HIP(hipSetDevice(s->device));
+  if (!s->b) MSDP_TRY(s->alloc(&s->b, std::max<int64_t>(s->m, 1)));
+  if (s->m > 0) MSDP_HIP(hipMemcpy(s->b, bv, sizeof(double) * s->m, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int msdp_solver_set_scalar_data(msdp_solver* s, const double* d_lin, const double* C_lin) {
+  GUARD(s);
+  MSDP_HIP(hipSetDevice(s->device));
+  const size_t nss = std::max<int64_t>(s->ns, 1), mm = std::max<int64_t>(s->m, 1);
+  if (!s->dlin) MSDP_TRY(s->alloc(&s->dlin, nss));
+  if (!s->Clin) MSDP_TRY(s->alloc(&s->Clin, nss * mm));
+  if (s->ns > 0) {
+    MSDP_HIP(hipMemcpy(s->dlin, d_lin, sizeof(double) * s->ns, hipMemcpyHostToDevice));
+    if (s->m > 0) MSDP_HIP(hipMemcpy(s->Clin, C_lin, sizeof(double) * s->ns * s->m, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int msdp_solver_configure(msdp_solver* s, const msdp_options* o) {
+  GUARD(s);
+  if (o->algorithm == MSDP_ALG_DUAL_CENTERING) {
+    if (s->ns > 0) return as_err("MadSDP: `DualCentering` does not support scalar (LP) blocks yet.");
+    if (s->nb == 0) return as_err("MadSDP: `DualCentering` needs at least one PSD block.");
+  }
+  if (o->schur == MSDP_SCHUR_QR &&
+      !(o->algorithm == MSDP_ALG_PRIMAL_DUAL && o->direction == MSDP_DIR_NT))
+    return as_err("QRSchur requires PrimalDual with the NT direction.");
+  if (o->schur == MSDP_SCHUR_QR) return as_err("QRSchur is not available in this build");
+  s->opt = *o;
+  return 0;
+}
+
+int msdp_solver_set_initial_point(msdp_solver* s, const double* X, const double* yv, const double* xv) {
+  GUARD(s);
+  size_t tot = 0;
+  for (auto& b : s->blk) tot += (size_t)b.n * b.n;
+  s->ip_X.assign(X, X + tot);
+  s->ip_y.assign(yv, yv + s->m);
+  if (xv) s->ip_x.assign(xv, xv + s->ns);
+  else s->ip_x.clear();
+  return 0;
+}
+
+int msdp_solver_initialize(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->initialize(); }
+int msdp_solver_iterate(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->iterate(); }
+int msdp_solver_update_termination(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->update_termination(); }
+int msdp_solver_solve(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->solve(); }
+int msdp_solver_synchronize(msdp_solver* s) { GUARD(s); MSDP_HIP(hipStreamSynchronize(s->st)); return 0; }
+
+int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
+  GUARD(s);
+  o->mu = s->mu; o->sigma = s->sigma; o->alpha = s->alpha; o->beta = s->beta;
+  o->inf_pr = s->inf_pr; o->inf_du = s->inf_du; o->inf_compl = s->inf_compl; o->obj_val = s->obj_val;
+  o->pobj = s->pobj; o->dobj = s->dobj; o->iter = s->iter; o->status = s->status;
+  o->n_newton = s->n_newton; o->n_predictor = s->n_predictor; o->dc_t = s->dc_t; o->dc_lambda = s->dc_lambda;
+  return 0;
+}
+
+int msdp_solver_trace(const msdp_solver* s, int32_t maxrec, double* out9, int32_t* nrec) {
+  GUARD(s);
+  const int n = (int)std::min<size_t>(s->trace.size(), (size_t)std::max(maxrec, 0));
+  for (int k = 0; k < n; ++k) std::memcpy(out9 + 9 * k, s->trace[k].data(), 9 * sizeof(double));
+  *nrec = (int)s->trace.size();
+  return 0;
+}
+
+int msdp_solver_update_residuals(msdp_solver* s) { GUARD(s); MSDP_TRY(s->finalize()); return s->update_residuals(); }
+int msdp_solver_prepare(msdp_solver* s, int32_t* ok) {
+  GUARD(s);
+  bool b = false;
+  MSDP_TRY(s->prepare(b));
+  *ok = b;
+  return 0;
+}
+int msdp_solver_factorize_schur(msdp_solver* s, int32_t* ok) {
+  GUARD(s);
+  bool b = false;
+  MSDP_TRY(s->factorize_schur(b));
+  *ok = b;
+  return 0;
+}
+int msdp_solver_solve_directions(msdp_solver* s, double mu, int32_t corrected) {
+  GUARD(s);
+  return s->solve_directions(mu, corrected != 0);
+}
+int msdp_solver_correction(msdp_solver* s) { GUARD(s); return s->correction(); }
+int msdp_solver_step_lengths(msdp_solver* s, double* a, double* bt) { GUARD(s); return s->step_lengths(*a, *bt); }
+int msdp_solver_affine_complementarity(msdp_solver* s, double a, double bt, double* out) {
+  GUARD(s);
+  return s->affine_complementarity(a, bt, *out);
+}
+int msdp_solver_take_step(msdp_solver* s, double a, double bt) { GUARD(s); return s->take_step(a, bt); }
+int msdp_solver_schur_solve(msdp_solver* s, double* rhs) {
+  GUARD(s);
+  MSDP_HIP(hipMemcpyAsync(s->wm, rhs, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
+  MSDP_TRY(potrs(s->st, (int)s->m, s->H, s->m, s->wm));
+  MSDP_HIP(hipMemcpyAsync(rhs, s->wm, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->st));
+  MSDP_HIP(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+static int arr(msdp_solver* s, int which, int block, double** p, size_t* count) {
+  auto needb = [&]() { return block >= 0 && block < s->nb; };
+  Blk* b = needb() ? &s->blk[block] : nullptr;
+  const size_t n2 = b ? (size_t)b->n * b->n : 0;
+  switch (which) {
+    case MSDP_X: if (!b) return -2; *p = b->X; *count = n2; return 0;
+    case MSDP_S: if (!b) return -2; *p = b->S; *count = n2; return 0;
+    case MSDP_DELX: if (!b) return -2; *p = b->delX; *count = n2; return 0;
+    case MSDP_DELS: if (!b) return -2; *p = b->delS; *count = n2; return 0;
+    case MSDP_RD: if (!b) return -2; *p = b->Rd; *count = n2; return 0;
+    case MSDP_W: if (!b) return -2; *p = b->W; *count = n2; return 0;
+    case MSDP_G: if (!b) return -2; *p = b->G; *count = n2; return 0;
+    case MSDP_SI: if (!b) return -2; *p = b->Si; *count = n2; return 0;
+    case MSDP_CORR: if (!b) return -2; *p = b->corr; *count = n2; return 0;
+    case MSDP_LS: if (!b) return -2; *p = b->Ls; *count = n2; return 0;
+    case MSDP_LX: if (!b) return -2; *p = b->Lx; *count = n2; return 0;
+    case MSDP_D: if (!b) return -2; *p = b->D; *count = b->n; return 0;
+    case MSDP_Y: *p = s->y; *count = s->m; return 0;
+    case MSDP_DELY: *p = s->dely; *count = s->m; return 0;
+    case MSDP_RP: *p = s->Rp; *count = s->m; return 0;
+    case MSDP_XLIN: *p = s->x; *count = s->ns; return 0;
+    case MSDP_ZLIN: *p = s->z; *count = s->ns; return 0;
+    case MSDP_DELXLIN: *p = s->delx; *count = s->ns; return 0;
+    case MSDP_DELZLIN: *p = s->delz; *count = s->ns; return 0;
+    case MSDP_RDLIN: *p = s->Rdlin; *count = s->ns; return 0;
+    case MSDP_H: *p = s->H; *count = (size_t)s->m * s->m; return 0;
+  }
+  return -2;
+}
+
+int msdp_solver_get(const msdp_solver* cs, int32_t which, int32_t block, double* host) {
+  GUARD(cs);
+  auto* s = const_cast<msdp_solver*>(cs);
+  MSDP_TRY(s->finalize());
+  double* p; size_t c;
+  if (arr(s, which, block, &p, &c) != 0) return as_err("bad array selector or block");
+  MSDP_HIP(hipMemcpyAsync(host, p, sizeof(double) * c, hipMemcpyDeviceToHost, s->st));
+  MSDP_HIP(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int msdp_solver_set(msdp_solver* s, int32_t which, int32_t block, const double* host) {
+  GUARD(s);
+  MSDP_TRY(s->finalize());
+  double* p; size_t c;
+  if (arr(s, which, block, &p, &c) != 0) return as_err("bad array selector or block");
+  MSDP_HIP(hipMemcpyAsync(p, host, sizeof(double) * c, hipMemcpyHostToDevice, s->st));
+  MSDP_HIP(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+// ---- kernel layer ----
+int msdp_k_gemm(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, double alpha,
+                const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                int64_t ldc) {
+  return gemm((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info) {
+  return potrf((hipStream_t)stream, n, A, lda, info, nullptr);
+}
+size_t msdp_k_cholesky_factor_scratch(int32_t) { return reduce_scratch_elems(0) + 64; }
+int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm, double* L, int64_t ldl,
+                           int32_t* info, double* scratch) {
+  hipStream_t st = (hipStream_t)stream;
+  MSDP_TRY(copy_mat(st, n, n, M, ldm, L, ldl));
+  MSDP_TRY(symmetrize(st, n, L, ldl));
+  MSDP_TRY(potrf(st, n, L, ldl, info, nullptr));
+  return chol_flush(st, n, L, ldl, scratch);
+}
+size_t msdp_k_trtri_work(int32_t n) { return trtri_work_elems(n); }
+int msdp_k_trtri(void* stream, int32_t n, const double* L, int64_t ldl, double* Li, int64_t ldi, double* work) {
+  return trtri_lower((hipStream_t)stream, n, L, ldl, Li, ldi, work);
+}
+size_t msdp_k_syevd_work(int32_t n) { return syevd_work_elems(n); }
+size_t msdp_k_syevd_iwork(int32_t n) { return syevd_iwork_elems(n); }
+int msdp_k_syevd(void* stream, int32_t n, const double* A, int64_t lda, double* w, double* V, int64_t ldv,
+                 int32_t vectors, double* work, int32_t* iwork) {
+  return syevd((hipStream_t)stream, n, A, lda, w, V, ldv, vectors != 0, work, iwork);
+}
+int msdp_k_eig_min(void* stream, int32_t n, const double* A, int64_t lda, double* out, double* work) {
+  return eig_min((hipStream_t)stream, n, A, lda, out, work, nullptr);
+}
+int msdp_k_schur_onehot(void* stream, int32_t m, const double* L, int64_t ldl, const double* R, int64_t ldr,
+                        double* H, int64_t ldh) {
+  return schur_onehot_identity((hipStream_t)stream, m, L, ldl, R, ldr, H, ldh);
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+"""ctypes binding of libmadsdp_hip.so (declared in include/madsdp_hip.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the shared object is missing, and creating a solver fails loudly when
+no HIP device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmadsdp_hip.so"
+
+c_double_p = C.POINTER(C.c_double)
+c_int64_p = C.POINTER(C.c_int64)
+c_int32_p = C.POINTER(C.c_int32)
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double), ("max_iter", C.c_int32), ("max_wall_time", C.c_double),
+        ("mu_min", C.c_double), ("regularize_schur", C.c_double), ("max_schur_reg", C.c_int32),
+        ("algorithm", C.c_int32), ("direction", C.c_int32), ("step", C.c_int32),
+        ("centering", C.c_int32), ("initial_point", C.c_int32), ("schur", C.c_int32),
+        ("tau", C.c_double), ("shrink", C.c_double), ("boundary", C.c_double),
+        ("sigma_exponent", C.c_double), ("initial_ratio", C.c_double), ("ratio", C.c_double),
+        ("identity_scale", C.c_double), ("initial_shift", C.c_double), ("max_shift", C.c_double),
+        ("dc_beta", C.c_double), ("dc_delta", C.c_double), ("dc_A", C.c_double),
+        ("dc_centering_max_iter", C.c_int32),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("mu", C.c_double), ("sigma", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
+        ("inf_pr", C.c_double), ("inf_du", C.c_double), ("inf_compl", C.c_double),
+        ("obj_val", C.c_double), ("pobj", C.c_double), ("dobj", C.c_double),
+        ("iter", C.c_int32), ("status", C.c_int32), ("n_newton", C.c_int32),
+        ("n_predictor", C.c_int32), ("dc_t", C.c_double), ("dc_lambda", C.c_double),
+    ]
+
+
+# name -> (restype, argtypes)
+P = C.c_void_p
+SIGNATURES = {
+    "msdp_last_error": (C.c_char_p, []),
+    "msdp_default_options": (None, [C.POINTER(Options)]),
+    "msdp_solver_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int32, c_int64_p, C.c_int64, C.c_int64]),
+    "msdp_solver_destroy": (C.c_int, [P]),
+    "msdp_solver_set_cost_block": (C.c_int, [P, C.c_int32, c_double_p]),
+    "msdp_solver_set_constraints_coo": (C.c_int, [P, C.c_int32, C.c_int64, c_int64_p, c_int64_p, c_int64_p, c_double_p]),
+    "msdp_solver_set_constraints_dense": (C.c_int, [P, C.c_int32, C.c_int64, c_int64_p, c_double_p]),
+    "msdp_solver_set_rhs": (C.c_int, [P, c_double_p]),
+    "msdp_solver_set_scalar_data": (C.c_int, [P, c_double_p, c_double_p]),
+    "msdp_solver_configure": (C.c_int, [P, C.POINTER(Options)]),
+    "msdp_solver_set_initial_point": (C.c_int, [P, c_double_p, c_double_p, c_double_p]),
+    "msdp_solver_initialize": (C.c_int, [P]),
+    "msdp_solver_iterate": (C.c_int, [P]),
+    "msdp_solver_update_termination": (C.c_int, [P]),
+    "msdp_solver_solve": (C.c_int, [P]),
+    "msdp_solver_get_info": (C.c_int, [P, C.POINTER(Info)]),
+    "msdp_solver_trace": (C.c_int, [P, C.c_int32, c_double_p, c_int32_p]),
+    "msdp_solver_update_residuals": (C.c_int, [P]),
+    "msdp_solver_prepare": (C.c_int, [P, c_int32_p]),
+    "msdp_solver_factorize_schur": (C.c_int, [P, c_int32_p]),
+    "msdp_solver_solve_directions": (C.c_int, [P, C.c_double, C.c_int32]),
+    "msdp_solver_correction": (C.c_int, [P]),
+    "msdp_solver_step_lengths": (C.c_int, [P, c_double_p, c_double_p]),
+    "msdp_solver_affine_complementarity": (C.c_int, [P, C.c_double, C.c_double, c_double_p]),
+    "msdp_solver_take_step": (C.c_int, [P, C.c_double, C.c_double]),
+    "msdp_solver_schur_solve": (C.c_int, [P, c_double_p]),
+    "msdp_solver_get": (C.c_int, [P, C.c_int32, C.c_int32, c_double_p]),
+    "msdp_solver_set": (C.c_int, [P, C.c_int32, C.c_int32, c_double_p]),
+    "msdp_solver_synchronize": (C.c_int, [P]),
+    "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                              P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
+    "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P]),
+    "msdp_k_cholesky_factor": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, P]),
+    "msdp_k_cholesky_factor_scratch": (C.c_size_t, [C.c_int32]),
+    "msdp_k_trtri": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P]),
+    "msdp_k_trtri_work": (C.c_size_t, [C.c_int32]),
+    "msdp_k_syevd": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P, C.c_int64, C.c_int32, P, P]),
+    "msdp_k_syevd_work": (C.c_size_t, [C.c_int32]),
+    "msdp_k_syevd_iwork": (C.c_size_t, [C.c_int32]),
+    "msdp_k_eig_min": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),
+    "msdp_k_schur_onehot": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, C.c_int64]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(the HIP library is required; there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc < 0:
+        msg = load().msdp_last_error().decode(errors="replace")
+        raise HipError(f"{what}: {msg}" if what else msg)
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def iptr(a):
+    return a.ctypes.data_as(c_int64_p)

@@ -1,0 +1,179 @@
+"""Device building blocks through the C ABI, against numpy/scipy FP64.
+
+Each kernel replaces a LAPACK/BLAS call of the reference (gemm, potrf,
+trtri/potri, syevd, lambda_min); the bar is FP64 roundoff relative to the
+problem's scale.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env(gpu):
+    import torch
+    from madsdp_amd import lib as L
+    lib = L.load()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return torch, lib, stream
+
+
+def dev(torch, a):
+    """Column-major FP64 device copy of a numpy array."""
+    return torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).cuda()
+
+
+def host(t, shape):
+    return t.cpu().numpy().reshape(shape, order="F")
+
+
+def P(t):
+    return C.c_void_p(t.data_ptr())
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 3, 129)])
+def test_gemm(env, ta, tb, M, N, K):
+    torch, lib, st = env
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Cm = rng.standard_normal((M, N))
+    dA, dB, dC = dev(torch, A), dev(torch, B), dev(torch, Cm)
+    print(f"gemm ta={ta} tb={tb} M={M} N={N} K={K}", flush=True)
+    rc = lib.msdp_k_gemm(st, ta, tb, M, N, K, 1.5, P(dA), A.shape[0], P(dB), B.shape[0], -0.5, P(dC), M)
+    assert rc == 0
+    torch.cuda.synchronize()
+    ref = 1.5 * (A.T if ta else A) @ (B.T if tb else B) - 0.5 * Cm
+    np.testing.assert_allclose(host(dC, (M, N)), ref, rtol=0, atol=1e-12 * max(1, K))
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 300])
+def test_cholesky_factor(env, n):
+    torch, lib, st = env
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    M = G @ G.T + n * np.eye(n)
+    M[np.triu_indices(n, 1)] += 1e-3  # asymmetric input: factor of (M+M')/2
+    dM, dL = dev(torch, M), dev(torch, np.zeros((n, n)))
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    scratch = torch.zeros(int(lib.msdp_k_cholesky_factor_scratch(n)), dtype=torch.float64, device="cuda")
+    print(f"cholesky n={n}", flush=True)
+    assert lib.msdp_k_cholesky_factor(st, n, P(dM), n, P(dL), n, P(info), P(scratch)) == 0
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    Lh = host(dL, (n, n))
+    ref = np.linalg.cholesky((M + M.T) / 2)
+    np.testing.assert_allclose(Lh, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert np.all(np.triu(Lh, 1) == 0)
+
+
+def test_cholesky_not_pd(env):
+    torch, lib, st = env
+    for M, pivot in ((np.array([[1.0, 2.0], [2.0, 1.0]]), 2), (np.zeros((2, 2)), 1)):
+        dM, dL = dev(torch, M), dev(torch, np.zeros((2, 2)))
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        scratch = torch.zeros(1024, dtype=torch.float64, device="cuda")
+        assert lib.msdp_k_cholesky_factor(st, 2, P(dM), 2, P(dL), 2, P(info), P(scratch)) == 0
+        torch.cuda.synchronize()
+        assert int(info.item()) == pivot
+
+
+def test_cholesky_flush(env):
+    """test/blocks.jl:24-34: entries below eps^2 max|L_ii| are flushed."""
+    torch, lib, st = env
+    eps = np.finfo(float).eps
+    F = np.array([[1, 0, 0], [np.finfo(float).tiny, 1, 0], [eps, 1, 1]])
+    M = F @ F.T
+    dM, dL = dev(torch, M), dev(torch, np.zeros((3, 3)))
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    scratch = torch.zeros(1024, dtype=torch.float64, device="cuda")
+    assert lib.msdp_k_cholesky_factor(st, 3, P(dM), 3, P(dL), 3, P(info), P(scratch)) == 0
+    torch.cuda.synchronize()
+    Lh = host(dL, (3, 3))
+    assert Lh[1, 0] == 0.0
+    assert Lh[2, 0] == eps
+    np.testing.assert_allclose(Lh @ Lh.T, M, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 130, 333])
+def test_trtri(env, n):
+    torch, lib, st = env
+    rng = np.random.default_rng(n + 11)
+    Lm = np.tril(rng.standard_normal((n, n))) + 4 * np.eye(n)
+    dL, dI = dev(torch, Lm), dev(torch, np.full((n, n), 7.0))
+    work = torch.zeros(int(lib.msdp_k_trtri_work(n)) + 1, dtype=torch.float64, device="cuda")
+    print(f"trtri n={n}", flush=True)
+    assert lib.msdp_k_trtri(st, n, P(dL), n, P(dI), n, P(work)) == 0
+    torch.cuda.synchronize()
+    ref = sla.solve_triangular(Lm, np.eye(n), lower=True)
+    np.testing.assert_allclose(host(dI, (n, n)), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def _spd_clustered(rng, n, clusters):
+    """SPD matrix with clustered and repeated eigenvalues (the near-central
+    NT scaling point is a multiple of the identity)."""
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.sort(rng.choice(clusters, size=n)) * (1 + 1e-13 * rng.standard_normal(n))
+    return (Q * lam) @ Q.T
+
+
+@pytest.mark.parametrize("n", [2, 10, 64, 65, 100, 257, 600])
+@pytest.mark.parametrize("kind", ["random", "clustered", "identity"])
+def test_syevd(env, n, kind):
+    torch, lib, st = env
+    rng = np.random.default_rng(n)
+    if kind == "random":
+        A = rng.standard_normal((n, n))
+        A = (A + A.T) / 2
+    elif kind == "clustered":
+        A = _spd_clustered(rng, n, np.array([1.0, 1.0 + 1e-9, 2.0, 5.0]))
+        A = (A + A.T) / 2
+    else:
+        A = 3.0 * np.eye(n)
+    dA = dev(torch, A)
+    w = torch.zeros(n, dtype=torch.float64, device="cuda")
+    V = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    work = torch.zeros(int(lib.msdp_k_syevd_work(n)), dtype=torch.float64, device="cuda")
+    iwork = torch.zeros(int(lib.msdp_k_syevd_iwork(n)), dtype=torch.int32, device="cuda")
+    print(f"syevd n={n} kind={kind}", flush=True)
+    assert lib.msdp_k_syevd(st, n, P(dA), n, P(w), P(V), n, 1, P(work), P(iwork)) == 0
+    torch.cuda.synchronize()
+    wh, Vh = w.cpu().numpy(), host(V, (n, n))
+    ref = np.linalg.eigvalsh(A)
+    scale = max(1.0, np.abs(ref).max())
+    np.testing.assert_allclose(wh, ref, rtol=0, atol=1e-12 * scale * max(1, n / 64))
+    np.testing.assert_allclose(Vh.T @ Vh, np.eye(n), rtol=0, atol=1e-11 * max(1, n / 64))
+    np.testing.assert_allclose(A @ Vh, Vh * wh, rtol=0, atol=1e-11 * scale * max(1, n / 64))
+
+
+@pytest.mark.parametrize("n", [3, 64, 65, 200, 500])
+def test_eig_min(env, n):
+    torch, lib, st = env
+    rng = np.random.default_rng(n + 3)
+    A = rng.standard_normal((n, n))
+    A = (A + A.T) / 2
+    dA = dev(torch, A)
+    out = torch.zeros(1, dtype=torch.float64, device="cuda")
+    work = torch.zeros(int(lib.msdp_k_syevd_work(n)), dtype=torch.float64, device="cuda")
+    assert lib.msdp_k_eig_min(st, n, P(dA), n, P(out), P(work)) == 0
+    torch.cuda.synchronize()
+    ref = np.linalg.eigvalsh(A)[0]
+    assert abs(out.item() - ref) <= 1e-12 * max(1, np.abs(A).max() * np.sqrt(n))
+
+
+def test_schur_onehot(env):
+    """Max-cut Schur complement H = W o W (lower), src/components.jl:102-104."""
+    torch, lib, st = env
+    rng = np.random.default_rng(5)
+    n = 300
+    W = rng.standard_normal((n, n))
+    W = W + W.T
+    dW, dH = dev(torch, W), dev(torch, np.zeros((n, n)))
+    assert lib.msdp_k_schur_onehot(st, n, P(dW), n, P(dW), n, P(dH), n) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(np.tril(host(dH, (n, n))), np.tril(W * W))
