@@ -135,8 +135,14 @@ int msdp_solver_schur_solve(msdp_solver* s, double* rhs_host);    /* schur_solve
 /* copy an iterate array (block index for matrix arrays, ignored otherwise) */
 int msdp_solver_get(const msdp_solver* s, int32_t which, int32_t block, double* host);
 int msdp_solver_set(msdp_solver* s, int32_t which, int32_t block, const double* host);
-/* wall-clock seconds spent on the device stream by the last solve */
+/* wait for the solver's stream */
 int msdp_solver_synchronize(msdp_solver* s);
+/* live timing of one kernel class on the solver's stream (0 = FP64 MFMA GEMM,
+ * 1 = tridiagonalisation update): HIP events around every launch, summed
+ * duration, launch count and algorithmic work (flops or bytes).  No
+ * counterpart in the reference; it feeds the roofline report. */
+int msdp_solver_profile_begin(msdp_solver* s, int32_t kernel_class);
+int msdp_solver_profile_end(msdp_solver* s, double* total_ms, int64_t* launches, double* work);
 
 /* ---- kernel layer: device pointers, caller's stream ---- */
 /* C = alpha op(A) op(B) + beta C (BLAS dgemm; directions.jl, step.jl products) */

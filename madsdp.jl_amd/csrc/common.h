@@ -131,3 +131,17 @@ int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t s
                    int batch, bool vectors);
 
 }  // namespace msdp
+
+namespace msdp {
+// --------------------------------------------------------------------------
+// Live kernel timing: when enabled for a kernel class on a stream, every
+// launch of that class is bracketed by HIP events on that stream and its
+// algorithmic work (flops or bytes) is accumulated.
+// --------------------------------------------------------------------------
+enum ProfClass { PROF_GEMM = 0, PROF_TRD = 1, PROF_NONE = -1 };
+bool prof_active(int cls, hipStream_t s);
+void prof_before(hipStream_t s);
+void prof_after(hipStream_t s, double work);
+int prof_begin(int cls, hipStream_t s);
+int prof_end(double* total_ms, int64_t* launches, double* work);
+}  // namespace msdp

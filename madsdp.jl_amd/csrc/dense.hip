@@ -31,7 +31,9 @@ static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 // through LDS.  Operand layouts of the f64 16x16x4 MFMA (wave64):
 //   A (16x4):  lane l holds A[l%16][l/16]
 //   B (4x16):  lane l holds B[l/16][l%16]
-//   D (16x16): lane l, register r holds D[4*(l/16)+r][l%16]
+//   D (16x16): lane l, register r holds D[l/16 + 4*r][l%16]
+//              (measured on gfx950: rows 0, 5, 10, 15 are the only rows on
+//              which this agrees with the f32 layout 4*(l/16)+r)
 // ==========================================================================
 constexpr int GT = 64;   // tile edge
 constexpr int GK = 16;   // k depth per LDS stage
@@ -93,12 +95,12 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(
   const v4d* accs[4] = {&acc00, &acc01, &acc10, &acc11};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int i0 = m0 + wm + 16 * (q >> 1) + 4 * lk;
+    const int i0 = m0 + wm + 16 * (q >> 1) + lk;
     const int gn = n0 + wn + 16 * (q & 1) + li;
     if (gn >= N) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int gm = i0 + r;
+      const int gm = i0 + 4 * r;
       if (gm >= M || (lower && gm < gn)) continue;
       double* c = C + gm + (int64_t)gn * ldc;
       const double v = alpha * (*accs[q])[r];
@@ -125,6 +127,8 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     return 0;
   }
   dim3 grid(cdiv(M, GT), cdiv(N, GT), batch);
+  const bool prof = prof_active(PROF_GEMM, s);
+  if (prof) prof_before(s);
   if (!ta && !tb)
     gemm_f64_kernel<false, false><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
   else if (ta && !tb)
@@ -133,6 +137,7 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     gemm_f64_kernel<false, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
   else
     gemm_f64_kernel<true, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+  if (prof) prof_after(s, 2.0 * M * N * (double)K * batch * (lower ? 0.5 : 1.0));
   MSDP_HIP(hipGetLastError());
   return 0;
 }

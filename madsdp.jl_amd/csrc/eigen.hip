@@ -296,7 +296,12 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
     double* pn = (k & 1) ? t.p0 : t.p1;
     const int L = n - k - 1;
     dim3 g(cdiv(L, 256), cdiv(L, 64));
+    const bool prof = prof_active(PROF_TRD, s);
+    if (prof) prof_before(s);
     trd_update_product_kernel<<<g, 256, 0, s>>>(n, k, t.A, n, t.V, t.w, pk);
+    // algorithmic bytes: the trailing L x L matrix read (and written back
+    // when an update is pending)
+    if (prof) prof_after(s, 8.0 * L * (double)L * (k >= 1 ? 2.0 : 1.0));
     trd_reflector_kernel<<<1, 1024, 0, s>>>(n, k, t.A, n, t.V, t.tau, t.d, t.e, t.w, pk, pn);
   }
   MSDP_HIP(hipGetLastError());

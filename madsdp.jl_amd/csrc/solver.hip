@@ -1178,6 +1178,15 @@ int msdp_solver_iterate(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->dev
 int msdp_solver_update_termination(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->update_termination(); }
 int msdp_solver_solve(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->solve(); }
 int msdp_solver_synchronize(msdp_solver* s) { GUARD(s); MSDP_HIP(hipStreamSynchronize(s->st)); return 0; }
+int msdp_solver_profile_begin(msdp_solver* s, int32_t cls) {
+  GUARD(s);
+  MSDP_HIP(hipStreamSynchronize(s->st));
+  return prof_begin(cls, s->st);
+}
+int msdp_solver_profile_end(msdp_solver* s, double* ms, int64_t* launches, double* work) {
+  GUARD(s);
+  return prof_end(ms, launches, work);
+}
 
 int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
   GUARD(s);

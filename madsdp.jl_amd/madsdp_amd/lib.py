@@ -73,6 +73,8 @@ SIGNATURES = {
     "msdp_solver_get": (C.c_int, [P, C.c_int32, C.c_int32, c_double_p]),
     "msdp_solver_set": (C.c_int, [P, C.c_int32, C.c_int32, c_double_p]),
     "msdp_solver_synchronize": (C.c_int, [P]),
+    "msdp_solver_profile_begin": (C.c_int, [P, C.c_int32]),
+    "msdp_solver_profile_end": (C.c_int, [P, c_double_p, c_int64_p, c_double_p]),
     "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
     "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P]),
