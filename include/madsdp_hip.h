@@ -150,7 +150,9 @@ int msdp_k_gemm(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t
                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                 double beta, double* C, int64_t ldc);
 /* lower Cholesky in place; *info_dev = 0 or first failing pivot (LAPACK potrf) */
-int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info_dev);
+int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info_dev,
+                 double* work);
+size_t msdp_k_potrf_work(int32_t n);
 /* _cholesky_factor! (blocks.jl:94-113): L = chol((M+M')/2) + flush; returns the
  * info through *info_dev */
 int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm, double* L,

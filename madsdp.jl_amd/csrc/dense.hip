@@ -109,6 +109,184 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(
   }
 }
 
+// Large-tile GEMM: 128x128 output tile per workgroup of 4 waves, each wave
+// a 64x64 quadrant of 4x4 v_mfma_f64_16x16x4 tiles (16 accumulators).  The
+// next 16-deep K slab is fetched into registers while the current one is
+// multiplied out of LDS (double-buffered, one barrier per slab).
+constexpr int BT = 128;
+constexpr int BK = 16;
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
+    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
+    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
+  if (guard && *guard != 0) return;
+  const int tm = blockIdx.x, tn = blockIdx.y;
+  if (lower && tn > tm) return;
+  const int64_t bz = blockIdx.z;
+  A += bz * sA;
+  B += bz * sB;
+  C += bz * sC;
+  __shared__ double As[2][BK][BT + 2];
+  __shared__ double Bs[2][BK][BT + 2];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = tm * BT, n0 = tn * BT;
+  double ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = tid + r * 256;
+      int mm, kk;
+      if (!TA) { mm = e & 127; kk = e >> 7; } else { kk = e & 15; mm = e >> 4; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      ra[r] = (gm < M && gk < K) ? (TA ? A[gk + (int64_t)gm * lda] : A[gm + (int64_t)gk * lda]) : 0.0;
+      int nn;
+      if (!TB) { kk = e & 15; nn = e >> 4; } else { nn = e & 127; kk = e >> 7; }
+      const int gn = n0 + nn, gk2 = k0 + kk;
+      rb[r] = (gn < N && gk2 < K) ? (TB ? B[gn + (int64_t)gk2 * ldb] : B[gk2 + (int64_t)gn * ldb]) : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = tid + r * 256;
+      int mm, kk;
+      if (!TA) { mm = e & 127; kk = e >> 7; } else { kk = e & 15; mm = e >> 4; }
+      As[buf][kk][mm] = ra[r];
+      int nn;
+      if (!TB) { kk = e & 15; nn = e >> 4; } else { nn = e & 127; kk = e >> 7; }
+      Bs[buf][kk][nn] = rb[r];
+    }
+  };
+  v4d acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+  const int nk = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        av[q] = As[buf][kk + lk][wm + 16 * q + li];
+        bv[q] = Bs[buf][kk + lk][wn + 16 * q + li];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    if (kt + 1 < nk) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int gn = n0 + wn + 16 * b + li;
+      if (gn >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * a + lk + 4 * r;
+        if (gm >= M || (lower && gm < gn)) continue;
+        double* c = C + gm + (int64_t)gn * ldc;
+        const double v = alpha * acc[a][b][r];
+        *c = beta == 0.0 ? v : v + beta * (*c);
+      }
+    }
+  }
+}
+
+// Split-K GEMM for skinny outputs with a long K (few output tiles): each
+// blockIdx.z computes the partial product over one K chunk into P; a second
+// kernel sums the chunks (fixed order, deterministic) into C.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, int kc,
+                                                          const double* __restrict__ A,
+                                                          int64_t lda,
+                                                          const double* __restrict__ B,
+                                                          int64_t ldb, double* __restrict__ P) {
+  const int tm = blockIdx.x, tn = blockIdx.y, part = blockIdx.z;
+  const int kbeg = part * kc, kend = min(K, kbeg + kc);
+  __shared__ double As[GK][GT + 1];
+  __shared__ double Bs[GK][GT + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  v4d acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+  const int m0 = tm * GT, n0 = tn * GT;
+  for (int k0 = kbeg; k0 < kend; k0 += GK) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * 256;
+      int mm, kk;
+      if (!TA) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      double v = 0.0;
+      if (gm < M && gk < kend) v = TA ? A[gk + (int64_t)gm * lda] : A[gm + (int64_t)gk * lda];
+      As[kk][mm] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * 256;
+      int nn, kk;
+      if (!TB) { kk = e & 15; nn = e >> 4; } else { nn = e & 63; kk = e >> 6; }
+      const int gn = n0 + nn, gk = k0 + kk;
+      double v = 0.0;
+      if (gn < N && gk < kend) v = TB ? B[gn + (int64_t)gk * ldb] : B[gk + (int64_t)gn * ldb];
+      Bs[kk][nn] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 4) {
+      const double a0 = As[kk + lk][wm + li];
+      const double a1 = As[kk + lk][wm + 16 + li];
+      const double b0 = Bs[kk + lk][wn + li];
+      const double b1 = Bs[kk + lk][wn + 16 + li];
+      acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+      acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double* Pp = P + (int64_t)part * M * N;
+  const v4d* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i0 = m0 + wm + 16 * (q >> 1) + lk;
+    const int gn = n0 + wn + 16 * (q & 1) + li;
+    if (gn >= N) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gm = i0 + 4 * r;
+      if (gm < M) Pp[gm + (int64_t)gn * M] = (*accs[q])[r];
+    }
+  }
+}
+
+__global__ void splitk_reduce_kernel(int M, int N, int parts, const double* P, double alpha,
+                                     double beta, double* C, int64_t ldc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)M * N) return;
+  double v = 0.0;
+  for (int q = 0; q < parts; ++q) v += P[e + (int64_t)q * M * N];
+  const int i = (int)(e % M), j = (int)(e / M);
+  double* c = C + i + (int64_t)j * ldc;
+  *c = beta == 0.0 ? alpha * v : alpha * v + beta * (*c);
+}
+
 static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                         double* C, int64_t ldc, int batch, int64_t sA, int64_t sB, int64_t sC,
@@ -126,9 +304,44 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     }
     return 0;
   }
-  dim3 grid(cdiv(M, GT), cdiv(N, GT), batch);
   const bool prof = prof_active(PROF_GEMM, s);
+  const int64_t tiles64 = (int64_t)cdiv(M, GT) * cdiv(N, GT) * batch;
+  if (!lower && !guard && batch == 1 && tiles64 < 128 && K >= 1024) {
+    // split K so that about 256 workgroups run
+    int parts = (int)std::min<int64_t>(std::max<int64_t>(2, 256 / tiles64), K / 256);
+    const int kc = ((K + parts - 1) / parts + GK - 1) / GK * GK;
+    parts = (K + kc - 1) / kc;
+    void* P = nullptr;
+    MSDP_HIP(hipMallocAsync(&P, sizeof(double) * (size_t)M * N * parts, s));
+    if (prof) prof_before(s);
+    dim3 g3(cdiv(M, GT), cdiv(N, GT), parts);
+    double* Pd = static_cast<double*>(P);
+    if (!ta && !tb) gemm_splitk_kernel<false, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
+    else if (ta && !tb) gemm_splitk_kernel<true, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
+    else if (!ta && tb) gemm_splitk_kernel<false, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
+    else gemm_splitk_kernel<true, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
+    splitk_reduce_kernel<<<cdiv((int64_t)M * N, 256), 256, 0, s>>>(M, N, parts, Pd, alpha, beta, C, ldc);
+    if (prof) prof_after(s, 2.0 * M * N * (double)K);
+    MSDP_HIP(hipGetLastError());
+    MSDP_HIP(hipFreeAsync(P, s));
+    return 0;
+  }
   if (prof) prof_before(s);
+  if (M >= BT && N >= BT) {
+    dim3 g2(cdiv(M, BT), cdiv(N, BT), batch);
+    if (!ta && !tb)
+      gemm128_f64_kernel<false, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    else if (ta && !tb)
+      gemm128_f64_kernel<true, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    else if (!ta && tb)
+      gemm128_f64_kernel<false, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    else
+      gemm128_f64_kernel<true, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    if (prof) prof_after(s, 2.0 * M * N * (double)K * batch * (lower ? 0.5 : 1.0));
+    MSDP_HIP(hipGetLastError());
+    return 0;
+  }
+  dim3 grid(cdiv(M, GT), cdiv(N, GT), batch);
   if (!ta && !tb)
     gemm_f64_kernel<false, false><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
   else if (ta && !tb)
@@ -152,93 +365,192 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, con
 // ==========================================================================
 // Cholesky
 // ==========================================================================
-// Factor one diagonal block (n <= 64) in LDS; one workgroup per batch entry.
-// info[b] gets info_base + (1-based failing column) on failure.
-__global__ __launch_bounds__(256) void potrf_small_kernel(int n, double* A, int64_t lda,
-                                                          int64_t stride, int* info,
-                                                          int info_base, int per_batch_info) {
+// Factor one diagonal block (n <= 64) with 256 threads: thread (ti, tc)
+// owns the 4 x 4 register tile of rows ti + 16a and columns tc + 16b (the
+// dynamic tile index is resolved by uniform predicates over static slots,
+// which keeps the tile in registers).  Each
+// column step: the pivot owner takes the square root, the owners of column j
+// scale it and publish it in LDS, every thread updates its tile (16 FMAs).
+// With `dinv`, L^-1 is formed the same way by right-looking forward
+// substitution over rows (64 x 64, ld 64, zero padded).
+// info[b] = info_base + first failing column.
+__global__ __launch_bounds__(256) void chol_diag_kernel(int n, double* A, int64_t lda,
+                                                        int64_t stride, int* info, int info_base,
+                                                        int per_batch_info, double* dinv) {
   int* inf = info + (per_batch_info ? blockIdx.x : 0);
   if (!per_batch_info && *inf != 0) return;
   A += (int64_t)blockIdx.x * stride;
-  __shared__ double Ls[64][65];
+  __shared__ double colv[64];
+  __shared__ double Lc[64][65];   // the factor, for the inverse
+  __shared__ double rdg[64];
   __shared__ int fail;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < n * n; e += 256) {
-    const int i = e % n, j = e / n;
-    Ls[i][j] = i >= j ? A[i + (int64_t)j * lda] : 0.0;
-  }
+  const int tid = threadIdx.x, ti = tid & 15, tc = tid >> 4;
+  double t[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = ti + 16 * a, c = tc + 16 * b;
+      t[a][b] = (i < n && c <= i) ? A[i + (int64_t)c * lda] : 0.0;
+    }
   if (tid == 0) fail = 0;
   __syncthreads();
+#pragma unroll 1
   for (int j = 0; j < n; ++j) {
-    if (tid == 0) {
-      const double d = Ls[j][j];
-      if (!(d > 0.0) || !isfinite(d)) fail = j + 1;
-      else Ls[j][j] = sqrt(d);
+    const int bj = j >> 4, cj = j & 15;
+    if (tc == cj && ti == cj) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (a == bj) {
+          const double d = t[a][a];
+          if (!(d > 0.0) || !isfinite(d)) {
+            fail = j + 1;
+          } else {
+            t[a][a] = sqrt(d);
+            colv[j] = t[a][a];
+            rdg[j] = 1.0 / t[a][a];
+          }
+        }
     }
     __syncthreads();
     if (fail) break;
-    const double djj = Ls[j][j];
-    for (int i = j + 1 + tid; i < n; i += 256) Ls[i][j] /= djj;
-    __syncthreads();
-    const int r = n - j - 1;
-    for (int e = tid; e < r * r; e += 256) {
-      const int i = j + 1 + e % r, k = j + 1 + e / r;
-      if (k <= i) Ls[i][k] -= Ls[i][j] * Ls[k][j];
+    if (tc == cj) {
+      const double ljj = colv[j];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int i = ti + 16 * a;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (b == bj && i > j && i < n) {
+            t[a][b] = t[a][b] / ljj;
+            colv[i] = t[a][b];
+          }
+      }
     }
     __syncthreads();
+    double li[4], lc[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      li[a] = colv[ti + 16 * a];
+      lc[a] = colv[tc + 16 * a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int i = ti + 16 * a, c = tc + 16 * b;
+        if (c > j && i >= c && i < n) t[a][b] -= li[a] * lc[b];
+      }
   }
   if (fail) {
     if (tid == 0) atomicCAS(inf, 0, info_base + fail);
     return;
   }
-  for (int e = tid; e < n * n; e += 256) {
-    const int i = e % n, j = e / n;
-    if (i >= j) A[i + (int64_t)j * lda] = Ls[i][j];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = ti + 16 * a, c = tc + 16 * b;
+      if (i < n && c <= i) A[i + (int64_t)c * lda] = t[a][b];
+      Lc[i][c] = (i < n && c <= i) ? t[a][b] : 0.0;
+    }
+  if (!dinv) return;
+  __syncthreads();
+  // X = L^-1: row k of X is final after rows < k are subtracted; acc starts at I
+  double x[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) x[a][b] = (ti + 16 * a == tc + 16 * b) ? 1.0 : 0.0;
+  __shared__ double xrow[64];
+#pragma unroll 1
+  for (int k = 0; k < n; ++k) {
+    const int bk = k >> 4;
+    if (ti == (k & 15)) {
+      const double r = rdg[k];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (a == bk) {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            x[a][b] *= r;
+            xrow[tc + 16 * b] = x[a][b];
+          }
+        }
+    }
+    __syncthreads();
+    double lr[4], xr[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      lr[a] = Lc[ti + 16 * a][k];
+      xr[a] = xrow[tc + 16 * a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (ti + 16 * a > k) x[a][b] -= lr[a] * xr[b];
+    __syncthreads();
   }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = ti + 16 * a, c = tc + 16 * b;
+      dinv[i + 64 * c] = (i < n && c < n && c <= i) ? x[a][b] : 0.0;
+    }
 }
 
-// A21 <- A21 * L11^-T for a 64-wide panel; one thread per row.
-__global__ __launch_bounds__(128) void trsm_panel_kernel(int rows, double* A21, int64_t lda,
-                                                         const double* L11, const int* info) {
+// A21 <- A21 * L11^-T in place, with L11^-1 from chol_diag_kernel; each
+// workgroup owns 64 rows.
+__global__ __launch_bounds__(256) void chol_panel_kernel(int rows, double* A21, int64_t lda,
+                                                         const double* dinv, const int* info) {
   if (*info != 0) return;
-  __shared__ double Ls[64][65];
-  __shared__ double dinv[64];
-  for (int e = threadIdx.x; e < 64 * 64; e += 128) {
-    const int i = e & 63, j = e >> 6;
-    Ls[i][j] = i >= j ? L11[i + (int64_t)j * lda] : 0.0;
+  __shared__ double T[64][65];
+  __shared__ double U[64][65];
+  const int r0 = blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e & 63, k = e >> 6;
+    T[r][k] = r0 + r < rows ? A21[(r0 + r) + (int64_t)k * lda] : 0.0;
+    U[r][k] = dinv[r + 64 * k];
   }
   __syncthreads();
-  if (threadIdx.x < 64) dinv[threadIdx.x] = 1.0 / Ls[threadIdx.x][threadIdx.x];
-  __syncthreads();
-  const int r = blockIdx.x * 128 + threadIdx.x;
-  if (r >= rows) return;
-  double x[64];
+  const int rr = tid & 15, cc = tid >> 4;
+  double acc[4][4] = {};
+#pragma unroll 8
+  for (int k = 0; k < 64; ++k) {
+    double tv[4], uv[4];
 #pragma unroll
-  for (int c = 0; c < 64; ++c) x[c] = A21[r + (int64_t)c * lda];
+    for (int q = 0; q < 4; ++q) { tv[q] = T[rr + 16 * q][k]; uv[q] = U[cc + 16 * q][k]; }
 #pragma unroll
-  for (int c = 0; c < 64; ++c) {
-    double v = x[c];
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int k = 0; k < c; ++k) v -= x[k] * Ls[c][k];
-    x[c] = v * dinv[c];
+      for (int b = 0; b < 4; ++b) acc[a][b] += tv[a] * uv[b];
   }
 #pragma unroll
-  for (int c = 0; c < 64; ++c) A21[r + (int64_t)c * lda] = x[c];
+  for (int a = 0; a < 4; ++a) {
+    const int r = r0 + rr + 16 * a;
+    if (r >= rows) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) A21[r + (int64_t)(cc + 16 * b) * lda] = acc[a][b];
+  }
 }
 
-size_t potrf_work_elems(int) { return 0; }
+size_t potrf_work_elems(int n) { return (size_t)cdiv(std::max(n, 1), 64) * 4096; }
 
-int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double*) {
+int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work) {
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int jb = std::min(64, n - j0);
     double* A11 = A + j0 + (int64_t)j0 * lda;
-    potrf_small_kernel<<<1, 256, 0, s>>>(jb, A11, lda, 0, info, j0, 0);
+    double* dinv = work + (size_t)(j0 / 64) * 4096;
+    chol_diag_kernel<<<1, 256, 0, s>>>(jb, A11, lda, 0, info, j0, 0, dinv);
     MSDP_HIP(hipGetLastError());
     const int rest = n - j0 - jb;
     if (rest > 0) {
       double* A21 = A11 + jb;
-      trsm_panel_kernel<<<cdiv(rest, 128), 128, 0, s>>>(rest, A21, lda, A11, info);
+      chol_panel_kernel<<<cdiv(rest, 64), 256, 0, s>>>(rest, A21, lda, dinv, info);
       MSDP_HIP(hipGetLastError());
       double* A22 = A21 + (int64_t)jb * lda;
       MSDP_TRY(gemm_guarded(s, false, true, rest, rest, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
@@ -252,7 +564,7 @@ int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t st
                         int* info) {
   if (n > 64) { set_error("potrf_small_batched: n > 64"); return -2; }
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * batch, s));
-  potrf_small_kernel<<<batch, 256, 0, s>>>(n, A, lda, stride, info, 0, 1);
+  chol_diag_kernel<<<batch, 256, 0, s>>>(n, A, lda, stride, info, 0, 1, nullptr);
   MSDP_HIP(hipGetLastError());
   return 0;
 }

@@ -149,128 +149,15 @@ int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t s
 // ==========================================================================
 // Householder tridiagonalisation (lower), A = Q T Q',  Q = H_0 ... H_{n-3}
 // ==========================================================================
-// Reflector kernel for step k (k = -1 for the first): forms w_k from p_k,
-// then the column j = k+1 of the updated matrix, its reflector v_j (stored
-// as column j of V, zero above row j+1) and d_j, e_j, tau_j.  It also zeroes
-// the accumulation buffer of the next product.
-__global__ __launch_bounds__(1024) void trd_reflector_kernel(int n, int k, const double* A,
-                                                             int64_t lda, double* V, double* tau,
-                                                             double* d, double* e, double* w,
-                                                             const double* p, double* pnext) {
-  __shared__ double red[16];
-  __shared__ double s_wj, s_vj, s_alpha, s_beta, s_tau, s_scale;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int j = k + 1;
-  const double* vk = k >= 0 ? V + (int64_t)k * n : nullptr;
-  // w_k = tau p - (tau^2/2)(p'v) v  over indices >= k+1
-  if (k >= 0) {
-    const double tk = tau[k];
-    double sdot = 0.0;
-    for (int i = k + 1 + tid; i < n; i += nt) sdot += p[i] * vk[i];
-    sdot = block_reduce<0>(sdot, red);
-    const double c2 = 0.5 * tk * tk * sdot;
-    for (int i = tid; i < n; i += nt) w[i] = i >= k + 1 ? tk * p[i] - c2 * vk[i] : 0.0;
-    __syncthreads();
-    if (tid == 0) { s_wj = w[j]; s_vj = vk[j]; }
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += nt) pnext[i] = 0.0;
-  // column j of the updated matrix, rows >= j
-  auto col = [&](int i) -> double {
-    double c = A[i + (int64_t)j * lda];
-    if (k >= 0) c -= vk[i] * s_wj + w[i] * s_vj;
-    return c;
-  };
-  if (tid == 0) d[j] = col(j);
-  if (j == n - 2) {
-    if (tid == 0) {
-      e[j] = col(n - 1);
-      tau[j] = 0.0;
-      double dl = A[(n - 1) + (int64_t)(n - 1) * lda];
-      if (k >= 0) dl -= 2.0 * vk[n - 1] * w[n - 1];
-      d[n - 1] = dl;
-    }
-    return;
-  }
-  if (j >= n - 1) return;
-  // Householder of x = col(j+1 .. n-1)
-  double ss = 0.0;
-  for (int i = j + 2 + tid; i < n; i += nt) {
-    const double c = col(i);
-    ss += c * c;
-  }
-  ss = block_reduce<0>(ss, red);
-  if (tid == 0) {
-    const double alpha = col(j + 1);
-    double beta, t, sc;
-    if (ss == 0.0) {
-      beta = alpha; t = 0.0; sc = 0.0;
-    } else {
-      beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-      t = (beta - alpha) / beta;
-      sc = 1.0 / (alpha - beta);
-    }
-    s_alpha = alpha; s_beta = beta; s_tau = t; s_scale = sc;
-    tau[j] = t;
-    e[j] = beta;
-  }
-  __syncthreads();
-  double* vj = V + (int64_t)j * n;
-  const double sc = s_scale;
-  for (int i = tid; i < n; i += nt) {
-    double v;
-    if (i <= j) v = 0.0;
-    else if (i == j + 1) v = 1.0;
-    else v = col(i) * sc;
-    vj[i] = v;
-  }
-}
-
-// Fused kernel of step k: applies the pending update of step k-1,
-// A <- A - v w' - w v' on [k+1, n)^2, and accumulates p_k = A v_k there.
-// One thread per row, 64 columns per workgroup.
-__global__ __launch_bounds__(256) void trd_update_product_kernel(int n, int k, double* A,
-                                                                 int64_t lda, const double* V,
-                                                                 const double* w, double* p) {
-  const int base = k + 1;
-  const int L = n - base;
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  const int c0 = blockIdx.y * 64;
-  __shared__ double sv[64], sw[64], sn[64];
-  const double* vprev = k >= 1 ? V + (int64_t)(k - 1) * n : nullptr;
-  const double* vk = V + (int64_t)k * n;
-  if (threadIdx.x < 64) {
-    const int c = base + c0 + threadIdx.x;
-    const bool ok = c0 + (int)threadIdx.x < L;
-    sv[threadIdx.x] = (ok && vprev) ? vprev[c] : 0.0;
-    sw[threadIdx.x] = (ok && vprev) ? w[c] : 0.0;
-    sn[threadIdx.x] = ok ? vk[c] : 0.0;
-  }
-  __syncthreads();
-  if (r >= L) return;
-  const int i = base + r;
-  const double vi = vprev ? vprev[i] : 0.0, wi = vprev ? w[i] : 0.0;
-  const int cend = min(64, L - c0);
-  double acc = 0.0;
-  double* colp = A + i + (int64_t)(base + c0) * lda;
-  if (vprev) {
-    for (int cc = 0; cc < cend; ++cc) {
-      double a = colp[(int64_t)cc * lda];
-      a -= vi * sw[cc] + wi * sv[cc];
-      colp[(int64_t)cc * lda] = a;
-      acc += a * sn[cc];
-    }
-  } else {
-    for (int cc = 0; cc < cend; ++cc) acc += colp[(int64_t)cc * lda] * sn[cc];
-  }
-  atomicAdd(p + i, acc);
-}
-
 struct TrdWork {
-  double *A, *V, *tau, *d, *e, *w, *p0, *p1;
+  double *A, *V, *tau, *d, *e, *w, *p0, *p1, *W, *hv, *part;
 };
 
-static size_t trd_work_elems(int n) { return 2 * (size_t)n * n + 7 * (size_t)n; }
+constexpr int TRD_NB = 32;
+
+static size_t trd_work_elems(int n) {
+  return 2 * (size_t)n * n + 8 * (size_t)n + (size_t)n * TRD_NB + 2 * TRD_NB + 64;
+}
 
 static TrdWork trd_carve(int n, double* work) {
   TrdWork t;
@@ -282,29 +169,210 @@ static TrdWork trd_carve(int n, double* work) {
   t.w = t.e + n;
   t.p0 = t.w + n;
   t.p1 = t.p0 + n;
+  t.W = t.p1 + n;
+  t.hv = t.W + (size_t)n * TRD_NB;  // hv[0:nb], hw[nb:2nb]
+  t.part = t.hv + 2 * TRD_NB;       // per-workgroup partial dots (<= n / 256 + 1)
   return t;
+}
+
+// Blocked tridiagonalisation (LAPACK's latrd scheme).  Within a panel of
+// TRD_NB columns the trailing matrix is not updated; column k is corrected on
+// the fly by the panel's reflectors V and their W = tau (A v - ...) vectors,
+// and after the panel A -= V W' + W V' is applied by two MFMA GEMMs.
+//
+// Column step of panel column j (global column k = k0 + j; k < 0: only the
+// completion of panel column j-1), in two kernels:
+//  trd_rows_kernel (grid, one row per thread): the corrections that involve
+//    the panel columns t < j-1:
+//      pc[i]   = p[i] - sum_t V[i,t] hw[t] + W[i,t] hv[t]      (A v_{k-1} now)
+//      base[i] = A[i,k] - sum_t V[i,t] W[k,t] + W[i,t] V[k,t]  (column k)
+//    and partial sums of pc . v_{k-1}.
+//  trd_reflector_kernel (one workgroup, O(n)): w_{k-1} = tau pc - c v_{k-1}
+//    (column j-1 of W), the last correction of column k with it, then the
+//    Householder reflector of column k (V[:, k]), d_k, e_k, tau_k.
+__global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int k0,
+                                                       const double* A, const double* V,
+                                                       const double* W, const double* p,
+                                                       const double* hv, double* pc,
+                                                       double* base, double* part) {
+  __shared__ double rowW[TRD_NB], rowV[TRD_NB];
+  __shared__ double red[4];
+  const int prev = j - 1;
+  const int start = j > 0 ? k0 + j : k;
+  const double* hw = hv + TRD_NB;
+  const int ncor = j > 0 ? prev : 0;
+  if (k >= 0 && threadIdx.x < ncor) {
+    rowW[threadIdx.x] = W[k + (int64_t)threadIdx.x * n];
+    rowV[threadIdx.x] = V[k + (int64_t)(k0 + threadIdx.x) * n];
+  }
+  __syncthreads();
+  const int i = start + blockIdx.x * 256 + threadIdx.x;
+  double sd = 0.0;
+  if (i < n) {
+    double pcv = j > 0 ? p[i] : 0.0;
+    double bv = k >= 0 ? A[i + (int64_t)k * n] : 0.0;
+    for (int t = 0; t < ncor; ++t) {
+      const double vt = V[i + (int64_t)(k0 + t) * n], wt = W[i + (int64_t)t * n];
+      pcv -= vt * hw[t] + wt * hv[t];
+      if (k >= 0) bv -= vt * rowW[t] + wt * rowV[t];
+    }
+    if (j > 0) {
+      pc[i] = pcv;
+      sd = pcv * V[i + (int64_t)(k0 + prev) * n];
+    }
+    if (k >= 0) base[i] = bv;
+  }
+  sd = block_reduce<0>(sd, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = sd;
+}
+
+__global__ __launch_bounds__(1024) void trd_reflector_kernel(
+    int n, int k, int j, int k0, int nparts, const double* A, double* V, double* W,
+    const double* pc, const double* base, const double* part, double* tau, double* d,
+    double* e) {
+  __shared__ double red[16];
+  __shared__ double s_scale, s_wk, s_vk;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int prev = j - 1;
+  const int kp = k0 + prev;
+  const double* vp = j > 0 ? V + (int64_t)kp * n : nullptr;
+  double* wp = j > 0 ? W + (int64_t)prev * n : nullptr;
+  if (j > 0) {
+    double sd = 0.0;
+    for (int q = tid; q < nparts; q += nt) sd += part[q];
+    sd = block_reduce<0>(sd, red);
+    const double tp = tau[kp];
+    const double c2 = 0.5 * tp * tp * sd;
+    for (int i = tid; i < n; i += nt) wp[i] = i > kp ? tp * pc[i] - c2 * vp[i] : 0.0;
+    if (tid == 0 && k >= 0) {
+      s_wk = tp * pc[k] - c2 * vp[k];
+      s_vk = vp[k];
+    }
+    __syncthreads();
+  }
+  if (k < 0) return;
+  auto col = [&](int i) -> double {
+    double c = base[i];
+    if (j > 0) c -= vp[i] * s_wk + wp[i] * s_vk;
+    return c;
+  };
+  if (tid == 0) d[k] = col(k);
+  if (k == n - 2) {
+    if (tid == 0) {
+      e[k] = col(n - 1);
+      tau[k] = 0.0;
+      d[n - 1] = A[(n - 1) + (int64_t)(n - 1) * n];
+    }
+    for (int i = tid; i < n; i += nt) V[i + (int64_t)k * n] = 0.0;
+    return;
+  }
+  double ss = 0.0;
+  for (int i = k + 2 + tid; i < n; i += nt) {
+    const double c = col(i);
+    ss += c * c;
+  }
+  ss = block_reduce<0>(ss, red);
+  if (tid == 0) {
+    const double alpha = col(k + 1);
+    double beta, t, sc;
+    if (ss == 0.0) {
+      beta = alpha; t = 0.0; sc = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+      t = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    s_scale = sc;
+    tau[k] = t;
+    e[k] = beta;
+  }
+  __syncthreads();
+  double* vk = V + (int64_t)k * n;
+  const double sc = s_scale;
+  for (int i = tid; i < n; i += nt) {
+    double v;
+    if (i <= k) v = 0.0;
+    else if (i == k + 1) v = 1.0;
+    else v = col(i) * sc;
+    vk[i] = v;
+  }
+}
+
+static int trd_column(hipStream_t s, int n, int k, int j, int k0, const TrdWork& t) {
+  const int start = j > 0 ? k0 + j : k;
+  const int nparts = std::max(1, cdiv(n - start, 256));
+  trd_rows_kernel<<<nparts, 256, 0, s>>>(n, k, j, k0, t.A, t.V, t.W, t.p0, t.hv, t.w, t.p1,
+                                          t.part);
+  trd_reflector_kernel<<<1, 1024, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.w, t.p1, t.part,
+                                          t.tau, t.d, t.e);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// p[c] = A[base:, c]' v (c >= base) for the trailing matrix (A symmetric,
+// full storage), one wavefront per column; then hv = V_panel' v and
+// hw = W_panel' v for the j panel columns before k.  Deterministic.
+__global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, int j, int k0,
+                                                                const double* A,
+                                                                const double* V,
+                                                                const double* W, double* p,
+                                                                double* hv) {
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int base = k + 1;
+  const int L = n - base;
+  const double* v = V + (int64_t)k * n;
+  const double* colp;
+  double* out;
+  if (gw < L) {
+    colp = A + (int64_t)(base + gw) * n;
+    out = p + base + gw;
+  } else if (gw < L + j) {
+    colp = V + (int64_t)(k0 + gw - L) * n;
+    out = hv + (gw - L);
+  } else if (gw < L + 2 * j) {
+    colp = W + (int64_t)(gw - L - j) * n;
+    out = hv + TRD_NB + (gw - L - j);
+  } else {
+    return;
+  }
+  double acc = 0.0;
+  for (int i = base + lane; i < n; i += 64) acc += colp[i] * v[i];
+  acc = wave_sum(acc);
+  if (lane == 0) *out = acc;
 }
 
 static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWork& t) {
   MSDP_TRY(copy_mat(s, n, n, A, lda, t.A, n));
   MSDP_TRY(fill(s, (int64_t)n * n, 0.0, t.V));
-  // k = -1: first reflector from column 0
-  trd_reflector_kernel<<<1, 1024, 0, s>>>(n, -1, t.A, n, t.V, t.tau, t.d, t.e, t.w, t.p1, t.p0);
-  MSDP_HIP(hipGetLastError());
-  for (int k = 0; k <= n - 3; ++k) {
-    double* pk = (k & 1) ? t.p1 : t.p0;
-    double* pn = (k & 1) ? t.p0 : t.p1;
-    const int L = n - k - 1;
-    dim3 g(cdiv(L, 256), cdiv(L, 64));
-    const bool prof = prof_active(PROF_TRD, s);
-    if (prof) prof_before(s);
-    trd_update_product_kernel<<<g, 256, 0, s>>>(n, k, t.A, n, t.V, t.w, pk);
-    // algorithmic bytes: the trailing L x L matrix read (and written back
-    // when an update is pending)
-    if (prof) prof_after(s, 8.0 * L * (double)L * (k >= 1 ? 2.0 : 1.0));
-    trd_reflector_kernel<<<1, 1024, 0, s>>>(n, k, t.A, n, t.V, t.tau, t.d, t.e, t.w, pk, pn);
+  const int last = n - 3;  // last column with a reflector
+  for (int k0 = 0; k0 <= last; k0 += TRD_NB) {
+    const int jb = std::min(TRD_NB, last - k0 + 1);
+    for (int j = 0; j < jb; ++j) {
+      const int k = k0 + j;
+      MSDP_TRY(trd_column(s, n, k, j, k0, t));
+      const int L = n - k - 1;
+      const bool prof = prof_active(PROF_TRD, s);
+      if (prof) prof_before(s);
+      trd_panel_product_kernel<<<cdiv(L + 2 * j, 4), 256, 0, s>>>(n, k, j, k0, t.A, t.V, t.W,
+                                                                   t.p0, t.hv);
+      // algorithmic bytes: the trailing L x L matrix read once
+      if (prof) prof_after(s, 8.0 * L * (double)L);
+    }
+    // w of the panel's last column
+    MSDP_TRY(trd_column(s, n, -1, jb, k0, t));
+    const int r0 = k0 + jb;
+    const int L2 = n - r0;
+    if (L2 > 0) {
+      const double* Vp = t.V + r0 + (int64_t)k0 * n;
+      const double* Wp = t.W + r0;
+      double* A22 = t.A + r0 + (int64_t)r0 * n;
+      MSDP_TRY(gemm(s, false, true, L2, L2, jb, -1.0, Vp, n, Wp, n, 1.0, A22, n));
+      MSDP_TRY(gemm(s, false, true, L2, L2, jb, -1.0, Wp, n, Vp, n, 1.0, A22, n));
+    }
   }
-  MSDP_HIP(hipGetLastError());
+  // the last 2 x 2 block: d_{n-2}, e_{n-2}, d_{n-1} (no reflector)
+  MSDP_TRY(trd_column(s, n, n - 2, 0, std::max(0, n - 2), t));
   return 0;
 }
 
@@ -417,6 +485,8 @@ struct DcVec {
   double *rot_c, *rot_s;
 };
 
+constexpr int DC_LDS_MAX = 4096;
+
 // M1: gather z, merge-sort the children's eigenvalues, deflate (one thread).
 __global__ __launch_bounds__(1024) void dc_deflate_kernel(const DcNode* nodes, const double* lam,
                                                           const double* e, const double* Z,
@@ -452,7 +522,16 @@ __global__ __launch_bounds__(1024) void dc_deflate_kernel(const DcNode* nodes, c
     src[rank] = c;
   }
   __syncthreads();
-  if (tid != 0) return;
+  // the serial scan below reads and rewrites ds/zs; keep them in LDS
+  __shared__ double lds_ds[DC_LDS_MAX], lds_zs[DC_LDS_MAX];
+  const bool use_lds = K <= DC_LDS_MAX;
+  if (use_lds)
+    for (int c = tid; c < K; c += nt) { lds_ds[c] = ds[c]; lds_zs[c] = zs[c]; }
+  __syncthreads();
+  double* gds = ds;
+  double* gzs = zs;
+  if (use_lds) { ds = lds_ds; zs = lds_zs; }
+  if (tid == 0) {
   double dmax = 0.0, zmax = 0.0;
   for (int r = 0; r < K; ++r) { dmax = fmax(dmax, fabs(ds[r])); zmax = fmax(zmax, fabs(zs[r])); }
   const double tol = 8.0 * DEPS * fmax(dmax, zmax);
@@ -505,23 +584,33 @@ __global__ __launch_bounds__(1024) void dc_deflate_kernel(const DcNode* nodes, c
   v.kp[blockIdx.x] = kp;
   v.nrot[blockIdx.x] = nr;
   v.rho[blockIdx.x] = rho;
+  }
+  __syncthreads();
+  if (use_lds)
+    for (int c = tid; c < K; c += nt) { gds[c] = lds_ds[c]; gzs[c] = lds_zs[c]; }
 }
 
 // M2: gather the children's eigenvector columns into the node's K x K
-// matrix Qg in final order (non-deflated first), applying the deflation
-// rotations.  One thread per row of the node.
+// matrix Qg in final order (non-deflated first): one thread per element;
+// then the deflation rotations, applied row by row in their scan order.
 __global__ __launch_bounds__(256) void dc_gather_kernel(const DcNode* nodes, const double* Z,
                                                         int64_t ldz, DcVec v, double* Qg) {
-  const DcNode nd = nodes[blockIdx.y];
+  const DcNode nd = nodes[blockIdx.z];
   const int lo = nd.lo, K = nd.hi - nd.lo;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= K) return;
-  double* Q = Qg + nd.qoff;
-  const int* src = v.src + lo;
-  const int* fpos = v.fpos + lo;
-  for (int r = 0; r < K; ++r)
-    Q[i + (int64_t)fpos[r] * K] = Z[(lo + i) + (int64_t)(lo + src[r]) * ldz];
+  const int r = blockIdx.y;
+  if (i >= K || r >= K) return;
+  Qg[nd.qoff + i + (int64_t)v.fpos[lo + r] * K] = Z[(lo + i) + (int64_t)(lo + v.src[lo + r]) * ldz];
+}
+
+__global__ __launch_bounds__(256) void dc_rotate_kernel(const DcNode* nodes, DcVec v, double* Qg) {
+  const DcNode nd = nodes[blockIdx.y];
+  const int lo = nd.lo, K = nd.hi - nd.lo;
   const int nr = v.nrot[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (nr == 0 || i >= K) return;
+  double* Q = Qg + nd.qoff;
+  const int* fpos = v.fpos + lo;
   for (int t = 0; t < nr; ++t) {
     const int a = fpos[v.rot_a[lo + t]], b = fpos[v.rot_b[lo + t]];
     const double c = v.rot_c[lo + t], s = v.rot_s[lo + t];
@@ -646,24 +735,31 @@ __global__ __launch_bounds__(256) void dc_vectors_kernel(const DcNode* nodes,
   for (int i = threadIdx.x; i < kp; i += 256) U[i] *= inv;
 }
 
-// M7: final sort of the node's eigenvalues (rank counting) and scatter of the
-// eigenvectors into Z's diagonal block.  First kernel: ranks.
-__global__ __launch_bounds__(1024) void dc_rank_kernel(const DcNode* nodes, DcVec v, double* lam,
-                                                       int* colsrc) {
-  const DcNode nd = nodes[blockIdx.x];
+// M7: final sort of the node's eigenvalues (rank counting, values staged in
+// LDS per workgroup) and scatter of the eigenvectors into Z's diagonal block.
+__global__ __launch_bounds__(256) void dc_rank_kernel(const DcNode* nodes, DcVec v, double* lam,
+                                                      int* colsrc) {
+  const DcNode nd = nodes[blockIdx.y];
   const int lo = nd.lo, K = nd.hi - nd.lo;
-  const int kp = v.kp[blockIdx.x];
-  // value of final column q: q < kp -> lamv[q], else deflated ds
-  for (int q = threadIdx.x; q < K; q += blockDim.x) {
-    const double val = q < kp ? v.lamv[lo + q] : v.ds[lo + v.defl[lo + q - kp]];
-    int r = 0;
-    for (int t = 0; t < K; ++t) {
-      const double vt = t < kp ? v.lamv[lo + t] : v.ds[lo + v.defl[lo + t - kp]];
-      r += (vt < val) || (vt == val && t < q);
-    }
-    v.zs[lo + r] = val;  // zs reused as the sorted eigenvalue buffer
-    colsrc[lo + r] = q;
+  const int kp = v.kp[blockIdx.y];
+  __shared__ double vals[DC_LDS_MAX];
+  const bool use_lds = K <= DC_LDS_MAX;
+  auto value = [&](int t) {
+    return t < kp ? v.lamv[lo + t] : v.ds[lo + v.defl[lo + t - kp]];
+  };
+  if (use_lds)
+    for (int t = threadIdx.x; t < K; t += blockDim.x) vals[t] = value(t);
+  __syncthreads();
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= K) return;
+  const double val = use_lds ? vals[q] : value(q);
+  int r = 0;
+  for (int t = 0; t < K; ++t) {
+    const double vt = use_lds ? vals[t] : value(t);
+    r += (vt < val) || (vt == val && t < q);
   }
+  v.zs[lo + r] = val;  // zs reused as the sorted eigenvalue buffer
+  colsrc[lo + r] = q;
 }
 
 __global__ __launch_bounds__(256) void dc_scatter_kernel(const DcNode* nodes, DcVec v,
@@ -768,7 +864,8 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
     dc_deflate_kernel<<<nn, 1024, 0, s>>>(nodes_dev, lam, e, Z, n, v);
     int maxK = 0;
     for (const auto& nd : level) maxK = std::max(maxK, nd.hi - nd.lo);
-    dc_gather_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes_dev, Z, n, v, Qg);
+    dc_gather_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes_dev, Z, n, v, Qg);
+    dc_rotate_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes_dev, v, Qg);
     MSDP_HIP(hipGetLastError());
     std::vector<int> kp(nn);
     MSDP_HIP(hipMemcpyAsync(kp.data(), v.kp, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
@@ -801,7 +898,7 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
       }
       MSDP_HIP(hipStreamSynchronize(s));  // keep `both` alive until consumed
     }
-    dc_rank_kernel<<<nn, 1024, 0, s>>>(nodes_dev, v, lam, colsrc);
+    dc_rank_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes_dev, v, lam, colsrc);
     dc_scatter_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes_dev, v, Qg, Qn, colsrc,
                                                                        lam, Z, n);
     MSDP_HIP(hipGetLastError());
@@ -818,15 +915,19 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
 __global__ __launch_bounds__(64) void larft_kernel(int nb, const double* G, const double* tau,
                                                    double* T) {
   __shared__ double Ts[64][65];
+  __shared__ double Gs[64][65];
   const int t = threadIdx.x;
-  for (int e = t; e < 64 * 64; e += 64) Ts[e & 63][e >> 6] = 0.0;
+  for (int e = t; e < 64 * 64; e += 64) {
+    Ts[e & 63][e >> 6] = 0.0;
+    Gs[e & 63][e >> 6] = (e & 63) < nb && (e >> 6) < nb ? G[(e & 63) + (int64_t)(e >> 6) * nb] : 0.0;
+  }
   __syncthreads();
   for (int i = 0; i < nb; ++i) {
     const double ti = tau[i];
     // column i: x = G[0:i, i]; T[0:i, i] = -ti * T[0:i,0:i] x
     if (t < i) {
       double v = 0.0;
-      for (int k = t; k < i; ++k) v += Ts[t][k] * G[k + (int64_t)i * nb];
+      for (int k = t; k < i; ++k) v += Ts[t][k] * Gs[k][i];
       Ts[t][i] = -ti * v;
     }
     if (t == 0) Ts[i][i] = ti;
@@ -864,8 +965,11 @@ static int ormtr(hipStream_t s, int n, const double* V, const double* tau, doubl
 // ==========================================================================
 // public entry points
 // ==========================================================================
+size_t eig_min_work_elems(int n);
 size_t syevd_work_elems(int n) {
-  return trd_work_elems(n) + dc_work_elems(n) + (size_t)n * n + 2 * 64 * 64 + 2 * 64 * (size_t)n + 64;
+  return std::max(trd_work_elems(n) + dc_work_elems(n) + (size_t)n * n + 2 * 64 * 64 +
+                      2 * 64 * (size_t)n + 64,
+                  eig_min_work_elems(n));
 }
 size_t syevd_iwork_elems(int n) { return dc_iwork_elems(n); }
 
@@ -891,6 +995,251 @@ int syevd(hipStream_t s, int n, const double* A, int64_t lda, double* w, double*
   return 0;
 }
 
+// ==========================================================================
+// Lanczos for lambda_min of an explicit symmetric matrix (the step-length
+// eigenvalue of src/step.jl:180-184, which the reference obtains with
+// LowRankOpt's Lanczos).  Full reorthogonalisation (classical Gram-Schmidt
+// applied twice); the Ritz value is accepted once the residual bound
+// beta_j |y_last| of the smallest Ritz pair is below LZ_RTOL |theta|, when
+// the Krylov space becomes invariant, or when it spans the whole space.
+// ==========================================================================
+constexpr int LZ_MIN_N = 400;     // below this, tridiagonalisation + Sturm
+constexpr double LZ_RTOL = 1e-10;
+constexpr int LZ_CHECK = 8;
+
+__device__ inline double lz_hash(int64_t i) {
+  uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+__global__ __launch_bounds__(1024) void lz_init_kernel(int n, double* q) {
+  __shared__ double red[16];
+  double ss = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = lz_hash(i);
+    q[i] = v;
+    ss += v * v;
+  }
+  ss = block_reduce<0>(ss, red);
+  const double inv = 1.0 / sqrt(ss);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) q[i] *= inv;
+}
+
+// w[c] = sum_i A[i, c] q[i]  (A symmetric), one workgroup per column
+__global__ __launch_bounds__(256) void lz_gemv_kernel(int n, const double* A, int64_t lda,
+                                                      const double* q, double* w) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const double* col = A + (int64_t)c * lda;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += col[i] * q[i];
+  v = block_reduce<0>(v, red);
+  if (threadIdx.x == 0) w[c] = v;
+}
+
+// h[c] = Q[:, c]' w, one workgroup per column
+__global__ __launch_bounds__(256) void lz_dots_kernel(int n, const double* Q, const double* w,
+                                                      double* h) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const double* col = Q + (int64_t)c * n;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += col[i] * w[i];
+  v = block_reduce<0>(v, red);
+  if (threadIdx.x == 0) h[c] = v;
+}
+
+// w -= Q[:, 0:ncols] h: 64 rows per workgroup, 4 lanes per row split the
+// columns, partial sums combined in LDS.
+__global__ __launch_bounds__(256) void lz_update_kernel(int n, int ncols, const double* Q,
+                                                        const double* h, double* w) {
+  __shared__ double part[4][64];
+  const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + r;
+  double v = 0.0;
+  if (i < n)
+    for (int c = g; c < ncols; c += 4) v += Q[i + (int64_t)c * n] * h[c];
+  part[g][r] = v;
+  __syncthreads();
+  if (g == 0 && i < n) w[i] -= (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+}
+
+// alpha_j = h1[j] + h2[j], beta_j = ||w||, q_{j+1} = w / beta_j
+__global__ __launch_bounds__(1024) void lz_finish_kernel(int n, int j, const double* w,
+                                                         const double* h1, const double* h2,
+                                                         double* alpha, double* beta, double* qn) {
+  __shared__ double red[16];
+  double ss = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) ss += w[i] * w[i];
+  ss = block_reduce<0>(ss, red);
+  const double b = sqrt(ss);
+  const double inv = b > 0 ? 1.0 / b : 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) qn[i] = w[i] * inv;
+  if (threadIdx.x == 0) {
+    alpha[j] = h1[j] + h2[j];
+    beta[j] = b;
+  }
+}
+
+// Smallest eigenvalue theta of the k x k Lanczos tridiagonal (alpha, beta)
+// by Sturm multisection, and the residual bound beta_{k-1} |y_{k-1}| of its
+// normalized eigenvector y (inverse iteration with partial pivoting).
+// out[0] = theta, out[1] = residual bound, out[2] = |beta_{k-1}|.
+__global__ __launch_bounds__(256) void lz_check_kernel(int k, const double* alpha_g,
+                                                       const double* beta_g, double* scratch_g,
+                                                       double* out) {
+  __shared__ double red[4];
+  // the tridiagonal and the inverse-iteration arrays live in LDS (k <= 1024)
+  __shared__ double alpha[1024], beta[1024];
+  __shared__ double scratch[7 * 1024];
+  (void)scratch_g;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    alpha[i] = alpha_g[i];
+    beta[i] = beta_g[i];
+  }
+  __syncthreads();
+  __shared__ double lo_s, hi_s;
+  __shared__ int first;
+  const int tid = threadIdx.x;
+  double gl = INFINITY, gu = -INFINITY, emax = 0.0;
+  for (int i = tid; i < k; i += 256) {
+    const double r = (i > 0 ? fabs(beta[i - 1]) : 0.0) + (i < k - 1 ? fabs(beta[i]) : 0.0);
+    gl = fmin(gl, alpha[i] - r);
+    gu = fmax(gu, alpha[i] + r);
+    if (i < k - 1) emax = fmax(emax, beta[i] * beta[i]);
+  }
+  gl = block_reduce<2>(gl, red);
+  gu = block_reduce<1>(gu, red);
+  emax = block_reduce<1>(emax, red);
+  const double pivmin = DBL_MIN * fmax(1.0, emax);
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  if (tid == 0) {
+    lo_s = gl - 2 * DEPS * tnorm - 2 * pivmin;
+    hi_s = gu + 2 * DEPS * tnorm + 2 * pivmin;
+  }
+  __syncthreads();
+  for (int round = 0; round < 12; ++round) {
+    const double lo = lo_s, hi = hi_s;
+    if (hi - lo <= 2 * DEPS * fmax(fabs(lo), fabs(hi)) + 4 * pivmin) break;
+    if (tid == 0) first = 256;
+    __syncthreads();
+    const double x = lo + (hi - lo) * (double)(tid + 1) / 257.0;
+    if (sturm_count(k, alpha, beta, x, pivmin) >= 1) atomicMin(&first, tid);
+    __syncthreads();
+    if (tid == 0) {
+      const int f = first;
+      lo_s = f == 0 ? lo : lo + (hi - lo) * (double)f / 257.0;
+      hi_s = f == 256 ? hi : lo + (hi - lo) * (double)(f + 1) / 257.0;
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  const double theta = 0.5 * (lo_s + hi_s);
+  // inverse iteration on (T - theta I) y = b with partial pivoting
+  double* u0 = scratch;           // k
+  double* u1 = scratch + k;       // k
+  double* u2 = scratch + 2 * k;   // k
+  double* y = scratch + 3 * k;    // k
+  double* bb = scratch + 4 * k;   // k
+  double* l = scratch + 5 * k;    // multipliers k
+  int* sw = reinterpret_cast<int*>(scratch + 6 * k);  // swaps k
+  const double tiny = DEPS * fmax(tnorm, DBL_MIN);
+  for (int i = 0; i < k; ++i) y[i] = 1.0;
+  for (int pass = 0; pass < 3; ++pass) {
+    // factor
+    double dcur = alpha[0] - theta, ucur = k > 1 ? beta[0] : 0.0;
+    for (int i = 0; i < k; ++i) bb[i] = y[i];
+    for (int i = 0; i < k - 1; ++i) {
+      const double sub = beta[i];
+      const double dn = alpha[i + 1] - theta;
+      const double un = i + 1 < k - 1 ? beta[i + 1] : 0.0;
+      if (fabs(dcur) >= fabs(sub)) {
+        const double f = dcur != 0.0 ? sub / dcur : 0.0;
+        u0[i] = dcur; u1[i] = ucur; u2[i] = 0.0; l[i] = f; sw[i] = 0;
+        dcur = dn - f * ucur;
+        ucur = un;
+      } else {
+        const double f = dcur / sub;
+        u0[i] = sub; u1[i] = dn; u2[i] = un; l[i] = f; sw[i] = 1;
+        dcur = ucur - f * dn;
+        ucur = -f * un;
+      }
+    }
+    u0[k - 1] = dcur;
+    // forward
+    for (int i = 0; i < k - 1; ++i) {
+      if (sw[i]) { const double t = bb[i]; bb[i] = bb[i + 1]; bb[i + 1] = t - l[i] * bb[i]; }
+      else bb[i + 1] -= l[i] * bb[i];
+    }
+    // backward
+    for (int i = k - 1; i >= 0; --i) {
+      double v = bb[i];
+      if (i + 1 < k) v -= u1[i] * y[i + 1];
+      if (i + 2 < k) v -= u2[i] * y[i + 2];
+      double piv = u0[i];
+      if (fabs(piv) < tiny) piv = piv < 0 ? -tiny : tiny;
+      y[i] = v / piv;
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < k; ++i) nrm = fmax(nrm, fabs(y[i]));
+    if (!(nrm > 0) || !isfinite(nrm)) break;
+    for (int i = 0; i < k; ++i) y[i] /= nrm;
+  }
+  double n2 = 0.0;
+  for (int i = 0; i < k; ++i) n2 += y[i] * y[i];
+  const double ylast = fabs(y[k - 1]) / sqrt(n2);
+  out[0] = theta;
+  out[1] = isfinite(ylast) ? fabs(beta[k - 1]) * ylast : INFINITY;
+  out[2] = fabs(beta[k - 1]);
+}
+
+static int lanczos_min(hipStream_t s, int n, const double* A, int64_t lda, double* out,
+                       double* work) {
+  const int kmax = std::min(n, 1024);
+  double* Q = work;                                  // n x (kmax + 1)
+  double* w = Q + (size_t)n * (kmax + 1);            // n
+  double* h1 = w + n;                                // kmax + 1
+  double* h2 = h1 + kmax + 1;                        // kmax + 1
+  double* al = h2 + kmax + 1;                        // kmax
+  double* be = al + kmax;                            // kmax
+  double* chk = be + kmax;                           // 4
+  double* scr = chk + 4;                             // 7 kmax
+  lz_init_kernel<<<1, 1024, 0, s>>>(n, Q);
+  MSDP_HIP(hipGetLastError());
+  double hv[3] = {0, 0, 0};
+  double anorm_est = 0.0;
+  for (int j = 0; j < kmax; ++j) {
+    const double* qj = Q + (size_t)j * n;
+    lz_gemv_kernel<<<n, 256, 0, s>>>(n, A, lda, qj, w);
+    lz_dots_kernel<<<j + 1, 256, 0, s>>>(n, Q, w, h1);
+    lz_update_kernel<<<cdiv(n, 64), 256, 0, s>>>(n, j + 1, Q, h1, w);
+    lz_dots_kernel<<<j + 1, 256, 0, s>>>(n, Q, w, h2);
+    lz_update_kernel<<<cdiv(n, 64), 256, 0, s>>>(n, j + 1, Q, h2, w);
+    lz_finish_kernel<<<1, 1024, 0, s>>>(n, j, w, h1, h2, al, be, Q + (size_t)(j + 1) * n);
+    MSDP_HIP(hipGetLastError());
+    const int k = j + 1;
+    if (k % LZ_CHECK == 0 || k == kmax) {
+      lz_check_kernel<<<1, 256, 0, s>>>(k, al, be, scr, chk);
+      MSDP_HIP(hipGetLastError());
+      MSDP_HIP(hipMemcpyAsync(hv, chk, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+      MSDP_HIP(hipStreamSynchronize(s));
+      anorm_est = std::max(anorm_est, std::fabs(hv[0]));
+      const bool invariant = hv[2] <= 1e-14 * std::max(anorm_est, 1e-300);
+      if (hv[1] <= LZ_RTOL * std::max(std::fabs(hv[0]), 1e-300) || invariant || k == n) break;
+    }
+  }
+  MSDP_HIP(hipMemcpyAsync(out, chk, sizeof(double), hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+size_t eig_min_work_elems(int n) {
+  const size_t kmax = std::min(n, 1024);
+  return std::max(trd_work_elems(n), (size_t)n * (kmax + 2) + 12 * kmax + 64);
+}
+
 int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out, double* work, int*) {
   if (n <= 0) return 0;
   if (n <= 64) {
@@ -899,6 +1248,7 @@ int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out, dou
     MSDP_HIP(hipMemcpyAsync(out, wv, sizeof(double), hipMemcpyDeviceToDevice, s));
     return 0;
   }
+  if (n >= LZ_MIN_N) return lanczos_min(s, n, A, lda, out, work);
   TrdWork t = trd_carve(n, work);
   MSDP_TRY(sytrd(s, n, A, lda, t));
   tridiag_min_kernel<<<1, 256, 0, s>>>(n, t.d, t.e, out);

@@ -59,7 +59,8 @@ int scalar_affine(hipStream_t st, int ns, const double* x, const double* dx, con
 int nt_rnt(hipStream_t st, int n, const double* M, const double* D, double* R);
 int sqrt_min(hipStream_t st, int n, const double* lam, double* D, double* minlam);
 int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, double* out);
-int potrs(hipStream_t st, int m, const double* L, int64_t ld, double* b);
+int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
+          const double* dinv, double* b, double* tmp);
 int scale_cols_vec(hipStream_t st, int rows, int cols, double* A, int64_t lda, const double* d);
 int vec_min(hipStream_t st, int n, const double* v, double* out);
 int schur_onehot_identity(hipStream_t st, int m, const double* L, int64_t ldl, const double* R,

@@ -308,55 +308,102 @@ int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, doubl
 }
 
 // ------------------------------------------------- triangular solves ------
-// Solve L y = b (trans = 0) or L' y = b (trans = 1) for one diagonal block
-// [k0, k0+kb) in place on b; one workgroup of 64 threads.
-__global__ __launch_bounds__(64) void trsv_diag_kernel(int kb, const double* L, int64_t ld,
-                                                       int k0, double* b, int trans) {
-  __shared__ double Ls[64][65];
-  __shared__ double x[64];
+// H y = b with H = L L' (potrs).  The factorisation leaves the inverses of
+// the 64 x 64 diagonal blocks (dinv) and L' is kept column-major (Lt), so
+// every block step of both sweeps is one launch with coalesced loads.
+__global__ __launch_bounds__(256) void trsv_fwd_step_kernel(int m, int k0, int kb,
+                                                            const double* L, int64_t ld,
+                                                            const double* dinv, double* b,
+                                                            double* x) {
+  __shared__ double Ds[64][65];
+  __shared__ double bs[64], xs[64];
+  __shared__ double part[4][64];
   const int t = threadIdx.x;
-  for (int e = t; e < kb * kb; e += 64) {
-    const int i = e % kb, j = e / kb;
-    Ls[i][j] = i >= j ? L[(k0 + i) + (int64_t)(k0 + j) * ld] : 0.0;
-  }
-  if (t < kb) x[t] = b[k0 + t];
+  for (int e = t; e < 64 * 64; e += 256) Ds[e & 63][e >> 6] = dinv[e];
+  if (t < 64) bs[t] = t < kb ? b[k0 + t] : 0.0;
   __syncthreads();
-  if (!trans) {
-    for (int i = 0; i < kb; ++i) {
-      if (t == 0) x[i] /= Ls[i][i];
-      __syncthreads();
-      if (t > i && t < kb) x[t] -= Ls[t][i] * x[i];
-      __syncthreads();
-    }
-  } else {
-    for (int i = kb - 1; i >= 0; --i) {
-      if (t == 0) x[i] /= Ls[i][i];
-      __syncthreads();
-      if (t < i) x[t] -= Ls[i][t] * x[i];
-      __syncthreads();
+  {  // x_k = D_k b_k: 4 lanes per row, 16 columns each
+    const int rr = t & 63, g = t >> 6;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v += Ds[rr][g + 4 * q] * bs[g + 4 * q];
+    part[g][rr] = v;
+  }
+  __syncthreads();
+  if (t < 64) {
+    const double v = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+    xs[t] = v;
+    if (blockIdx.x == 0 && t < kb) x[k0 + t] = v;
+  }
+  __syncthreads();
+  const int rr = t & 63, g = t >> 6;
+  const int r = k0 + kb + blockIdx.x * 64 + rr;
+  double v = 0.0;
+  if (r < m) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = g + 4 * q;
+      if (c < kb) v += L[r + (int64_t)(k0 + c) * ld] * xs[c];
     }
   }
-  if (t < kb) b[k0 + t] = x[t];
+  part[g][rr] = v;
+  __syncthreads();
+  if (g == 0 && r < m) b[r] -= (part[0][rr] + part[1][rr]) + (part[2][rr] + part[3][rr]);
 }
 
-int potrs(hipStream_t st, int m, const double* L, int64_t ld, double* b) {
-  // forward: L y = b
-  for (int k0 = 0; k0 < m; k0 += 64) {
-    const int kb = std::min(64, m - k0);
-    trsv_diag_kernel<<<1, 64, 0, st>>>(kb, L, ld, k0, b, 0);
-    const int rest = m - k0 - kb;
-    if (rest > 0)
-      MSDP_TRY(gemm(st, false, false, rest, 1, kb, -1.0, L + (k0 + kb) + (int64_t)k0 * ld, ld,
-                    b + k0, m, 1.0, b + k0 + kb, m));
+__global__ __launch_bounds__(256) void trsv_bwd_step_kernel(int k0, int kb, const double* Lt,
+                                                            int64_t ld, const double* dinv,
+                                                            double* x, double* z) {
+  __shared__ double Ds[64][65];
+  __shared__ double xs_in[64], zs[64];
+  __shared__ double part[4][64];
+  const int t = threadIdx.x;
+  for (int e = t; e < 64 * 64; e += 256) Ds[e & 63][e >> 6] = dinv[e];
+  if (t < 64) xs_in[t] = t < kb ? x[k0 + t] : 0.0;
+  __syncthreads();
+  {  // z_k = D_k' x_k
+    const int rr = t & 63, g = t >> 6;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v += Ds[g + 4 * q][rr] * xs_in[g + 4 * q];
+    part[g][rr] = v;
   }
-  // backward: L' x = y
-  const int nblk = (m + 63) / 64;
+  __syncthreads();
+  if (t < 64) {
+    const double v = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+    zs[t] = v;
+    if (blockIdx.x == 0 && t < kb) z[k0 + t] = v;
+  }
+  __syncthreads();
+  const int rr = t & 63, g = t >> 6;
+  const int r = blockIdx.x * 64 + rr;
+  double v = 0.0;
+  if (r < k0) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = g + 4 * q;
+      if (c < kb) v += Lt[r + (int64_t)(k0 + c) * ld] * zs[c];
+    }
+  }
+  part[g][rr] = v;
+  __syncthreads();
+  if (g == 0 && r < k0) x[r] -= (part[0][rr] + part[1][rr]) + (part[2][rr] + part[3][rr]);
+}
+
+int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
+          const double* dinv, double* b, double* tmp) {
+  if (m <= 0) return 0;
+  const int nblk = cdiv(m, 64);
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int k0 = bi * 64, kb = std::min(64, m - k0);
+    const int rest = m - k0 - kb;
+    trsv_fwd_step_kernel<<<std::max(1, cdiv(rest, 64)), 256, 0, st>>>(
+        m, k0, kb, L, ld, dinv + (size_t)bi * 4096, b, tmp);
+  }
   for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int k0 = bi * 64;
-    const int kb = std::min(64, m - k0);
-    trsv_diag_kernel<<<1, 64, 0, st>>>(kb, L, ld, k0, b, 1);
-    if (k0 > 0)
-      MSDP_TRY(gemm(st, true, false, k0, 1, kb, -1.0, L + k0, ld, b + k0, m, 1.0, b, m));
+    const int k0 = bi * 64, kb = std::min(64, m - k0);
+    trsv_bwd_step_kernel<<<std::max(1, cdiv(k0, 64)), 256, 0, st>>>(
+        k0, kb, Lt, ld, dinv + (size_t)bi * 4096, tmp, b);
   }
   MSDP_HIP(hipGetLastError());
   return 0;

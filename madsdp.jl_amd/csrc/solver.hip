@@ -72,7 +72,7 @@ struct msdp_solver {
          *Rp = nullptr, *x = nullptr, *z = nullptr, *delx = nullptr, *delz = nullptr,
          *Rdlin = nullptr, *zinv = nullptr, *corrlin = nullptr, *wm = nullptr, *ws1 = nullptr,
          *ws2 = nullptr, *Cs = nullptr;
-  double *H = nullptr, *Hs = nullptr;
+  double *H = nullptr, *Hs = nullptr, *Ht = nullptr, *Hdinv = nullptr, *wm2 = nullptr;
   int* info = nullptr;    // per-block factorisation infos (4 per block) + 8
   double* sc = nullptr;   // scalar slots
   double* red = nullptr;  // reduction scratch
@@ -285,13 +285,17 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&Cs, mm * nss));
   MSDP_TRY(alloc(&H, mm * mm));
   MSDP_TRY(alloc(&Hs, mm * mm));
+  MSDP_TRY(alloc(&Ht, mm * mm));
+  MSDP_TRY(alloc(&Hdinv, potrf_work_elems((int)mm)));
+  MSDP_TRY(alloc(&wm2, mm));
   MSDP_TRY(alloc(&info, 4 * nb + 8));
   nslots = 32 + 8 * nb;
   MSDP_TRY(alloc(&sc, nslots));
   MSDP_TRY(alloc(&red, reduce_scratch_elems(0) + 64));
   MSDP_TRY(alloc(&ework, syevd_work_elems(nmax)));
   MSDP_TRY(alloc(&eiwork, syevd_iwork_elems(nmax)));
-  MSDP_TRY(alloc(&trwork, std::max(trtri_work_elems(nmax), (size_t)m + 64)));
+  MSDP_TRY(alloc(&trwork, std::max(std::max(trtri_work_elems(nmax), potrf_work_elems(nmax)),
+                                    (size_t)m + 64)));
   int64_t tot = 0;
   for (auto& bb : blk) tot += bb.n;
   MSDP_TRY(alloc(&eigall, std::max<int64_t>(tot, 1)));
@@ -331,7 +335,7 @@ int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) 
   const int n = bk.n;
   MSDP_TRY(copy_mat(st, n, n, M, n, L, n));
   MSDP_TRY(symmetrize(st, n, L, n));
-  MSDP_TRY(potrf(st, n, L, n, inf, nullptr));
+  MSDP_TRY(potrf(st, n, L, n, inf, trwork));
   MSDP_TRY(chol_flush(st, n, L, n, red));
   return 0;
 }
@@ -472,11 +476,15 @@ int msdp_solver::factorize_schur(bool& ok) {
   for (int t = 0; t < opt.max_schur_reg; ++t) {
     MSDP_TRY(copy_mat(st, (int)m, (int)m, Hs, m, H, m));
     if (reg > 0) MSDP_TRY(add_diag(st, (int)m, H, m, nullptr, reg));
-    MSDP_TRY(potrf(st, (int)m, H, m, info, nullptr));
+    MSDP_TRY(potrf(st, (int)m, H, m, info, Hdinv));
     int hinfo = 0;
     MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
     MSDP_HIP(hipStreamSynchronize(st));
-    if (hinfo == 0) { ok = true; return 0; }
+    if (hinfo == 0) {
+      MSDP_TRY(transpose(st, (int)m, (int)m, H, m, Ht, m));
+      ok = true;
+      return 0;
+    }
     reg = reg == 0 ? 1e-12 : 10 * reg;
   }
   return 0;
@@ -515,7 +523,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
       MSDP_TRY(scalar_op(st, SOP_RHS, (int)ns, mu_, x, z, zinv, Rdlin, corrlin, ws1, nullptr, corrected));
       MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, ws1, ns, 1.0, dely, m));
     }
-    MSDP_TRY(potrs(st, (int)m, H, m, dely));
+    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dely, wm2));
   }
   // directions
   for (auto& bk : blk) {
@@ -606,7 +614,7 @@ int msdp_solver::step_lengths(double& a, double& bt) {
       const double* dX = primal ? bk.delX : bk.delS;
       for (;;) {
         MSDP_TRY(axpby(st, (int64_t)n * n, 1.0, Xm, al, dX, bk.buf));
-        MSDP_TRY(potrf(st, n, bk.buf, n, info, nullptr));
+        MSDP_TRY(potrf(st, n, bk.buf, n, info, trwork));
         int hinfo = 0;
         MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
         MSDP_HIP(hipStreamSynchronize(st));
@@ -892,7 +900,7 @@ int msdp_solver::newton_direction(bool& ok) {  // :89-110
   MSDP_TRY(factorize_schur(ok));
   if (!ok) return 0;
   MSDP_HIP(hipMemcpyAsync(dc_d, dc_g, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  MSDP_TRY(potrs(st, (int)m, H, m, dc_d));
+  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dc_d, wm2));
   MSDP_TRY(dot(st, m, dc_g, dc_d, slot(11), red));
   MSDP_TRY(read_slots());
   dc_lambda = std::sqrt(std::max(hsc[11], 0.0));
@@ -916,7 +924,7 @@ int msdp_solver::dc_initialize() {  // :112-143
   MSDP_TRY(newton_direction(ok));
   if (!ok) { set_error("MadSDP: failed to factorize the dual Hessian at y0."); return -3; }
   MSDP_HIP(hipMemcpyAsync(dc_dy, b, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  MSDP_TRY(potrs(st, (int)m, H, m, dc_dy));
+  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dc_dy, wm2));
   MSDP_TRY(dot(st, m, b, dc_dy, slot(12), red));
   MSDP_TRY(dot(st, m, b, dc_d, slot(13), red));
   MSDP_TRY(read_slots());
@@ -940,7 +948,7 @@ int msdp_solver::dc_initialize() {  // :112-143
 int msdp_solver::predictor_step(bool& ok) {  // :232-264
   MSDP_TRY(axpby(st, m, 1.0, y, 1.0, dc_d, dc_yhat));
   MSDP_HIP(hipMemcpyAsync(dc_dy, b, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  MSDP_TRY(potrs(st, (int)m, H, m, dc_dy));
+  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dc_dy, wm2));
   MSDP_TRY(axpby(st, m, dc_t, dc_dy, 0.0, dc_dy, dc_dy));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
   int64_t off = 0;
@@ -1234,7 +1242,7 @@ int msdp_solver_take_step(msdp_solver* s, double a, double bt) { GUARD(s); retur
 int msdp_solver_schur_solve(msdp_solver* s, double* rhs) {
   GUARD(s);
   MSDP_HIP(hipMemcpyAsync(s->wm, rhs, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
-  MSDP_TRY(potrs(s->st, (int)s->m, s->H, s->m, s->wm));
+  MSDP_TRY(potrs(s->st, (int)s->m, s->H, s->Ht, s->m, s->Hdinv, s->wm, s->wm2));
   MSDP_HIP(hipMemcpyAsync(rhs, s->wm, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->st));
   MSDP_HIP(hipStreamSynchronize(s->st));
   return 0;
@@ -1297,16 +1305,19 @@ int msdp_k_gemm(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int3
                 int64_t ldc) {
   return gemm((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
-int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info) {
-  return potrf((hipStream_t)stream, n, A, lda, info, nullptr);
+int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info, double* work) {
+  return potrf((hipStream_t)stream, n, A, lda, info, work);
 }
-size_t msdp_k_cholesky_factor_scratch(int32_t) { return reduce_scratch_elems(0) + 64; }
+size_t msdp_k_potrf_work(int32_t n) { return potrf_work_elems(n); }
+size_t msdp_k_cholesky_factor_scratch(int32_t n) {
+  return reduce_scratch_elems(0) + 64 + potrf_work_elems(n);
+}
 int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm, double* L, int64_t ldl,
                            int32_t* info, double* scratch) {
   hipStream_t st = (hipStream_t)stream;
   MSDP_TRY(copy_mat(st, n, n, M, ldm, L, ldl));
   MSDP_TRY(symmetrize(st, n, L, ldl));
-  MSDP_TRY(potrf(st, n, L, ldl, info, nullptr));
+  MSDP_TRY(potrf(st, n, L, ldl, info, scratch + reduce_scratch_elems(0) + 64));
   return chol_flush(st, n, L, ldl, scratch);
 }
 size_t msdp_k_trtri_work(int32_t n) { return trtri_work_elems(n); }

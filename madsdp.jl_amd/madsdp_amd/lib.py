@@ -77,7 +77,8 @@ SIGNATURES = {
     "msdp_solver_profile_end": (C.c_int, [P, c_double_p, c_int64_p, c_double_p]),
     "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
-    "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P]),
+    "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),
+    "msdp_k_potrf_work": (C.c_size_t, [C.c_int32]),
     "msdp_k_cholesky_factor": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, P]),
     "msdp_k_cholesky_factor_scratch": (C.c_size_t, [C.c_int32]),
     "msdp_k_trtri": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P]),

@@ -36,7 +36,8 @@ def P(t):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 3, 129)])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 3, 129),
+                                   (300, 260, 40), (64, 200, 3000), (3, 1, 5000)])
 def test_gemm(env, ta, tb, M, N, K):
     torch, lib, st = env
     rng = np.random.default_rng(M * 7 + N + K)
@@ -77,7 +78,7 @@ def test_cholesky_not_pd(env):
     for M, pivot in ((np.array([[1.0, 2.0], [2.0, 1.0]]), 2), (np.zeros((2, 2)), 1)):
         dM, dL = dev(torch, M), dev(torch, np.zeros((2, 2)))
         info = torch.zeros(1, dtype=torch.int32, device="cuda")
-        scratch = torch.zeros(1024, dtype=torch.float64, device="cuda")
+        scratch = torch.zeros(int(lib.msdp_k_cholesky_factor_scratch(2)), dtype=torch.float64, device="cuda")
         assert lib.msdp_k_cholesky_factor(st, 2, P(dM), 2, P(dL), 2, P(info), P(scratch)) == 0
         torch.cuda.synchronize()
         assert int(info.item()) == pivot
@@ -91,7 +92,7 @@ def test_cholesky_flush(env):
     M = F @ F.T
     dM, dL = dev(torch, M), dev(torch, np.zeros((3, 3)))
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    scratch = torch.zeros(1024, dtype=torch.float64, device="cuda")
+    scratch = torch.zeros(int(lib.msdp_k_cholesky_factor_scratch(3)), dtype=torch.float64, device="cuda")
     assert lib.msdp_k_cholesky_factor(st, 3, P(dM), 3, P(dL), 3, P(info), P(scratch)) == 0
     torch.cuda.synchronize()
     Lh = host(dL, (3, 3))
@@ -151,12 +152,29 @@ def test_syevd(env, n, kind):
     np.testing.assert_allclose(A @ Vh, Vh * wh, rtol=0, atol=1e-11 * scale * max(1, n / 64))
 
 
-@pytest.mark.parametrize("n", [3, 64, 65, 200, 500])
-def test_eig_min(env, n):
+@pytest.mark.parametrize("n,kind", [(3, "random"), (64, "random"), (65, "random"), (200, "random"),
+                                    (500, "random"), (1000, "random"), (1000, "cluster"),
+                                    (1500, "step")])
+def test_eig_min(env, n, kind):
+    """Below 400 by tridiagonalisation + Sturm, above by Lanczos."""
     torch, lib, st = env
     rng = np.random.default_rng(n + 3)
-    A = rng.standard_normal((n, n))
-    A = (A + A.T) / 2
+    if kind == "random":
+        A = rng.standard_normal((n, n))
+        A = (A + A.T) / 2
+    elif kind == "cluster":  # three nearly equal smallest eigenvalues
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        lam = np.concatenate([[-2.0, -2.0 + 1e-9, -2.0 + 2e-9], rng.uniform(-1, 3, n - 3)])
+        A = (Q * lam) @ Q.T
+        A = (A + A.T) / 2
+    else:  # L^-1 D L^-T of an IPM-like step: X = I + low-rank, D indefinite
+        G = rng.standard_normal((n, 20))
+        X = np.eye(n) + G @ G.T / n
+        D = rng.standard_normal((n, n)) / np.sqrt(n)
+        D = (D + D.T) / 2 - 0.3 * np.eye(n)
+        Li = np.linalg.inv(np.linalg.cholesky(X))
+        A = Li @ D @ Li.T
+        A = (A + A.T) / 2
     dA = dev(torch, A)
     out = torch.zeros(1, dtype=torch.float64, device="cuda")
     work = torch.zeros(int(lib.msdp_k_syevd_work(n)), dtype=torch.float64, device="cuda")
