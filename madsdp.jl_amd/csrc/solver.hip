@@ -8,6 +8,7 @@
 // mu, the termination quantities and the factorisation statuses, i.e. the
 // values the reference's control flow branches on.
 #include "ops.h"
+#include "qr.h"
 #include "madsdp_hip.h"
 
 #include <algorithm>
@@ -82,6 +83,11 @@ struct msdp_solver {
   double* swork = nullptr;
   size_t swork_elems = 0;
   double* eigall = nullptr;  // DualCentering proximity eigenvalues, all blocks
+  // QRSchur workspace (qr_schur.jl:5-10): B with the reflectors, scale, tau
+  double *qB = nullptr, *qscale = nullptr, *qtau = nullptr, *qbase = nullptr, *qy = nullptr,
+         *qc = nullptr, *qt = nullptr, *qrd = nullptr;
+  int* qbad = nullptr;
+  int64_t qrows = 0;
   // DualCentering vectors
   double *dc_d = nullptr, *dc_g = nullptr, *dc_dy = nullptr, *dc_yhat = nullptr;
   // host scalars (src/solver.jl:37-48)
@@ -166,6 +172,9 @@ struct msdp_solver {
   int dc_initialize();
   int dc_iterate();
   int predictor_step(bool& ok);
+  // QRSchur
+  int qr_factorize(bool& ok);
+  int qr_solve_directions(double mu_, bool corrected);
 };
 
 static int as_err(const char* msg) {
@@ -443,6 +452,7 @@ int msdp_solver::prepare(bool& ok) {
 // factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
 int msdp_solver::factorize_schur(bool& ok) {
   ok = false;
+  if (opt.schur == MSDP_SCHUR_QR) return qr_factorize(ok);
   if (m == 0) { ok = true; return 0; }
   MSDP_TRY(fill(st, m * m, 0.0, Hs));
   for (auto& bk : blk) {
@@ -506,6 +516,7 @@ int msdp_solver::apply_E(Blk& bk, const double* M, bool diag, double* out) {
 // solve_directions! (primal_dual.jl:44-54) = rhs! (step.jl:198-224),
 // schur_solve! (schur_solvers.jl:21), directions! (step.jl:229-259)
 int msdp_solver::solve_directions(double mu_, bool corrected) {
+  if (opt.schur == MSDP_SCHUR_QR) return qr_solve_directions(mu_, corrected);
   // rhs
   if (m > 0) {
     MSDP_HIP(hipMemcpyAsync(dely, Rp, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
@@ -1017,6 +1028,120 @@ int msdp_solver::dc_iterate() {  // :266-289
 }
 
 // ==========================================================================
+// QRSchur (src/qr_schur.jl)
+// ==========================================================================
+// factorize_schur! (qr_schur.jl:24-88)
+int msdp_solver::qr_factorize(bool& ok) {
+  ok = false;
+  int64_t rows = ns;
+  for (auto& bk : blk) rows += (int64_t)bk.n * (bk.n + 1) / 2;
+  if (!qB) {
+    if ((double)rows * (double)std::max<int64_t>(m, 1) * 8.0 > 64e9)
+      return as_err("QRSchur: the dense scaled constraint matrix would exceed 64 GB");
+    const size_t rr = std::max<int64_t>(rows, 1), mm = std::max<int64_t>(m, 1);
+    MSDP_TRY(alloc(&qB, rr * mm));
+    MSDP_TRY(alloc(&qbase, rr));
+    MSDP_TRY(alloc(&qy, rr));
+    MSDP_TRY(alloc(&qscale, mm));
+    MSDP_TRY(alloc(&qtau, mm));
+    MSDP_TRY(alloc(&qc, mm));
+    MSDP_TRY(alloc(&qt, mm));
+    MSDP_TRY(alloc(&qrd, 2));
+    MSDP_TRY(alloc(&qbad, 1));
+    qrows = rows;
+  }
+  if (m == 0) { ok = true; return 0; }
+  MSDP_TRY(fill(st, rows * m, 0.0, qB));
+  int64_t off = 0;
+  for (auto& bk : blk) {
+    const int n = bk.n;
+    const int64_t n2 = (int64_t)n * n;
+    const BlockOps& o = bk.ops;
+    if (o.kind == BlockOps::DENSE) {
+      const int cb = (int)std::max<int64_t>(1, std::min<int64_t>(o.ncon, (int64_t)(swork_elems / (2 * n2))));
+      for (int j0 = 0; j0 < o.ncon; j0 += cb) {
+        const int c = std::min(cb, o.ncon - j0);
+        double* T = swork;
+        double* P = swork + (int64_t)cb * n2;
+        MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, o.mats + (int64_t)j0 * n2, n, bk.G, n, 0.0, T, n, c, n2, 0, n2));
+        MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.G, n, T, n, 0.0, P, n, c, 0, n2, n2));
+        MSDP_TRY(qr_pack_dense(st, n, c, P, o.con_list + j0, qB, rows, off));
+      }
+    } else if (o.kind == BlockOps::COO) {
+      MSDP_TRY(qr_pack_coo(st, o, bk.G, qB, rows, off));
+    }
+    off += (int64_t)n * (n + 1) / 2;
+  }
+  MSDP_TRY(qr_scalar_rows(st, (int)m, (int)ns, Clin, x, z, qB, rows, off));
+  MSDP_TRY(qr_factor(st, rows, (int)m, qB, qscale, qtau, qbad, qrd));
+  int bad = 0;
+  double rd[2] = {0, 0};
+  MSDP_HIP(hipMemcpyAsync(&bad, qbad, sizeof(int), hipMemcpyDeviceToHost, st));
+  MSDP_HIP(hipMemcpyAsync(rd, qrd, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  MSDP_HIP(hipStreamSynchronize(st));
+  if (bad) return 0;              // zero or non-finite column (qr_schur.jl:74-77)
+  if (rows < m) return 0;         // more multipliers than rows (:83-85)
+  // A dense Householder QR leaves roundoff where SPQR's exact zero pivot would
+  // be; a pivot below this tolerance counts as zero (:87).
+  if (!(rd[1] > 64 * EPS * std::max(1.0, rd[0]))) return 0;
+  ok = true;
+  return 0;
+}
+
+// solve_directions! (qr_schur.jl:90-160)
+int msdp_solver::qr_solve_directions(double mu_, bool corrected) {
+  const int64_t rows = qrows;
+  MSDP_TRY(fill(st, rows, 0.0, qbase));
+  int64_t off = 0;
+  for (auto& bk : blk) {
+    const int n = bk.n;
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Rd, n, bk.G, n, 0.0, bk.tmp, n));
+    MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.G, n, bk.tmp, n, 0.0, bk.tmp2, n));
+    MSDP_TRY(qr_base_block(st, n, bk.tmp2, bk.D, mu_, bk.RNT, corrected, qbase, off));
+    off += (int64_t)n * (n + 1) / 2;
+  }
+  MSDP_TRY(qr_base_scalar(st, (int)ns, x, z, Rdlin, corrlin, mu_, corrected, qbase, off));
+  if (m > 0) {
+    // qbase = Q' base; c = R'^-1 (scale .* Rp); t = c - qbase[1:m]
+    MSDP_HIP(hipMemcpyAsync(qy, qbase, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
+    MSDP_TRY(qr_apply_vec(st, rows, (int)m, qB, qtau, qy, true));
+    MSDP_TRY(qr_mul(st, (int)m, qscale, Rp, qc));
+    MSDP_TRY(qr_trsv(st, (int)m, qB, rows, qc, true));
+    MSDP_TRY(qr_combine(st, (int)m, qc, qy, qt));
+    // dely = scale .* (R^-1 t)
+    MSDP_HIP(hipMemcpyAsync(dely, qt, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    MSDP_TRY(qr_trsv(st, (int)m, qB, rows, dely, false));
+    MSDP_TRY(qr_mul(st, (int)m, qscale, dely, dely));
+    // direction = base + Q [t; 0]
+    MSDP_TRY(fill(st, rows, 0.0, qy));
+    MSDP_HIP(hipMemcpyAsync(qy, qt, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    MSDP_TRY(qr_apply_vec(st, rows, (int)m, qB, qtau, qy, false));
+    MSDP_TRY(axpby(st, rows, 1.0, qbase, 1.0, qy, qy));
+  } else {
+    MSDP_HIP(hipMemcpyAsync(qy, qbase, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
+  }
+  off = 0;
+  for (auto& bk : blk) {
+    const int n = bk.n;
+    const int64_t n2 = (int64_t)n * n;
+    MSDP_TRY(qr_smat(st, n, qy, off, bk.buf));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.G, n, bk.buf, n, 0.0, bk.tmp, n));
+    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.G, n, 0.0, bk.delX, n));
+    MSDP_TRY(symmetrize(st, n, bk.delX, n));
+    MSDP_TRY(jtprod(st, bk.ops, dely, bk.tmp2));
+    MSDP_TRY(axpby(st, n2, 1.0, bk.Rd, -1.0, bk.tmp2, bk.delS));
+    MSDP_TRY(symmetrize(st, n, bk.delS, n));
+    off += (int64_t)n * (n + 1) / 2;
+  }
+  if (ns > 0) {
+    MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, dely, m, 0.0, ws1, ns));
+    MSDP_TRY(axpby(st, ns, 1.0, Rdlin, -1.0, ws1, delz));
+    MSDP_TRY(qr_delx(st, (int)ns, x, z, qy, off, delx));
+  }
+  return 0;
+}
+
+// ==========================================================================
 // C ABI
 // ==========================================================================
 #define GUARD(s) \
@@ -1165,7 +1290,6 @@ int msdp_solver_configure(msdp_solver* s, const msdp_options* o) {
   if (o->schur == MSDP_SCHUR_QR &&
       !(o->algorithm == MSDP_ALG_PRIMAL_DUAL && o->direction == MSDP_DIR_NT))
     return as_err("QRSchur requires PrimalDual with the NT direction.");
-  if (o->schur == MSDP_SCHUR_QR) return as_err("QRSchur is not available in this build");
   s->opt = *o;
   return 0;
 }
