@@ -38,6 +38,16 @@ def build_model(workload):
         return instances.maxcut_100(), dict(
             workload="max-cut G(100,0.1) unit weights: one 100x100 PSD block, m=100, plain FP64",
             n=100, m=100, seed=0)
+    if workload == "dense250":
+        return instances.dense_250_m8000(), dict(
+            workload="random dense SDP: one 250x250 PSD block + 400 nonnegative scalars, m=8000 "
+                     "dense constraints, NT + EigenStep + Mehrotra, DenseCholesky, tol 1e-7",
+            n=250, m=8000, n_scalar=400, seed=2)
+    if workload == "blocks96":
+        return instances.blocks_96_m6000(), dict(
+            workload="block-diagonal SOS-style SDP: 96 PSD blocks of 8..120, 200 scalars, "
+                     "m=6000, each constraint dense on 1-3 blocks, tol 1e-7",
+            m=6000, n_scalar=200, seed=3)
     if workload.startswith("maxcut"):
         n = int(workload[6:])
         return instances.maxcut(n, 10.0 / n, 1, "pm1"), dict(

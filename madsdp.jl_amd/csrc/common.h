@@ -125,6 +125,21 @@ size_t syevd_iwork_elems(int n);
 // Smallest eigenvalue of the symmetric matrix A into *out_dev.
 int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out_dev,
             double* work, int* iwork);
+// One eigenproblem of a variable-size batch (n <= 128): eigenvalues into w
+// (ascending), optionally w[0] also into *wmin, vectors into V (ld ldv) with
+// Vtmp (n x n) as scratch.
+struct EigBatch {
+  int n;
+  const double* A;
+  int64_t lda;
+  double* w;
+  double* wmin;
+  double* V;
+  int64_t ldv;
+  double* Vtmp;
+};
+constexpr int JACOBI_VB_MAX = 128;
+int jacobi_vbatched(hipStream_t s, const EigBatch* items_dev, int count, bool vectors);
 // Batched Jacobi for many small matrices (n <= 64).
 int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t strideA,
                    double* w, int64_t stridew, double* V, int64_t ldv, int64_t strideV,

@@ -147,6 +147,133 @@ int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t s
 }
 
 // ==========================================================================
+// Variable-size batched Jacobi (n <= 128): one workgroup per matrix of the
+// batch, the matrix in LDS, the rotations accumulated in a global scratch
+// (Vtmp, n x n) and the sorted vectors written to V.  One launch covers the
+// eigenproblems of all small PSD blocks of a solve.
+// ==========================================================================
+__global__ __launch_bounds__(512) void jacobi_vb_kernel(const EigBatch* items, int vectors) {
+  const EigBatch it = items[blockIdx.x];
+  const int n = it.n;
+  if (n <= 0) return;
+  __shared__ double A[128][129];
+  __shared__ double cs[64], sn[64];
+  __shared__ int pp[64], qq[64];
+  __shared__ int rotated;
+  __shared__ double fro;
+  __shared__ double red[16];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = n + (n & 1);
+  double v = 0.0;
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    const double a = it.A[i + (int64_t)j * it.lda];
+    A[i][j] = a;
+    v += a * a;
+  }
+  v = block_reduce<0>(v, red);
+  if (tid == 0) fro = sqrt(v);
+  double* Vt = it.Vtmp;
+  if (vectors)
+    for (int e = tid; e < n * n; e += nt) Vt[e] = (e % n) == (e / n) ? 1.0 : 0.0;
+  __syncthreads();
+  const double absthr = 1e-3 * DEPS * fro;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int round = 0; round < np - 1; ++round) {
+      if (tid < np / 2) {
+        const int i = tid;
+        const int ka = i, kb = np - 1 - i;
+        int p = ka == 0 ? 0 : ((ka - 1 + round) % (np - 1)) + 1;
+        int q = kb == 0 ? 0 : ((kb - 1 + round) % (np - 1)) + 1;
+        if (p > q) { const int t = p; p = q; q = t; }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = A[p][q], app = A[p][p], aqq = A[q][q];
+          if (fabs(apq) > fmax(DEPS * sqrt(fabs(app) * fabs(aqq)), absthr) && apq != 0.0) {
+            const double th = (aqq - app) / (2.0 * apq);
+            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+            rotated = 1;
+          }
+        }
+        pp[i] = p; qq[i] = q; cs[i] = c; sn[i] = s;
+      }
+      __syncthreads();
+      const int npairs = np / 2;
+      for (int e = tid; e < npairs * n; e += nt) {
+        const int i = e / n, k = e % n;
+        const double s = sn[i];
+        if (s == 0.0) continue;
+        const double c = cs[i];
+        const int p = pp[i], q = qq[i];
+        const double rp = A[p][k], rq = A[q][k];
+        A[p][k] = c * rp - s * rq;
+        A[q][k] = s * rp + c * rq;
+      }
+      __syncthreads();
+      for (int e = tid; e < npairs * n; e += nt) {
+        const int i = e / n, k = e % n;
+        const double s = sn[i];
+        if (s == 0.0) continue;
+        const double c = cs[i];
+        const int p = pp[i], q = qq[i];
+        const double ap = A[k][p], aq = A[k][q];
+        A[k][p] = c * ap - s * aq;
+        A[k][q] = s * ap + c * aq;
+        if (vectors) {
+          double* vp = Vt + k + (int64_t)p * n;
+          double* vq = Vt + k + (int64_t)q * n;
+          const double x = *vp, y = *vq;
+          *vp = c * x - s * y;
+          *vq = s * x + c * y;
+        }
+      }
+      __syncthreads();
+      if (tid < npairs && sn[tid] != 0.0) {
+        A[pp[tid]][qq[tid]] = 0.0;
+        A[qq[tid]][pp[tid]] = 0.0;
+      }
+      __syncthreads();
+    }
+    if (!rotated) break;
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += nt) {
+    const double di = A[i][i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[j][j];
+      r += (dj < di) || (dj == di && j < i);
+    }
+    it.w[r] = di;
+    if (r == 0 && it.wmin) *it.wmin = di;
+  }
+  if (!vectors) return;
+  __syncthreads();
+  // sorted columns: rank of column i computed again per element (n <= 128)
+  for (int e = tid; e < n * n; e += nt) {
+    const int k = e % n, i = e / n;
+    const double di = A[i][i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[j][j];
+      r += (dj < di) || (dj == di && j < i);
+    }
+    it.V[k + (int64_t)r * it.ldv] = Vt[k + (int64_t)i * n];
+  }
+}
+
+int jacobi_vbatched(hipStream_t s, const EigBatch* items, int count, bool vectors) {
+  if (count <= 0) return 0;
+  jacobi_vb_kernel<<<count, 512, 0, s>>>(items, vectors ? 1 : 0);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
 // Householder tridiagonalisation (lower), A = Q T Q',  Q = H_0 ... H_{n-3}
 // ==========================================================================
 struct TrdWork {

@@ -83,6 +83,10 @@ struct msdp_solver {
   double* swork = nullptr;
   size_t swork_elems = 0;
   double* eigall = nullptr;  // DualCentering proximity eigenvalues, all blocks
+  double* eigw = nullptr;    // step-length eigenvalues of the small blocks
+  // batched Jacobi descriptors of the small (n <= 128) blocks
+  EigBatch *eb_nt = nullptr, *eb_step = nullptr, *eb_dc = nullptr;
+  int nsmall = 0;
   // QRSchur workspace (qr_schur.jl:5-10): B with the reflectors, scale, tau
   double *qB = nullptr, *qscale = nullptr, *qtau = nullptr, *qbase = nullptr, *qy = nullptr,
          *qc = nullptr, *qt = nullptr, *qrd = nullptr;
@@ -153,6 +157,7 @@ struct msdp_solver {
   // PrimalDual
   int update_residuals();
   int prepare(bool& ok);
+  int nt_scaling(Blk& bk, int i);
   int factorize_schur(bool& ok);
   int apply_E(Blk& b, const double* M, bool diag, double* out);
   int solve_directions(double mu_, bool corrected);
@@ -308,6 +313,30 @@ int msdp_solver::finalize() {
   int64_t tot = 0;
   for (auto& bb : blk) tot += bb.n;
   MSDP_TRY(alloc(&eigall, std::max<int64_t>(tot, 1)));
+  MSDP_TRY(alloc(&eigw, 2 * std::max<int64_t>(tot, 1)));
+  {
+    std::vector<EigBatch> nt_items, step_items, dc_items;
+    int64_t off = 0;
+    for (int i = 0; i < nb; ++i) {
+      Blk& bk = blk[i];
+      if (bk.n <= JACOBI_VB_MAX) {
+        nt_items.push_back(EigBatch{bk.n, bk.Nm, bk.n, bk.lam, nullptr, bk.Vv, bk.n, bk.Mm});
+        step_items.push_back(EigBatch{bk.n, bk.tmp2, bk.n, eigw + 2 * off, sc + bslot(i, 3), nullptr, 0, nullptr});
+        step_items.push_back(EigBatch{bk.n, bk.buf2, bk.n, eigw + 2 * off + bk.n, sc + bslot(i, 4), nullptr, 0, nullptr});
+        dc_items.push_back(EigBatch{bk.n, bk.tmp2, bk.n, eigall + off, nullptr, nullptr, 0, nullptr});
+      }
+      off += bk.n;
+    }
+    nsmall = (int)nt_items.size();
+    if (nsmall > 0) {
+      MSDP_TRY(alloc(&eb_nt, nsmall));
+      MSDP_TRY(alloc(&eb_step, 2 * nsmall));
+      MSDP_TRY(alloc(&eb_dc, nsmall));
+      MSDP_HIP(hipMemcpy(eb_nt, nt_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(eb_step, step_items.data(), sizeof(EigBatch) * 2 * nsmall, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpy(eb_dc, dc_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice));
+    }
+  }
   // chunk buffers of the dense Schur assembly: up to 64 constraints or 256 MiB
   size_t want = 0;
   for (auto& bb : blk)
@@ -426,16 +455,17 @@ int msdp_solver::prepare(bool& ok) {
       MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n));
       MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Mm, n, bk.Mm, n, 0.0, bk.Nm, n));
       MSDP_TRY(symmetrize(st, n, bk.Nm, n));
-      MSDP_TRY(syevd(st, n, bk.Nm, n, bk.lam, bk.Vv, n, true, ework, eiwork));
-      MSDP_TRY(sqrt_min(st, n, bk.lam, bk.D, slot(bslot(i, 2))));
-      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Vv, n, 0.0, bk.G, n));
-      MSDP_TRY(scale_cols(st, n, n, bk.G, n, bk.D, -1));
-      MSDP_TRY(transpose(st, n, n, bk.Vv, n, bk.tmp, n));
-      MSDP_TRY(scale_rows(st, n, n, bk.tmp, n, bk.D, +1));
-      MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, bk.Lxinv, n, 0.0, bk.Gi, n));
-      MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.G, n, bk.G, n, 0.0, bk.W, n));
-      MSDP_TRY(symmetrize(st, n, bk.W, n));
+      if (n > JACOBI_VB_MAX) {
+        MSDP_TRY(syevd(st, n, bk.Nm, n, bk.lam, bk.Vv, n, true, ework, eiwork));
+        MSDP_TRY(nt_scaling(bk, i));
+      }
     }
+  }
+  if (nt() && nsmall > 0) {
+    // eigen-decompositions of all small blocks in one launch
+    MSDP_TRY(jacobi_vbatched(st, eb_nt, nsmall, true));
+    for (int i = 0; i < nb; ++i)
+      if (blk[i].n <= JACOBI_VB_MAX) MSDP_TRY(nt_scaling(blk[i], i));
   }
   if (ns > 0) MSDP_TRY(scalar_op(st, SOP_ZINV, (int)ns, 0, nullptr, z, nullptr, nullptr, nullptr, zinv, nullptr, false));
   std::vector<int> inf;
@@ -446,6 +476,21 @@ int msdp_solver::prepare(bool& ok) {
     if (inf[8 + 4 * i] != 0 || inf[8 + 4 * i + 1] != 0) ok = false;
     if (ok && nt() && !(hsc[bslot(i, 2)] > 0.0)) ok = false;  // directions.jl:55-58
   }
+  return 0;
+}
+
+// D = sqrt(lambda), G = Lx V D^-1/2, Gi = D^1/2 V' Lx^-1, W = G G'
+// (directions.jl:55-73) from the eigen-decomposition of N in (lam, Vv).
+int msdp_solver::nt_scaling(Blk& bk, int i) {
+  const int n = bk.n;
+  MSDP_TRY(sqrt_min(st, n, bk.lam, bk.D, slot(bslot(i, 2))));
+  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Vv, n, 0.0, bk.G, n));
+  MSDP_TRY(scale_cols(st, n, n, bk.G, n, bk.D, -1));
+  MSDP_TRY(transpose(st, n, n, bk.Vv, n, bk.tmp, n));
+  MSDP_TRY(scale_rows(st, n, n, bk.tmp, n, bk.D, +1));
+  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, bk.Lxinv, n, 0.0, bk.Gi, n));
+  MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.G, n, bk.G, n, 0.0, bk.W, n));
+  MSDP_TRY(symmetrize(st, n, bk.W, n));
   return 0;
 }
 
@@ -590,13 +635,16 @@ int msdp_solver::step_lengths(double& a, double& bt) {
       const int n = bk.n;
       const double* Ls[2] = {bk.Lxinv, bk.Lsinv};
       const double* Dl[2] = {bk.delX, bk.delS};
+      double* T[2] = {bk.tmp2, bk.buf2};
       for (int k = 0; k < 2; ++k) {
         MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
-        MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, bk.tmp2, n));
-        MSDP_TRY(symmetrize(st, n, bk.tmp2, n));
-        MSDP_TRY(eig_min(st, n, bk.tmp2, n, slot(bslot(i, 3 + k)), ework, eiwork));
+        MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n));
+        MSDP_TRY(symmetrize(st, n, T[k], n));
+        if (n > JACOBI_VB_MAX)
+          MSDP_TRY(eig_min(st, n, T[k], n, slot(bslot(i, 3 + k)), ework, eiwork));
       }
     }
+    if (nsmall > 0) MSDP_TRY(jacobi_vbatched(st, eb_step, 2 * nsmall, false));
     if (ns > 0) {
       MSDP_TRY(scalar_min_step(st, (int)ns, x, delx, slot(3)));
       MSDP_TRY(scalar_min_step(st, (int)ns, z, delz, slot(4)));
@@ -976,13 +1024,15 @@ int msdp_solver::predictor_step(bool& ok) {  // :232-264
     MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Dc, n, 0.0, bk.tmp, n));
     MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lx, n, 0.0, bk.tmp2, n));
     MSDP_TRY(symmetrize(st, n, bk.tmp2, n));
-    MSDP_TRY(syevd(st, n, bk.tmp2, n, eigall + off, bk.Vv, n, false, ework, eiwork));
+    if (n > JACOBI_VB_MAX)
+      MSDP_TRY(syevd(st, n, bk.tmp2, n, eigall + off, bk.Vv, n, false, ework, eiwork));
     // ||Ls^-1 Shat Ls^-T||_F^2 (:219-229)
     MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lsinv, n, bk.Shat, n, 0.0, bk.tmp, n));
-    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lsinv, n, 0.0, bk.tmp2, n));
-    MSDP_TRY(dot(st, n2, bk.tmp2, bk.tmp2, slot(bslot(i, 6)), red));
+    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lsinv, n, 0.0, bk.buf2, n));
+    MSDP_TRY(dot(st, n2, bk.buf2, bk.buf2, slot(bslot(i, 6)), red));
     off += n;
   }
+  if (nsmall > 0) MSDP_TRY(jacobi_vbatched(st, eb_dc, nsmall, false));
   std::vector<int> inf;
   MSDP_TRY(read_infos(inf));
   for (int i = 0; i < nb; ++i)
