@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "madsdp.jl_amd"))
 
 METRIC = "solves/s + p50/p99 latency @1e-7 gap: mixed/batched vs FP64 plain, 1 MI355X"
-PEAKS = {"mfma": 78.6, "hbm": 8000.0}  # FP64 MFMA TFLOP/s, HBM GB/s
+PEAKS = {"mfma": 78.6, "mfma32": 157.3, "hbm": 8000.0}  # FP64/FP32 MFMA TFLOP/s, HBM GB/s
 
 
 def build_model(workload):
@@ -95,7 +95,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="maxcut3000")
-    ap.add_argument("--profile", default="gemm", choices=["gemm", "trd", "none"])
+    ap.add_argument("--profile", default="gemm", choices=["gemm", "trd", "gemm32", "none"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -109,11 +110,11 @@ def main():
     from madsdp_amd import MadSDPSolver, Status
 
     model, config = build_model(args.workload)
-    s = MadSDPSolver(model, device=0)
+    s = MadSDPSolver(model, device=0, precision=args.precision)
     for _ in range(args.warmup):
         s.solve()
         print(f"warmup solve: status {s.status.name}, {s.iter} iterations", file=sys.stderr, flush=True)
-    cls = {"gemm": 0, "trd": 1, "none": -1}[args.profile]
+    cls = {"gemm": 0, "trd": 1, "gemm32": 2, "none": -1}[args.profile]
     if cls >= 0:
         s.lib.msdp_solver_profile_begin(s._h, cls)
     lat = []
@@ -130,11 +131,13 @@ def main():
         ms, launches, work = C.c_double(), C.c_int64(), C.c_double()
         s.lib.msdp_solver_profile_end(s._h, C.byref(ms), C.byref(launches), C.byref(work))
         if launches.value > 0 and ms.value > 0:
-            if cls == 0:
+            if cls in (0, 2):
                 achieved = work.value / (ms.value * 1e-3) / 1e12
-                roof = {"bound": "mfma", "achieved": achieved, "peak": PEAKS["mfma"],
-                        "unit": "TFLOP/s", "frac": achieved / PEAKS["mfma"], "traffic": None,
-                        "kernel": "gemm_f64_kernel (FP64 MFMA 16x16x4)",
+                peak = PEAKS["mfma"] if cls == 0 else PEAKS["mfma32"]
+                roof = {"bound": "mfma", "achieved": achieved, "peak": peak,
+                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                        "kernel": "gemm_f64_kernel (FP64 MFMA 16x16x4)" if cls == 0
+                        else "gemm128_f32_kernel (FP32 MFMA 16x16x4)",
                         "launches": launches.value, "avg_launch_us": 1e3 * ms.value / launches.value,
                         "algorithmic_per_launch": work.value / launches.value,
                         "kernel_share_of_solve": ms.value * 1e-3 / total}
@@ -163,6 +166,7 @@ def main():
         "config": config,
         "p50_latency_ms": float(np.percentile(lat_ms, 50)),
         "p99_latency_ms": float(np.percentile(lat_ms, 99)),
+        "precision": args.precision,
         "iterations": int(s.iter),
         "status": s.status.name,
         "objective": s.primal_objective(),

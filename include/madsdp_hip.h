@@ -66,6 +66,14 @@ typedef struct msdp_options {
   double initial_shift, max_shift; /* DualShift 1, 1e10 */
   double dc_beta, dc_delta, dc_A;  /* DualCentering 0.2, 1, 2 (dc_A < 0: theoretical) */
   int32_t dc_centering_max_iter;   /* 50 */
+  /* precision of the Schur assembly: 0 = FP64 (the reference's arithmetic);
+   * 1 = mixed: FP32 MFMA assembly of the dense-constraint blocks while
+   * mu >= mu_switch, FP64 Cholesky, and every Schur solve refined against
+   * the FP64 operator (at most refine_max_iter steps, FP64 assembly if the
+   * refinement stalls).  Not in the reference: an extension of this build. */
+  int32_t precision;               /* 0 */
+  double mu_switch;                /* 1e-5 */
+  int32_t refine_max_iter;         /* 10 */
 } msdp_options;
 
 /* iteration data of MadSDPSolver (src/solver.jl:37-48) */
@@ -74,6 +82,7 @@ typedef struct msdp_info {
   double pobj, dobj;
   int32_t iter, status, n_newton, n_predictor;
   double dc_t, dc_lambda;
+  int32_t lowp_factorizations, fp64_fallbacks, refine_steps, pad;
 } msdp_info;
 
 /* which iterate array msdp_solver_get/set moves */
@@ -170,6 +179,11 @@ size_t msdp_k_syevd_iwork(int32_t n);
 /* smallest eigenvalue of a symmetric matrix into *out_dev (LRO.lanczos_eigmin!) */
 int msdp_k_eig_min(void* stream, int32_t n, const double* A, int64_t lda, double* out_dev,
                    double* work);
+/* C(FP64) = alpha op(A) op(B) + beta C with FP32 A, B on FP32 MFMA (the
+ * mixed-precision Schur assembly product); no counterpart in the reference */
+int msdp_k_gemm32(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
+                  float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+                  double beta, double* C, int64_t ldc);
 /* Hadamard Schur complement of one-hot constraints: H[i,j] += L[i,j] R[j,i]
  * for i >= j (max-cut H = W o W, src/components.jl:102-104) */
 int msdp_k_schur_onehot(void* stream, int32_t m, const double* L, int64_t ldl, const double* R,

@@ -145,6 +145,19 @@ int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t s
                    double* w, int64_t stridew, double* V, int64_t ldv, int64_t strideV,
                    int batch, bool vectors);
 
+// --------------------------------------------------------------------------
+// FP32 MFMA GEMM and conversions (gemm32.hip), for the mixed-precision
+// Schur assembly
+// --------------------------------------------------------------------------
+int gemm32(hipStream_t s, bool ta, bool tb, int M, int N, int K, float alpha, const float* A,
+           int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc,
+           int batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0, bool lower = false);
+int gemm32_to64(hipStream_t s, bool ta, bool tb, int M, int N, int K, float alpha,
+                const float* A, int64_t lda, const float* B, int64_t ldb, double beta,
+                double* C, int64_t ldc, bool lower = false);
+int d2f(hipStream_t s, int64_t count, const double* x, float* y);
+int svec_pack32(hipStream_t s, int n, int cb, const float* P, float* B, int64_t ldb);
+
 }  // namespace msdp
 
 namespace msdp {
@@ -153,7 +166,7 @@ namespace msdp {
 // launch of that class is bracketed by HIP events on that stream and its
 // algorithmic work (flops or bytes) is accumulated.
 // --------------------------------------------------------------------------
-enum ProfClass { PROF_GEMM = 0, PROF_TRD = 1, PROF_NONE = -1 };
+enum ProfClass { PROF_GEMM = 0, PROF_TRD = 1, PROF_GEMM32 = 2, PROF_NONE = -1 };
 bool prof_active(int cls, hipStream_t s);
 void prof_before(hipStream_t s);
 void prof_after(hipStream_t s, double work);

@@ -46,6 +46,8 @@ struct SchurScaling {
 
 int jprod(hipStream_t st, const BlockOps& b, const double* X, double* out);
 int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out);
+int scatter_lower_add(hipStream_t st, int mi, const int* con, const double* Hb, double* H,
+                      int64_t ldh);
 int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
                 double* work, size_t work_elems);
 

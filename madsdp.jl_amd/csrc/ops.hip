@@ -135,6 +135,15 @@ __global__ void scatter_lower_kernel(int mi, const int* dcon, const double* Hb, 
   H[dcon[a] + (int64_t)dcon[b] * ldh] += Hb[a + (int64_t)b * mi];
 }
 
+int scatter_lower_add(hipStream_t st, int mi, const int* con, const double* Hb, double* H,
+                      int64_t ldh) {
+  if (mi <= 0) return 0;
+  dim3 g(cdiv(mi, 128), mi);
+  scatter_lower_kernel<<<g, 128, 0, st>>>(mi, con, Hb, H, ldh);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
 int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
                 double* work, size_t work_elems) {
   const int n = b.n;

@@ -50,6 +50,9 @@ struct Blk {
   double *X, *S, *Rd, *Ls, *Si, *Lsinv, *delX, *delS, *Lx, *Lxinv, *corr, *buf, *buf2, *tmp,
       *tmp2, *W, *G, *Gi, *D, *RNT, *Mm, *Nm, *Vv, *lam, *Shat, *Dc, *Lh;
   bool rd_nonzero = true;
+  // mixed precision: FP32 copies of the dense constraints, the scaled
+  // constraint panel and the scaling factors
+  float *mats32 = nullptr, *B32 = nullptr, *Lm32 = nullptr, *F32 = nullptr;
   // host copies of the constraint data until finalize
   std::vector<int> h_con, h_row, h_col;
   std::vector<double> h_val;
@@ -87,6 +90,13 @@ struct msdp_solver {
   // batched Jacobi descriptors of the small (n <= 128) blocks
   EigBatch *eb_nt = nullptr, *eb_step = nullptr, *eb_dc = nullptr;
   int nsmall = 0;
+  // mixed precision state
+  float* swork32 = nullptr;
+  size_t swork32_elems = 0;
+  double *rh = nullptr, *rr = nullptr, *rdl = nullptr;
+  double reg_used = 0.0;
+  bool h_approx = false, force_fp64 = false;
+  int lowp_count = 0, fallback_count = 0, refine_steps = 0;
   // QRSchur workspace (qr_schur.jl:5-10): B with the reflectors, scale, tau
   double *qB = nullptr, *qscale = nullptr, *qtau = nullptr, *qbase = nullptr, *qy = nullptr,
          *qc = nullptr, *qt = nullptr, *qrd = nullptr;
@@ -158,6 +168,10 @@ struct msdp_solver {
   int update_residuals();
   int prepare(bool& ok);
   int nt_scaling(Blk& bk, int i);
+  int ensure_mixed();
+  int schur_block_fp32(Blk& bk, const SchurScaling& sc);
+  int apply_H(const double* v, double* out);
+  int schur_solve(double* v);
   int factorize_schur(bool& ok);
   int apply_E(Blk& b, const double* M, bool diag, double* out);
   int solve_directions(double mu_, bool corrected);
@@ -302,6 +316,9 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&Ht, mm * mm));
   MSDP_TRY(alloc(&Hdinv, potrf_work_elems((int)mm)));
   MSDP_TRY(alloc(&wm2, mm));
+  MSDP_TRY(alloc(&rh, mm));
+  MSDP_TRY(alloc(&rr, mm));
+  MSDP_TRY(alloc(&rdl, mm));
   MSDP_TRY(alloc(&info, 4 * nb + 8));
   nslots = 32 + 8 * nb;
   MSDP_TRY(alloc(&sc, nslots));
@@ -499,6 +516,10 @@ int msdp_solver::factorize_schur(bool& ok) {
   ok = false;
   if (opt.schur == MSDP_SCHUR_QR) return qr_factorize(ok);
   if (m == 0) { ok = true; return 0; }
+  // mixed precision: FP32 assembly of the dense blocks while mu >= mu_switch
+  const bool lowp = opt.precision == 1 && !dual_centering() && mu >= opt.mu_switch && !force_fp64;
+  bool any_lowp = false;
+  if (lowp) MSDP_TRY(ensure_mixed());
   MSDP_TRY(fill(st, m * m, 0.0, Hs));
   for (auto& bk : blk) {
     SchurScaling sc;
@@ -518,6 +539,11 @@ int msdp_solver::factorize_schur(bool& ok) {
       }
     }
     (void)n;
+    if (lowp && bk.ops.kind == BlockOps::DENSE) {
+      MSDP_TRY(schur_block_fp32(bk, sc));
+      any_lowp = true;
+      continue;
+    }
     MSDP_TRY(schur_block(st, bk.ops, sc, Hs, m, swork, swork_elems));
   }
   if (ns > 0) {
@@ -537,10 +563,19 @@ int msdp_solver::factorize_schur(bool& ok) {
     MSDP_HIP(hipStreamSynchronize(st));
     if (hinfo == 0) {
       MSDP_TRY(transpose(st, (int)m, (int)m, H, m, Ht, m));
+      reg_used = reg > 0 ? reg : 0.0;
+      h_approx = any_lowp;
+      if (any_lowp) ++lowp_count;
       ok = true;
       return 0;
     }
     reg = reg == 0 ? 1e-12 : 10 * reg;
+  }
+  if (any_lowp) {
+    // the FP32-assembled H is not numerically PD: FP64 from here on
+    ++fallback_count;
+    force_fp64 = true;
+    return factorize_schur(ok);
   }
   return 0;
 }
@@ -579,7 +614,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
       MSDP_TRY(scalar_op(st, SOP_RHS, (int)ns, mu_, x, z, zinv, Rdlin, corrlin, ws1, nullptr, corrected));
       MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, ws1, ns, 1.0, dely, m));
     }
-    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dely, wm2));
+    MSDP_TRY(schur_solve(dely));
   }
   // directions
   for (auto& bk : blk) {
@@ -917,6 +952,8 @@ int msdp_solver::initialize() {
   start_time = now_s();
   trace.clear();
   alpha = beta = sigma = 0.0;
+  force_fp64 = false;
+  lowp_count = fallback_count = refine_steps = 0;
   if (dual_centering()) return dc_initialize();
   MSDP_TRY(update_residuals());
   double compl_ = 0;
@@ -1078,6 +1115,122 @@ int msdp_solver::dc_iterate() {  // :266-289
 }
 
 // ==========================================================================
+// Mixed precision: FP32 MFMA Schur assembly + FP64 iterative refinement
+// ==========================================================================
+int msdp_solver::ensure_mixed() {
+  size_t want = 0;
+  for (auto& bk : blk) {
+    if (bk.ops.kind != BlockOps::DENSE || bk.mats32) continue;
+    const size_t n2 = (size_t)bk.n * bk.n;
+    const int mi = bk.ops.ncon;
+    MSDP_TRY(alloc(&bk.mats32, n2 * mi));
+    MSDP_TRY(d2f(st, (int64_t)(n2 * mi), bk.ops.mats, bk.mats32));
+    MSDP_TRY(alloc(&bk.B32, n2 * mi));  // vec (HKM) is the larger layout
+    MSDP_TRY(alloc(&bk.Lm32, n2));
+    MSDP_TRY(alloc(&bk.F32, n2));
+    want = std::max(want, 2 * n2 * (size_t)std::min(mi, 64));
+  }
+  if (want > swork32_elems) {
+    MSDP_TRY(alloc(&swork32, want));
+    swork32_elems = want;
+  }
+  return 0;
+}
+
+// <A_j, L A_k R> of a dense block with FP32 operands and accumulation:
+// P_j = op(Lm) A_j op(F2) by batched FP32 GEMMs, B = svec(P) (vec for HKM),
+// H += B' B widened to FP64.
+int msdp_solver::schur_block_fp32(Blk& bk, const SchurScaling& sc) {
+  const int n = bk.n;
+  const int64_t n2 = (int64_t)n * n;
+  const BlockOps& o = bk.ops;
+  const int mi = o.ncon;
+  const int64_t drow = sc.full_vec ? n2 : (int64_t)n * (n + 1) / 2;
+  MSDP_TRY(d2f(st, n2, sc.Lm, bk.Lm32));
+  MSDP_TRY(d2f(st, n2, sc.F2, bk.F32));
+  const int cb = (int)std::max<int64_t>(1, std::min<int64_t>(mi, (int64_t)(swork32_elems / (2 * n2))));
+  float* T = swork32;
+  float* P = swork32 + (int64_t)cb * n2;
+  for (int j0 = 0; j0 < mi; j0 += cb) {
+    const int c = std::min(cb, mi - j0);
+    MSDP_TRY(gemm32(st, false, sc.transF2, n, n, n, 1.0f, bk.mats32 + (int64_t)j0 * n2, n, bk.F32,
+                    n, 0.0, T, n, c, n2, 0, n2));
+    float* dst = sc.full_vec ? bk.B32 + (int64_t)j0 * n2 : P;
+    MSDP_TRY(gemm32(st, sc.transL, false, n, n, n, 1.0f, bk.Lm32, n, T, n, 0.0, dst, n, c, 0, n2, n2));
+    if (!sc.full_vec) MSDP_TRY(svec_pack32(st, n, c, P, bk.B32 + (int64_t)j0 * drow, drow));
+  }
+  if (o.dcon_identity) {
+    MSDP_TRY(gemm32_to64(st, true, false, mi, mi, (int)drow, 1.0f, bk.B32, drow, bk.B32, drow, 1.0,
+                         Hs, m, true));
+  } else {
+    MSDP_TRY(gemm32_to64(st, true, false, mi, mi, (int)drow, 1.0f, bk.B32, drow, bk.B32, drow, 0.0,
+                         o.Hbuf, mi, true));
+    MSDP_TRY(scatter_lower_add(st, mi, o.con_list, o.Hbuf, Hs, m));
+  }
+  return 0;
+}
+
+// out = (H + reg I) v with the FP64 operator H_jk = <A_j, L A_k R> +
+// C_lin diag(x ./ z) C_lin', applied matrix-free: A(L A*(v) R).
+int msdp_solver::apply_H(const double* v, double* out) {
+  MSDP_TRY(axpby(st, m, reg_used, v, 0.0, v, out));
+  for (auto& bk : blk) {
+    if (bk.ops.kind == BlockOps::EMPTY) continue;
+    const int n = bk.n;
+    const double *L, *R;
+    if (dual_centering()) { L = bk.Si; R = bk.Si; }
+    else if (nt()) { L = bk.W; R = bk.W; }
+    else { L = bk.Si; R = bk.X; }
+    MSDP_TRY(jtprod(st, bk.ops, v, bk.tmp2));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, L, n, bk.tmp2, n, 0.0, bk.tmp, n));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, R, n, 0.0, bk.buf, n));
+    MSDP_TRY(symmetrize(st, n, bk.buf, n));
+    MSDP_TRY(jprod(st, bk.ops, bk.buf, out));
+  }
+  if (ns > 0) {
+    MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, v, m, 0.0, ws1, ns));
+    MSDP_TRY(scalar_op(st, SOP_XZ, (int)ns, 0, x, nullptr, zinv, nullptr, nullptr, ws2, nullptr, false));
+    MSDP_TRY(qr_mul(st, (int)ns, ws1, ws2, ws1));
+    MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, ws1, ns, 1.0, out, m));
+  }
+  return 0;
+}
+
+// schur_solve! (schur_solvers.jl:21).  With an FP32-assembled H, the FP64
+// solution is recovered by iterative refinement against the FP64 operator;
+// if it stalls above 1e-9 relative, H is re-assembled in FP64.
+int msdp_solver::schur_solve(double* v) {
+  if (!h_approx) return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+  MSDP_HIP(hipMemcpyAsync(rh, v, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+  MSDP_TRY(dot(st, m, rh, rh, slot(15), red));
+  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2));
+  double prev = INFINITY, hn = 0.0, rn = 0.0;
+  for (int it = 0;; ++it) {
+    MSDP_TRY(apply_H(v, rr));
+    MSDP_TRY(axpby(st, m, 1.0, rh, -1.0, rr, rr));
+    MSDP_TRY(dot(st, m, rr, rr, slot(16), red));
+    MSDP_TRY(read_slots());
+    hn = std::sqrt(hsc[15]);
+    rn = std::sqrt(hsc[16]);
+    if (!(rn > 1e-15 * hn) || it >= opt.refine_max_iter || !(rn < 0.5 * prev)) break;
+    prev = rn;
+    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, rr, wm2));
+    MSDP_TRY(axpby(st, m, 1.0, v, 1.0, rr, v));
+    ++refine_steps;
+  }
+  if (rn <= 1e-9 * hn || !std::isfinite(hn)) return 0;
+  // stalled: FP64 assembly from here on (the conditioning of H only grows
+  // as mu decreases)
+  ++fallback_count;
+  force_fp64 = true;
+  bool ok = false;
+  MSDP_TRY(factorize_schur(ok));
+  if (!ok) { set_error("FP64 re-factorisation of the Schur complement failed"); return -4; }
+  MSDP_HIP(hipMemcpyAsync(v, rh, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+}
+
+// ==========================================================================
 // QRSchur (src/qr_schur.jl)
 // ==========================================================================
 // factorize_schur! (qr_schur.jl:24-88)
@@ -1228,6 +1381,9 @@ void msdp_default_options(msdp_options* o) {
   o->dc_delta = 1.0;
   o->dc_A = 2.0;
   o->dc_centering_max_iter = 50;
+  o->precision = 0;
+  o->mu_switch = 1e-5;
+  o->refine_max_iter = 10;
 }
 
 int msdp_solver_create(msdp_solver** out, int device, int32_t nblocks, const int64_t* dims,
@@ -1376,6 +1532,8 @@ int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
   o->inf_pr = s->inf_pr; o->inf_du = s->inf_du; o->inf_compl = s->inf_compl; o->obj_val = s->obj_val;
   o->pobj = s->pobj; o->dobj = s->dobj; o->iter = s->iter; o->status = s->status;
   o->n_newton = s->n_newton; o->n_predictor = s->n_predictor; o->dc_t = s->dc_t; o->dc_lambda = s->dc_lambda;
+  o->lowp_factorizations = s->lowp_count; o->fp64_fallbacks = s->fallback_count;
+  o->refine_steps = s->refine_steps; o->pad = 0;
   return 0;
 }
 
@@ -1506,6 +1664,12 @@ int msdp_k_syevd(void* stream, int32_t n, const double* A, int64_t lda, double* 
 }
 int msdp_k_eig_min(void* stream, int32_t n, const double* A, int64_t lda, double* out, double* work) {
   return eig_min((hipStream_t)stream, n, A, lda, out, work, nullptr);
+}
+int msdp_k_gemm32(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K,
+                  float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+                  double beta, double* C, int64_t ldc) {
+  return gemm32_to64((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta,
+                     C, ldc, false);
 }
 int msdp_k_schur_onehot(void* stream, int32_t m, const double* L, int64_t ldl, const double* R, int64_t ldr,
                         double* H, int64_t ldh) {

@@ -28,6 +28,7 @@ class Options(C.Structure):
         ("identity_scale", C.c_double), ("initial_shift", C.c_double), ("max_shift", C.c_double),
         ("dc_beta", C.c_double), ("dc_delta", C.c_double), ("dc_A", C.c_double),
         ("dc_centering_max_iter", C.c_int32),
+        ("precision", C.c_int32), ("mu_switch", C.c_double), ("refine_max_iter", C.c_int32),
     ]
 
 
@@ -38,6 +39,8 @@ class Info(C.Structure):
         ("obj_val", C.c_double), ("pobj", C.c_double), ("dobj", C.c_double),
         ("iter", C.c_int32), ("status", C.c_int32), ("n_newton", C.c_int32),
         ("n_predictor", C.c_int32), ("dc_t", C.c_double), ("dc_lambda", C.c_double),
+        ("lowp_factorizations", C.c_int32), ("fp64_fallbacks", C.c_int32),
+        ("refine_steps", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -87,6 +90,8 @@ SIGNATURES = {
     "msdp_k_syevd_work": (C.c_size_t, [C.c_int32]),
     "msdp_k_syevd_iwork": (C.c_size_t, [C.c_int32]),
     "msdp_k_eig_min": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),
+    "msdp_k_gemm32": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
     "msdp_k_schur_onehot": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, C.c_int64]),
 }
 

@@ -121,7 +121,12 @@ class DualCentering:
     centering_max_iter: int = 50
 
 
-OPTION_FIELDS = ("tol", "max_iter", "max_wall_time", "mu_min", "regularize_schur", "max_schur_reg")
+OPTION_FIELDS = ("tol", "max_iter", "max_wall_time", "mu_min", "regularize_schur", "max_schur_reg",
+                 "mu_switch", "refine_max_iter")
+# Schur assembly precision of the dense constraint blocks: "fp64" (plain) or
+# "mixed" (FP32 MFMA assembly while mu >= mu_switch, FP64 Cholesky and
+# iterative refinement against the FP64 operator; FP64 below mu_switch).
+PRECISIONS = {"fp64": 0, "mixed": 1}
 
 X_, S_, Y_, XLIN, ZLIN, DELX, DELS, DELY, DELXLIN, DELZLIN, RD, RP, RDLIN, W_, G_, SI, CORR, LS, LX, H_, D_ = range(21)
 
@@ -132,8 +137,12 @@ class MadSDPSolver:
     def __init__(self, model: Model, direction=None, step=None, centering=None, algorithm=None,
                  initial_point=None, schur=None, device: int = 0, **options):
         for k in options:
-            if k not in OPTION_FIELDS and k not in ("print_level", "output_file", "datasparsity"):
+            if k not in OPTION_FIELDS and k not in ("print_level", "output_file", "datasparsity",
+                                                    "precision"):
                 raise TypeError(f"unknown option {k}")
+        prec = options.get("precision", "fp64")
+        if prec not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         direction = NT() if direction is None else direction
         step = EigenStep() if step is None else step
         centering = Mehrotra() if centering is None else centering
@@ -163,6 +172,7 @@ class MadSDPSolver:
         for k in OPTION_FIELDS:
             if k in options:
                 setattr(opt, k, options[k])
+        opt.precision = PRECISIONS[prec]
         if isinstance(algorithm, DualCentering):
             opt.algorithm = 1
             opt.dc_beta, opt.dc_delta = algorithm.beta, algorithm.delta
