@@ -168,6 +168,9 @@ def main():
         "p99_latency_ms": float(np.percentile(lat_ms, 99)),
         "precision": args.precision,
         "iterations": int(s.iter),
+        "lowp_factorizations": int(s.info().lowp_factorizations),
+        "refine_steps": int(s.info().refine_steps),
+        "fp64_fallbacks": int(s.info().fp64_fallbacks),
         "status": s.status.name,
         "objective": s.primal_objective(),
         "roofline": roof,

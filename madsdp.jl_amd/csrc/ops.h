@@ -30,7 +30,12 @@ struct BlockOps {
   bool dcon_identity = false;  // con_list == 0..m-1
   double* Bbuf = nullptr;      // svec/vec panel of the scaled constraints
   double* Hbuf = nullptr;      // ncon x ncon block of H before the scatter
+  int jt_split = 1;            // A*(y): constraint ranges summed separately
+  double* jt_part = nullptr;   // [jt_split x n^2] partial sums
 };
+
+// number of constraint ranges of the dense A*(y) for a block of side n
+int jtprod_split(int n, int ncon);
 
 // (L, R) of H_jk = <A_j, L A_k R> and, for dense blocks, the factored form
 // P_j = op(Lm) A_j op(F2) with H_jk = <P_j, P_k>.

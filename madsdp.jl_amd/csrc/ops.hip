@@ -62,13 +62,37 @@ __global__ void jtprod_coo_kernel(int npos, const int64_t* pos_lin, const int* p
   out[pos_lin[p]] = s;
 }
 
-__global__ void jtprod_dense_kernel(int64_t n2, int mi, const int* dcon, const double* mats,
-                                    const double* y, double* out) {
+// out (or part[split]) = sum over the split's constraint range; the ranges
+// are added in a fixed order afterwards, so the result is deterministic.
+__global__ void jtprod_dense_kernel(int64_t n2, int mi, int per, const int* dcon,
+                                    const double* mats, const double* y, double* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n2) return;
+  const int j0 = blockIdx.y * per, j1 = min(mi, j0 + per);
+  double s0 = 0.0, s1 = 0.0;
+  int j = j0;
+  for (; j + 1 < j1; j += 2) {
+    s0 += y[dcon[j]] * mats[(int64_t)j * n2 + e];
+    s1 += y[dcon[j + 1]] * mats[(int64_t)(j + 1) * n2 + e];
+  }
+  if (j < j1) s0 += y[dcon[j]] * mats[(int64_t)j * n2 + e];
+  out[(int64_t)blockIdx.y * n2 + e] = s0 + s1;
+}
+
+__global__ void jtprod_reduce_kernel(int64_t n2, int split, const double* part, double* out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n2) return;
   double s = 0.0;
-  for (int j = 0; j < mi; ++j) s += y[dcon[j]] * mats[(int64_t)j * n2 + e];
+  for (int k = 0; k < split; ++k) s += part[(int64_t)k * n2 + e];
   out[e] = s;
+}
+
+int jtprod_split(int n, int ncon) {
+  const int64_t n2 = (int64_t)n * n;
+  const int tiles = cdiv(n2, 256);
+  int split = cdiv(4096, tiles);
+  split = std::min(split, std::max(1, ncon / 32));
+  return std::max(1, std::min(split, 64));
 }
 
 int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out) {
@@ -79,7 +103,14 @@ int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out) {
     jtprod_coo_kernel<<<cdiv(b.npos, 256), 256, 0, st>>>(b.npos, b.pos_lin, b.pos_ptr, b.pos_ent,
                                                           b.con, b.val, y, out);
   } else {
-    jtprod_dense_kernel<<<cdiv(n2, 256), 256, 0, st>>>(n2, b.ncon, b.con_list, b.mats, y, out);
+    const int split = b.jt_part ? b.jt_split : 1;
+    const int per = cdiv(b.ncon, split);
+    jtprod_dense_kernel<<<dim3(cdiv(n2, 256), split), 256, 0, st>>>(
+        n2, b.ncon, per, b.con_list, b.mats, y, split > 1 ? b.jt_part : out);
+    if (split > 1) {
+      MSDP_HIP(hipGetLastError());
+      jtprod_reduce_kernel<<<cdiv(n2, 256), 256, 0, st>>>(n2, split, b.jt_part, out);
+    }
   }
   MSDP_HIP(hipGetLastError());
   return 0;
@@ -318,101 +349,117 @@ int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, doubl
 
 // ------------------------------------------------- triangular solves ------
 // H y = b with H = L L' (potrs).  The factorisation leaves the inverses of
-// the 64 x 64 diagonal blocks (dinv) and L' is kept column-major (Lt), so
-// every block step of both sweeps is one launch with coalesced loads.
-__global__ __launch_bounds__(256) void trsv_fwd_step_kernel(int m, int k0, int kb,
-                                                            const double* L, int64_t ld,
-                                                            const double* dinv, double* b,
-                                                            double* x) {
+// the 64 x 64 diagonal blocks (dinv) and L' is kept column-major (Lt).  Each
+// sweep walks super-blocks of 512 rows: one workgroup solves the 512 x 512
+// diagonal triangle in eight 64-row steps (x_j = D_j b_j, then the rows below
+// within the super-block), then a grid-wide kernel applies the super-block's
+// panel to every remaining row.  32 launches per sweep at m = 8000.
+constexpr int TRSV_SB = 512;
+
+__global__ __launch_bounds__(1024) void trsv_super_kernel(int K0, int KB, const double* M,
+                                                          int64_t ld, const double* dinv,
+                                                          const double* rhs, double* out,
+                                                          int upper) {
   __shared__ double Ds[64][65];
-  __shared__ double bs[64], xs[64];
-  __shared__ double part[4][64];
+  __shared__ double bs[TRSV_SB];
+  __shared__ double xs[64];
+  __shared__ double part[16][65];
+  __shared__ double part2[4][256];
   const int t = threadIdx.x;
-  for (int e = t; e < 64 * 64; e += 256) Ds[e & 63][e >> 6] = dinv[e];
-  if (t < 64) bs[t] = t < kb ? b[k0 + t] : 0.0;
-  __syncthreads();
-  {  // x_k = D_k b_k: 4 lanes per row, 16 columns each
-    const int rr = t & 63, g = t >> 6;
-    double v = 0.0;
+  for (int i = t; i < KB; i += 1024) bs[i] = rhs[K0 + i];
+  const int nsub = (KB + 63) / 64;
+  for (int s = 0; s < nsub; ++s) {
+    const int j = upper ? nsub - 1 - s : s;
+    const int j0 = 64 * j, jb = min(64, KB - j0);
+    const double* D = dinv + (size_t)j * 4096;
+    for (int e = t; e < 4096; e += 1024) Ds[e & 63][e >> 6] = D[e];
+    __syncthreads();
+    {  // x_j = D_j b_j (forward) or D_j' b_j (backward): row r, 4 columns per thread
+      const int r = t & 63, g = t >> 6;
+      double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v += Ds[rr][g + 4 * q] * bs[g + 4 * q];
-    part[g][rr] = v;
-  }
-  __syncthreads();
-  if (t < 64) {
-    const double v = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
-    xs[t] = v;
-    if (blockIdx.x == 0 && t < kb) x[k0 + t] = v;
-  }
-  __syncthreads();
-  const int rr = t & 63, g = t >> 6;
-  const int r = k0 + kb + blockIdx.x * 64 + rr;
-  double v = 0.0;
-  if (r < m) {
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * g + q;
+        if (c < jb) v += (upper ? Ds[c][r] : Ds[r][c]) * bs[j0 + c];
+      }
+      part[g][r] = v;
+    }
+    __syncthreads();
+    if (t < 64) {
+      double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = g + 4 * q;
-      if (c < kb) v += L[r + (int64_t)(k0 + c) * ld] * xs[c];
+      for (int g = 0; g < 16; ++g) v += part[g][t];
+      xs[t] = t < jb ? v : 0.0;
+      if (t < jb) out[K0 + j0 + t] = v;
+    }
+    __syncthreads();
+    // remaining rows of the super-block: below (forward) or above (backward)
+    const int lo = upper ? 0 : j0 + jb, hi = upper ? j0 : KB;
+    for (int base = lo; base < hi; base += 256) {
+      const int ri = t & 255, cg = t >> 8;
+      const int i = base + ri;
+      double v = 0.0;
+      if (i < hi) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int c = 16 * cg + q;
+          if (c < jb) v += M[(K0 + i) + (int64_t)(K0 + j0 + c) * ld] * xs[c];
+        }
+      }
+      part2[cg][ri] = v;
+      __syncthreads();
+      if (t < 256 && i < hi) bs[i] -= (part2[0][ri] + part2[1][ri]) + (part2[2][ri] + part2[3][ri]);
+      __syncthreads();
     }
   }
-  part[g][rr] = v;
-  __syncthreads();
-  if (g == 0 && r < m) b[r] -= (part[0][rr] + part[1][rr]) + (part[2][rr] + part[3][rr]);
 }
 
-__global__ __launch_bounds__(256) void trsv_bwd_step_kernel(int k0, int kb, const double* Lt,
-                                                            int64_t ld, const double* dinv,
-                                                            double* x, double* z) {
-  __shared__ double Ds[64][65];
-  __shared__ double xs_in[64], zs[64];
+// target[r0 + r] -= sum_{c < KB} M[r0 + r, K0 + c] v[K0 + c], r < rows
+__global__ __launch_bounds__(256) void trsv_panel_kernel(int r0, int rows, int K0, int KB,
+                                                         const double* M, int64_t ld,
+                                                         const double* v, double* target) {
+  __shared__ double xs[TRSV_SB];
   __shared__ double part[4][64];
   const int t = threadIdx.x;
-  for (int e = t; e < 64 * 64; e += 256) Ds[e & 63][e >> 6] = dinv[e];
-  if (t < 64) xs_in[t] = t < kb ? x[k0 + t] : 0.0;
+  for (int i = t; i < KB; i += 256) xs[i] = v[K0 + i];
   __syncthreads();
-  {  // z_k = D_k' x_k
-    const int rr = t & 63, g = t >> 6;
-    double v = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) v += Ds[g + 4 * q][rr] * xs_in[g + 4 * q];
-    part[g][rr] = v;
-  }
-  __syncthreads();
-  if (t < 64) {
-    const double v = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
-    zs[t] = v;
-    if (blockIdx.x == 0 && t < kb) z[k0 + t] = v;
-  }
-  __syncthreads();
-  const int rr = t & 63, g = t >> 6;
-  const int r = blockIdx.x * 64 + rr;
-  double v = 0.0;
-  if (r < k0) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = g + 4 * q;
-      if (c < kb) v += Lt[r + (int64_t)(k0 + c) * ld] * zs[c];
+  const int lane = t & 63, w = t >> 6;
+  const int rr = blockIdx.x * 64 + lane;
+  const int per = (KB + 3) / 4;
+  const int c0 = w * per, c1 = min(KB, c0 + per);
+  double a0 = 0.0, a1 = 0.0;
+  if (rr < rows) {
+    const double* col = M + (r0 + rr) + (int64_t)K0 * ld;
+    int c = c0;
+    for (; c + 1 < c1; c += 2) {
+      a0 += col[(int64_t)c * ld] * xs[c];
+      a1 += col[(int64_t)(c + 1) * ld] * xs[c + 1];
     }
+    if (c < c1) a0 += col[(int64_t)c * ld] * xs[c];
   }
-  part[g][rr] = v;
+  part[w][lane] = a0 + a1;
   __syncthreads();
-  if (g == 0 && r < k0) x[r] -= (part[0][rr] + part[1][rr]) + (part[2][rr] + part[3][rr]);
+  if (w == 0 && rr < rows)
+    target[r0 + rr] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
           const double* dinv, double* b, double* tmp) {
   if (m <= 0) return 0;
-  const int nblk = cdiv(m, 64);
-  for (int bi = 0; bi < nblk; ++bi) {
-    const int k0 = bi * 64, kb = std::min(64, m - k0);
-    const int rest = m - k0 - kb;
-    trsv_fwd_step_kernel<<<std::max(1, cdiv(rest, 64)), 256, 0, st>>>(
-        m, k0, kb, L, ld, dinv + (size_t)bi * 4096, b, tmp);
+  const int nsb = cdiv(m, TRSV_SB);
+  // forward: L x = b (x -> tmp, b overwritten by the updates)
+  for (int k = 0; k < nsb; ++k) {
+    const int K0 = k * TRSV_SB, KB = std::min(TRSV_SB, m - K0);
+    trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, L, ld, dinv + (size_t)(K0 / 64) * 4096, b, tmp, 0);
+    const int rest = m - K0 - KB;
+    if (rest > 0)
+      trsv_panel_kernel<<<cdiv(rest, 64), 256, 0, st>>>(K0 + KB, rest, K0, KB, L, ld, tmp, b);
   }
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int k0 = bi * 64, kb = std::min(64, m - k0);
-    trsv_bwd_step_kernel<<<std::max(1, cdiv(k0, 64)), 256, 0, st>>>(
-        k0, kb, Lt, ld, dinv + (size_t)bi * 4096, tmp, b);
+  // backward: L' z = x (z -> b, tmp overwritten by the updates)
+  for (int k = nsb - 1; k >= 0; --k) {
+    const int K0 = k * TRSV_SB, KB = std::min(TRSV_SB, m - K0);
+    trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, Lt, ld, dinv + (size_t)(K0 / 64) * 4096, tmp, b, 1);
+    if (K0 > 0) trsv_panel_kernel<<<cdiv(K0, 64), 256, 0, st>>>(0, K0, K0, KB, Lt, ld, b, tmp);
   }
   MSDP_HIP(hipGetLastError());
   return 0;

@@ -243,6 +243,8 @@ int msdp_solver::finalize() {
       const size_t drow = n2;  // vec (HKM) is the larger of vec / svec
       MSDP_TRY(alloc(&o.Bbuf, drow * mi));
       if (!ident) MSDP_TRY(alloc(&o.Hbuf, (size_t)mi * mi));
+      o.jt_split = jtprod_split(b.n, mi);
+      if (o.jt_split > 1) MSDP_TRY(alloc(&o.jt_part, (size_t)o.jt_split * n2));
       std::vector<double>().swap(b.h_mats);
     } else if (b.has_coo && !b.h_con.empty()) {
       // sort entries by constraint (stable), then build the ranges
@@ -1212,7 +1214,7 @@ int msdp_solver::schur_solve(double* v) {
     MSDP_TRY(read_slots());
     hn = std::sqrt(hsc[15]);
     rn = std::sqrt(hsc[16]);
-    if (!(rn > 1e-15 * hn) || it >= opt.refine_max_iter || !(rn < 0.5 * prev)) break;
+    if (!(rn > 1e-12 * hn) || it >= opt.refine_max_iter || !(rn < 0.5 * prev)) break;
     prev = rn;
     MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, rr, wm2));
     MSDP_TRY(axpby(st, m, 1.0, v, 1.0, rr, v));
