@@ -8,7 +8,7 @@
  *     (LowRankOpt.Model fields) and its iterate, resident in device memory,
  *     and exposes the algorithmic-component seam of MadSDP: the driver steps
  *     of src/solver.jl and the component methods of src/components.jl.
- *     A Julia component (see INTEGRATION.md) binds these with `ccall`.
+ *     A Julia component binds these with `ccall`.
  *
  *  2. Kernel layer (msdp_k_*): the device building blocks on caller-owned
  *     device pointers, on a caller-given HIP stream -- the replacements of
@@ -147,11 +147,57 @@ int msdp_solver_set(msdp_solver* s, int32_t which, int32_t block, const double* 
 /* wait for the solver's stream */
 int msdp_solver_synchronize(msdp_solver* s);
 /* live timing of one kernel class on the solver's stream (0 = FP64 MFMA GEMM,
- * 1 = tridiagonalisation update): HIP events around every launch, summed
+ * 1 = tridiagonalisation update, 2 = FP32 MFMA GEMM): HIP events around every launch, summed
  * duration, launch count and algorithmic work (flops or bytes).  No
  * counterpart in the reference; it feeds the roofline report. */
 int msdp_solver_profile_begin(msdp_solver* s, int32_t kernel_class);
 int msdp_solver_profile_end(msdp_solver* s, double* total_ms, int64_t* launches, double* work);
+/* Run the solver on a caller-owned HIP stream (before msdp_solver_initialize;
+ * the solver never destroys it).  Without it the solver creates its own
+ * non-blocking stream.  No counterpart in the reference: Julia runs one
+ * solver per task, the serving loop below runs one per stream. */
+int msdp_solver_set_stream(msdp_solver* s, void* stream);
+/* Device bytes the solver holds once initialized, estimated from the sizes
+ * alone (admission by memory footprint; no counterpart in the reference). */
+int msdp_solver_footprint(const msdp_solver* s, int64_t* bytes);
+/* Copy the solution (X, S, y, x, z) to the host and free the device state;
+ * afterwards msdp_solver_get serves those arrays, get_info and trace keep
+ * working, and every other call fails.  No counterpart in the reference. */
+int msdp_solver_release(msdp_solver* s);
+
+/* ---- serving loop: continuous batching of independent solves ----
+ * No counterpart in the reference (MadSDP solves one problem per call,
+ * src/solver.jl:385-396).  Requests are prepared solvers (problem data set,
+ * options configured, nothing on the device yet) with an arrival time in
+ * seconds after msdp_server_start.  `slots` worker threads each own a HIP
+ * stream; a request is admitted in submission order once it has arrived, a
+ * slot is free and its footprint fits in the memory budget (a request larger
+ * than the budget runs alone).  An admitted request is uploaded, solved on its
+ * slot's stream concurrently with the other slots, and released.  The caller
+ * keeps ownership of the solver handles and must not use a submitted one
+ * until its request is done. */
+typedef struct msdp_server msdp_server;
+typedef struct {
+  double arrival, admitted, done;  /* seconds after msdp_server_start */
+  int64_t footprint;               /* bytes */
+  int32_t rc;                      /* msdp_solver_solve's return (0 = ok) */
+  int32_t slot;
+  int32_t finished;                /* 1 once done */
+  int32_t pad;
+} msdp_request_info;
+typedef struct {
+  int64_t submitted, completed, peak_bytes;
+  int32_t slots, peak_active;
+  double elapsed;                  /* seconds since start */
+} msdp_server_stats;
+int msdp_server_create(msdp_server** out, int device, int32_t slots, int64_t mem_budget_bytes);
+int msdp_server_start(msdp_server* v);
+int msdp_server_submit(msdp_server* v, msdp_solver* s, double arrival_s, int64_t* id);
+/* block until every submitted request is done */
+int msdp_server_wait_all(msdp_server* v);
+int msdp_server_request(const msdp_server* v, int64_t id, msdp_request_info* out);
+int msdp_server_get_stats(const msdp_server* v, msdp_server_stats* out);
+int msdp_server_destroy(msdp_server* v);
 
 /* ---- kernel layer: device pointers, caller's stream ---- */
 /* C = alpha op(A) op(B) + beta C (BLAS dgemm; directions.jl, step.jl products) */

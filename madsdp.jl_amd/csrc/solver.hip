@@ -58,6 +58,7 @@ struct Blk {
   std::vector<double> h_val;
   std::vector<int> h_dcon;
   std::vector<double> h_mats;
+  std::vector<double> h_C;
   bool has_coo = false, has_dense = false;
 };
 
@@ -66,6 +67,14 @@ struct Blk {
 struct msdp_solver {
   int device = 0;
   hipStream_t st = nullptr;
+  bool own_stream = true;
+  // host copies of b, d_lin, C_lin until finalize
+  std::vector<double> h_b, h_dlin, h_Clin;
+  // released: device state freed after a solve; the solution stays on the host
+  bool released = false;
+  std::vector<double> r_y, r_x, r_z;
+  std::vector<std::vector<double>> r_X, r_S;
+  double* hsc_pin = nullptr;
   int nb = 0;
   int64_t m = 0, ns = 0;
   std::vector<Blk> blk;
@@ -119,27 +128,47 @@ struct msdp_solver {
   std::vector<double> hsc;
 
   ~msdp_solver() {
-    if (st) (void)hipStreamSynchronize(st);
-    for (void* p : allocs) (void)hipFree(p);
-    if (st) (void)hipStreamDestroy(st);
+    free_device();
+    if (hsc_pin) (void)hipHostFree(hsc_pin);
+    if (st && own_stream) (void)hipStreamDestroy(st);
   }
 
+  // device memory is stream-ordered (hipMallocAsync on the solver's stream),
+  // so solves on other streams never wait for this solver's allocations
+  int ensure_stream() {
+    if (st) return 0;
+    MSDP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    own_stream = true;
+    return 0;
+  }
   template <class T>
   int alloc(T** p, size_t count) {
+    MSDP_TRY(ensure_stream());
     void* q = nullptr;
-    MSDP_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    MSDP_HIP(hipMallocAsync(&q, std::max<size_t>(count, 1) * sizeof(T), st));
     allocs.push_back(q);
     *p = reinterpret_cast<T*>(q);
     return 0;
   }
+  void free_device() {
+    if (!st || allocs.empty()) return;
+    (void)hipStreamSynchronize(st);
+    for (void* p : allocs) (void)hipFreeAsync(p, st);
+    allocs.clear();
+    (void)hipStreamSynchronize(st);
+  }
+  int release();
+  int keep_solution();
+  int64_t footprint() const;
 
   double* slot(int k) { return sc + k; }
   // slots: 0..31 global, 32 + 8*i per block
   int bslot(int i, int k) const { return 32 + 8 * i + k; }
   int read_slots() {
     hsc.resize(nslots);
-    MSDP_HIP(hipMemcpyAsync(hsc.data(), sc, sizeof(double) * nslots, hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipMemcpyAsync(hsc_pin, sc, sizeof(double) * nslots, hipMemcpyDeviceToHost, st));
     MSDP_HIP(hipStreamSynchronize(st));
+    std::memcpy(hsc.data(), hsc_pin, sizeof(double) * nslots);
     return 0;
   }
   int read_infos(std::vector<int>& out) {
@@ -205,8 +234,10 @@ static int as_err(const char* msg) {
 // set-up
 // ==========================================================================
 int msdp_solver::finalize() {
+  if (released) return as_err("solver released: its device state was freed after the solve");
   if (finalized) return 0;
   MSDP_HIP(hipSetDevice(device));
+  MSDP_TRY(ensure_stream());
   int nmax = 1;
   for (auto& b : blk) nmax = std::max(nmax, b.n);
   const size_t nm2 = (size_t)nmax * nmax;
@@ -222,10 +253,11 @@ int msdp_solver::finalize() {
     }
     MSDP_TRY(alloc(&b.D, b.n));
     MSDP_TRY(alloc(&b.lam, b.n));
-    if (!b.C) {
-      MSDP_TRY(alloc(&b.C, n2));
+    MSDP_TRY(alloc(&b.C, n2));
+    if (b.h_C.size() == n2)
+      MSDP_HIP(hipMemcpyAsync(b.C, b.h_C.data(), sizeof(double) * n2, hipMemcpyHostToDevice, st));
+    else
       MSDP_HIP(hipMemsetAsync(b.C, 0, sizeof(double) * n2, st));
-    }
     // constraints
     BlockOps& o = b.ops;
     o.n = b.n;
@@ -235,8 +267,8 @@ int msdp_solver::finalize() {
       o.ncon = mi;
       MSDP_TRY(alloc(&o.con_list, mi));
       MSDP_TRY(alloc(&o.mats, (size_t)mi * n2));
-      MSDP_HIP(hipMemcpy(o.con_list, b.h_dcon.data(), sizeof(int) * mi, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.mats, b.h_mats.data(), sizeof(double) * mi * n2, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpyAsync(o.con_list, b.h_dcon.data(), sizeof(int) * mi, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.mats, b.h_mats.data(), sizeof(double) * mi * n2, hipMemcpyHostToDevice, st));
       bool ident = mi == m;
       for (int j = 0; ident && j < mi; ++j) ident = b.h_dcon[j] == j;
       o.dcon_identity = ident;
@@ -245,7 +277,6 @@ int msdp_solver::finalize() {
       if (!ident) MSDP_TRY(alloc(&o.Hbuf, (size_t)mi * mi));
       o.jt_split = jtprod_split(b.n, mi);
       if (o.jt_split > 1) MSDP_TRY(alloc(&o.jt_part, (size_t)o.jt_split * n2));
-      std::vector<double>().swap(b.h_mats);
     } else if (b.has_coo && !b.h_con.empty()) {
       // sort entries by constraint (stable), then build the ranges
       const int64_t nnz = (int64_t)b.h_con.size();
@@ -290,15 +321,16 @@ int msdp_solver::finalize() {
       MSDP_TRY(alloc(&o.pos_lin, plin.size()));
       MSDP_TRY(alloc(&o.pos_ptr, pptr.size()));
       MSDP_TRY(alloc(&o.pos_ent, pent.size()));
-      MSDP_HIP(hipMemcpy(o.con, con.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.row, row.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.con_list, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.con_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.pos_lin, plin.data(), sizeof(int64_t) * plin.size(), hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.pos_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(o.pos_ent, pent.data(), sizeof(int) * pent.size(), hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpyAsync(o.con, con.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.row, row.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.con_list, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.con_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.pos_lin, plin.data(), sizeof(int64_t) * plin.size(), hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.pos_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.pos_ent, pent.data(), sizeof(int) * pent.size(), hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipStreamSynchronize(st));  // the host arrays above are locals
     } else {
       o.kind = BlockOps::EMPTY;
     }
@@ -307,11 +339,21 @@ int msdp_solver::finalize() {
   const size_t mm = std::max<int64_t>(m, 1), nss = std::max<int64_t>(ns, 1);
   double** mv[] = {&y, &dely, &Rp, &wm, &dc_d, &dc_g, &dc_dy, &dc_yhat};
   for (double** p : mv) { MSDP_TRY(alloc(p, mm)); MSDP_HIP(hipMemsetAsync(*p, 0, sizeof(double) * mm, st)); }
-  if (!b) { MSDP_TRY(alloc(&b, mm)); MSDP_HIP(hipMemsetAsync(b, 0, sizeof(double) * mm, st)); }
+  MSDP_TRY(alloc(&b, mm));
+  if ((int64_t)h_b.size() == m && m > 0)
+    MSDP_HIP(hipMemcpyAsync(b, h_b.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+  else
+    MSDP_HIP(hipMemsetAsync(b, 0, sizeof(double) * mm, st));
   double** sv[] = {&x, &z, &delx, &delz, &Rdlin, &zinv, &corrlin, &ws1, &ws2};
   for (double** p : sv) { MSDP_TRY(alloc(p, nss)); MSDP_HIP(hipMemsetAsync(*p, 0, sizeof(double) * nss, st)); }
-  if (!dlin) { MSDP_TRY(alloc(&dlin, nss)); MSDP_HIP(hipMemsetAsync(dlin, 0, sizeof(double) * nss, st)); }
-  if (!Clin) { MSDP_TRY(alloc(&Clin, mm * nss)); MSDP_HIP(hipMemsetAsync(Clin, 0, sizeof(double) * mm * nss, st)); }
+  MSDP_TRY(alloc(&dlin, nss));
+  MSDP_TRY(alloc(&Clin, mm * nss));
+  MSDP_HIP(hipMemsetAsync(dlin, 0, sizeof(double) * nss, st));
+  MSDP_HIP(hipMemsetAsync(Clin, 0, sizeof(double) * mm * nss, st));
+  if (ns > 0 && (int64_t)h_dlin.size() == ns)
+    MSDP_HIP(hipMemcpyAsync(dlin, h_dlin.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
+  if (ns > 0 && m > 0 && (int64_t)h_Clin.size() == ns * m)
+    MSDP_HIP(hipMemcpyAsync(Clin, h_Clin.data(), sizeof(double) * ns * m, hipMemcpyHostToDevice, st));
   MSDP_TRY(alloc(&Cs, mm * nss));
   MSDP_TRY(alloc(&H, mm * mm));
   MSDP_TRY(alloc(&Hs, mm * mm));
@@ -324,6 +366,7 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&info, 4 * nb + 8));
   nslots = 32 + 8 * nb;
   MSDP_TRY(alloc(&sc, nslots));
+  if (!hsc_pin) MSDP_HIP(hipHostMalloc((void**)&hsc_pin, sizeof(double) * nslots, hipHostMallocDefault));
   MSDP_TRY(alloc(&red, reduce_scratch_elems(0) + 64));
   MSDP_TRY(alloc(&ework, syevd_work_elems(nmax)));
   MSDP_TRY(alloc(&eiwork, syevd_iwork_elems(nmax)));
@@ -351,9 +394,10 @@ int msdp_solver::finalize() {
       MSDP_TRY(alloc(&eb_nt, nsmall));
       MSDP_TRY(alloc(&eb_step, 2 * nsmall));
       MSDP_TRY(alloc(&eb_dc, nsmall));
-      MSDP_HIP(hipMemcpy(eb_nt, nt_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(eb_step, step_items.data(), sizeof(EigBatch) * 2 * nsmall, hipMemcpyHostToDevice));
-      MSDP_HIP(hipMemcpy(eb_dc, dc_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice));
+      MSDP_HIP(hipMemcpyAsync(eb_nt, nt_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(eb_step, step_items.data(), sizeof(EigBatch) * 2 * nsmall, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(eb_dc, dc_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipStreamSynchronize(st));
     }
   }
   // chunk buffers of the dense Schur assembly: up to 64 constraints or 256 MiB
@@ -380,7 +424,83 @@ int msdp_solver::finalize() {
     blk[i].cnorm = hsc[bslot(i, 0)];
     cnorm_max = std::max(cnorm_max, blk[i].cnorm);
   }
+  // read_slots synchronised the stream: the host staging can go
+  for (auto& bb : blk) {
+    std::vector<double>().swap(bb.h_mats);
+    std::vector<double>().swap(bb.h_C);
+    std::vector<double>().swap(bb.h_val);
+    std::vector<int>().swap(bb.h_con);
+    std::vector<int>().swap(bb.h_row);
+    std::vector<int>().swap(bb.h_col);
+  }
+  std::vector<double>().swap(h_b);
+  std::vector<double>().swap(h_dlin);
+  std::vector<double>().swap(h_Clin);
   finalized = true;
+  return 0;
+}
+
+// Device bytes held by a finalized solver (estimate from the sizes alone, so
+// a server can admit a request before anything is allocated).
+int64_t msdp_solver::footprint() const {
+  const int64_t d = sizeof(double);
+  const int64_t mm = std::max<int64_t>(m, 1), nss = std::max<int64_t>(ns, 1);
+  int64_t bytes = 0, nmax = 1, tot = 0;
+  int64_t swork_b = 0;
+  for (const auto& b : blk) {
+    const int64_t n2 = (int64_t)b.n * b.n;
+    nmax = std::max<int64_t>(nmax, b.n);
+    tot += b.n;
+    bytes += 26 * n2 * d + 2 * b.n * d;
+    if (b.has_dense) {
+      const int64_t mi = (int64_t)b.h_dcon.size();
+      bytes += 2 * mi * n2 * d + mi * 4 + (mi == m ? 0 : mi * mi * d);
+      bytes += (int64_t)jtprod_split(b.n, (int)mi) * n2 * d;
+      if (opt.precision == 1) bytes += 2 * mi * n2 * 4 + 2 * n2 * 4;
+      swork_b = std::max(swork_b, 2 * n2 * std::min<int64_t>(mi, 64) * d);
+    } else if (b.has_coo) {
+      bytes += (int64_t)b.h_con.size() * (4 * 3 + 8 + 8 + 4 + 4) + 64;
+    }
+  }
+  bytes += std::min<int64_t>(swork_b, 256ll << 20);
+  bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
+  bytes += (int64_t)potrf_work_elems((int)mm) * d;
+  bytes += ((int64_t)syevd_work_elems((int)nmax) + trtri_work_elems((int)nmax) + 3 * tot) * d;
+  return bytes + (1 << 20);
+}
+
+// Copies the solution (X, S, y, x, z) to the host, frees the device state and
+// drops a borrowed stream (its owner may destroy it next).
+int msdp_solver::release() {
+  if (released) return 0;
+  int rc = 0;
+  if (finalized) rc = keep_solution();
+  free_device();
+  released = finalized;
+  if (!own_stream) st = nullptr;
+  return rc;
+}
+
+int msdp_solver::keep_solution() {
+  MSDP_HIP(hipSetDevice(device));
+  r_y.resize(m);
+  r_x.resize(ns);
+  r_z.resize(ns);
+  if (m > 0) MSDP_HIP(hipMemcpyAsync(r_y.data(), y, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+  if (ns > 0) {
+    MSDP_HIP(hipMemcpyAsync(r_x.data(), x, sizeof(double) * ns, hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipMemcpyAsync(r_z.data(), z, sizeof(double) * ns, hipMemcpyDeviceToHost, st));
+  }
+  r_X.resize(nb);
+  r_S.resize(nb);
+  for (int i = 0; i < nb; ++i) {
+    const size_t n2 = (size_t)blk[i].n * blk[i].n;
+    r_X[i].resize(n2);
+    r_S[i].resize(n2);
+    MSDP_HIP(hipMemcpyAsync(r_X[i].data(), blk[i].X, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipMemcpyAsync(r_S[i].data(), blk[i].S, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
+  }
+  MSDP_HIP(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -1349,8 +1469,28 @@ int msdp_solver::qr_solve_directions(double mu_, bool corrected) {
 // ==========================================================================
 // C ABI
 // ==========================================================================
-#define GUARD(s) \
+// The stream-ordered allocator keeps freed memory for reuse instead of
+// returning it to the driver, so solves that come and go (a serving loop)
+// never pay for, or synchronise on, device allocation.
+static void keep_pool(int device) {
+  static bool done[64] = {};
+  if (device < 0 || device >= 64 || done[device]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    uint64_t thr = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  done[device] = true;
+}
+
+#define GUARD0(s) \
   if (!(s)) return as_err("null solver");
+#define GUARD(s) \
+  GUARD0(s)      \
+  if ((s)->released) return as_err("solver released: its device state was freed after the solve");
+#define GUARD_READY(s) \
+  GUARD(s)             \
+  if (!(s)->finalized) return as_err("call msdp_solver_initialize first");
 
 extern "C" {
 
@@ -1409,8 +1549,7 @@ int msdp_solver_create(msdp_solver** out, int device, int32_t nblocks, const int
     s->blk[i].n = (int)dims[i];
   }
   msdp_default_options(&s->opt);
-  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete s; return fail_hip(e, "hipStreamCreate", __FILE__, __LINE__); }
+  keep_pool(device);
   *out = s;
   return 0;
 }
@@ -1423,11 +1562,16 @@ int msdp_solver_destroy(msdp_solver* s) {
 int msdp_solver_set_cost_block(msdp_solver* s, int32_t block, const double* C) {
   GUARD(s);
   if (block < 0 || block >= s->nb) return as_err("block out of range");
-  MSDP_HIP(hipSetDevice(s->device));
+  if (s->released) return as_err("solver released");
   Blk& b = s->blk[block];
   const size_t n2 = (size_t)b.n * b.n;
-  if (!b.C) MSDP_TRY(s->alloc(&b.C, n2));
-  MSDP_HIP(hipMemcpy(b.C, C, sizeof(double) * n2, hipMemcpyHostToDevice));
+  if (!s->finalized) {  // staged on the host until finalize
+    b.h_C.assign(C, C + n2);
+    return 0;
+  }
+  MSDP_HIP(hipSetDevice(s->device));
+  MSDP_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * n2, hipMemcpyHostToDevice, s->st));
+  MSDP_HIP(hipStreamSynchronize(s->st));
   return 0;
 }
 
@@ -1470,21 +1614,33 @@ int msdp_solver_set_constraints_dense(msdp_solver* s, int32_t block, int64_t m_i
 
 int msdp_solver_set_rhs(msdp_solver* s, const double* bv) {
   GUARD(s);
+  if (s->released) return as_err("solver released");
+  if (!s->finalized) {
+    s->h_b.assign(bv, bv + s->m);
+    return 0;
+  }
   MSDP_HIP(hipSetDevice(s->device));
-  if (!s->b) MSDP_TRY(s->alloc(&s->b, std::max<int64_t>(s->m, 1)));
-  if (s->m > 0) MSDP_HIP(hipMemcpy(s->b, bv, sizeof(double) * s->m, hipMemcpyHostToDevice));
+  if (s->m > 0) {
+    MSDP_HIP(hipMemcpyAsync(s->b, bv, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
+    MSDP_HIP(hipStreamSynchronize(s->st));
+  }
   return 0;
 }
 
 int msdp_solver_set_scalar_data(msdp_solver* s, const double* d_lin, const double* C_lin) {
   GUARD(s);
+  if (s->released) return as_err("solver released");
+  if (!s->finalized) {
+    s->h_dlin.assign(d_lin, d_lin + s->ns);
+    s->h_Clin.assign(C_lin, C_lin + s->ns * s->m);
+    return 0;
+  }
   MSDP_HIP(hipSetDevice(s->device));
-  const size_t nss = std::max<int64_t>(s->ns, 1), mm = std::max<int64_t>(s->m, 1);
-  if (!s->dlin) MSDP_TRY(s->alloc(&s->dlin, nss));
-  if (!s->Clin) MSDP_TRY(s->alloc(&s->Clin, nss * mm));
   if (s->ns > 0) {
-    MSDP_HIP(hipMemcpy(s->dlin, d_lin, sizeof(double) * s->ns, hipMemcpyHostToDevice));
-    if (s->m > 0) MSDP_HIP(hipMemcpy(s->Clin, C_lin, sizeof(double) * s->ns * s->m, hipMemcpyHostToDevice));
+    MSDP_HIP(hipMemcpyAsync(s->dlin, d_lin, sizeof(double) * s->ns, hipMemcpyHostToDevice, s->st));
+    if (s->m > 0)
+      MSDP_HIP(hipMemcpyAsync(s->Clin, C_lin, sizeof(double) * s->ns * s->m, hipMemcpyHostToDevice, s->st));
+    MSDP_HIP(hipStreamSynchronize(s->st));
   }
   return 0;
 }
@@ -1514,22 +1670,45 @@ int msdp_solver_set_initial_point(msdp_solver* s, const double* X, const double*
 }
 
 int msdp_solver_initialize(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->initialize(); }
-int msdp_solver_iterate(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->iterate(); }
-int msdp_solver_update_termination(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->update_termination(); }
+int msdp_solver_iterate(msdp_solver* s) { GUARD_READY(s); MSDP_HIP(hipSetDevice(s->device)); return s->iterate(); }
+int msdp_solver_update_termination(msdp_solver* s) { GUARD_READY(s); MSDP_HIP(hipSetDevice(s->device)); return s->update_termination(); }
 int msdp_solver_solve(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->solve(); }
-int msdp_solver_synchronize(msdp_solver* s) { GUARD(s); MSDP_HIP(hipStreamSynchronize(s->st)); return 0; }
-int msdp_solver_profile_begin(msdp_solver* s, int32_t cls) {
+int msdp_solver_synchronize(msdp_solver* s) {
+  GUARD0(s);
+  if (s->st && !s->released) MSDP_HIP(hipStreamSynchronize(s->st));
+  return 0;
+}
+int msdp_solver_set_stream(msdp_solver* s, void* stream) {
   GUARD(s);
+  if (s->finalized) return as_err("set the stream before msdp_solver_initialize");
+  if (!stream) return as_err("null stream");
+  if (s->st && s->own_stream) MSDP_HIP(hipStreamDestroy(s->st));
+  s->st = (hipStream_t)stream;
+  s->own_stream = false;
+  return 0;
+}
+int msdp_solver_footprint(const msdp_solver* s, int64_t* bytes) {
+  GUARD0(s);
+  if (!bytes) return as_err("null bytes");
+  *bytes = s->footprint();
+  return 0;
+}
+int msdp_solver_release(msdp_solver* s) {
+  GUARD0(s);
+  return s->release();
+}
+int msdp_solver_profile_begin(msdp_solver* s, int32_t cls) {
+  GUARD_READY(s);
   MSDP_HIP(hipStreamSynchronize(s->st));
   return prof_begin(cls, s->st);
 }
 int msdp_solver_profile_end(msdp_solver* s, double* ms, int64_t* launches, double* work) {
-  GUARD(s);
+  GUARD0(s);
   return prof_end(ms, launches, work);
 }
 
 int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
-  GUARD(s);
+  GUARD0(s);
   o->mu = s->mu; o->sigma = s->sigma; o->alpha = s->alpha; o->beta = s->beta;
   o->inf_pr = s->inf_pr; o->inf_du = s->inf_du; o->inf_compl = s->inf_compl; o->obj_val = s->obj_val;
   o->pobj = s->pobj; o->dobj = s->dobj; o->iter = s->iter; o->status = s->status;
@@ -1540,7 +1719,7 @@ int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
 }
 
 int msdp_solver_trace(const msdp_solver* s, int32_t maxrec, double* out9, int32_t* nrec) {
-  GUARD(s);
+  GUARD0(s);
   const int n = (int)std::min<size_t>(s->trace.size(), (size_t)std::max(maxrec, 0));
   for (int k = 0; k < n; ++k) std::memcpy(out9 + 9 * k, s->trace[k].data(), 9 * sizeof(double));
   *nrec = (int)s->trace.size();
@@ -1549,32 +1728,32 @@ int msdp_solver_trace(const msdp_solver* s, int32_t maxrec, double* out9, int32_
 
 int msdp_solver_update_residuals(msdp_solver* s) { GUARD(s); MSDP_TRY(s->finalize()); return s->update_residuals(); }
 int msdp_solver_prepare(msdp_solver* s, int32_t* ok) {
-  GUARD(s);
+  GUARD_READY(s);
   bool b = false;
   MSDP_TRY(s->prepare(b));
   *ok = b;
   return 0;
 }
 int msdp_solver_factorize_schur(msdp_solver* s, int32_t* ok) {
-  GUARD(s);
+  GUARD_READY(s);
   bool b = false;
   MSDP_TRY(s->factorize_schur(b));
   *ok = b;
   return 0;
 }
 int msdp_solver_solve_directions(msdp_solver* s, double mu, int32_t corrected) {
-  GUARD(s);
+  GUARD_READY(s);
   return s->solve_directions(mu, corrected != 0);
 }
-int msdp_solver_correction(msdp_solver* s) { GUARD(s); return s->correction(); }
-int msdp_solver_step_lengths(msdp_solver* s, double* a, double* bt) { GUARD(s); return s->step_lengths(*a, *bt); }
+int msdp_solver_correction(msdp_solver* s) { GUARD_READY(s); return s->correction(); }
+int msdp_solver_step_lengths(msdp_solver* s, double* a, double* bt) { GUARD_READY(s); return s->step_lengths(*a, *bt); }
 int msdp_solver_affine_complementarity(msdp_solver* s, double a, double bt, double* out) {
-  GUARD(s);
+  GUARD_READY(s);
   return s->affine_complementarity(a, bt, *out);
 }
-int msdp_solver_take_step(msdp_solver* s, double a, double bt) { GUARD(s); return s->take_step(a, bt); }
+int msdp_solver_take_step(msdp_solver* s, double a, double bt) { GUARD_READY(s); return s->take_step(a, bt); }
 int msdp_solver_schur_solve(msdp_solver* s, double* rhs) {
-  GUARD(s);
+  GUARD_READY(s);
   MSDP_HIP(hipMemcpyAsync(s->wm, rhs, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
   MSDP_TRY(potrs(s->st, (int)s->m, s->H, s->Ht, s->m, s->Hdinv, s->wm, s->wm2));
   MSDP_HIP(hipMemcpyAsync(rhs, s->wm, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->st));
@@ -1613,8 +1792,19 @@ static int arr(msdp_solver* s, int which, int block, double** p, size_t* count) 
 }
 
 int msdp_solver_get(const msdp_solver* cs, int32_t which, int32_t block, double* host) {
-  GUARD(cs);
+  GUARD0(cs);
   auto* s = const_cast<msdp_solver*>(cs);
+  if (s->released) {  // the solution kept on the host
+    const std::vector<double>* v = nullptr;
+    if (which == MSDP_Y) v = &s->r_y;
+    else if (which == MSDP_XLIN) v = &s->r_x;
+    else if (which == MSDP_ZLIN) v = &s->r_z;
+    else if ((which == MSDP_X || which == MSDP_S) && block >= 0 && block < s->nb)
+      v = which == MSDP_X ? &s->r_X[block] : &s->r_S[block];
+    if (!v) return as_err("solver released: only X, S, y, x and z are kept");
+    std::memcpy(host, v->data(), sizeof(double) * v->size());
+    return 0;
+  }
   MSDP_TRY(s->finalize());
   double* p; size_t c;
   if (arr(s, which, block, &p, &c) != 0) return as_err("bad array selector or block");

@@ -8,3 +8,4 @@ from .model import BlockConstraints, Model  # noqa: F401
 from .solver import (AdaptiveMu, CholeskyBacktrack, DenseCholesky, DualCentering,  # noqa: F401
                      DualShift, EigenStep, HKM, InitialPoint, MadSDPSolver, Mehrotra, NT,
                      PrimalDual, QRSchur, ScaledIdentity, Status, madsdp)
+from .serve import Server, serve  # noqa: F401,E402

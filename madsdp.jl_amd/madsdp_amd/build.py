@@ -11,7 +11,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "libmadsdp_hip.so"
-SOURCES = ["dense.hip", "gemm32.hip", "eigen.hip", "ops.hip", "qr.hip", "solver.hip",
+SOURCES = ["dense.hip", "gemm32.hip", "eigen.hip", "ops.hip", "qr.hip", "solver.hip", "server.hip",
            "profile.hip"]
 ARCH = os.environ.get("MSDP_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

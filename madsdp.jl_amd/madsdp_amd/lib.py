@@ -44,6 +44,17 @@ class Info(C.Structure):
     ]
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [("arrival", C.c_double), ("admitted", C.c_double), ("done", C.c_double),
+                ("footprint", C.c_int64), ("rc", C.c_int32), ("slot", C.c_int32),
+                ("finished", C.c_int32), ("pad", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("peak_bytes", C.c_int64),
+                ("slots", C.c_int32), ("peak_active", C.c_int32), ("elapsed", C.c_double)]
+
+
 # name -> (restype, argtypes)
 P = C.c_void_p
 SIGNATURES = {
@@ -78,6 +89,16 @@ SIGNATURES = {
     "msdp_solver_synchronize": (C.c_int, [P]),
     "msdp_solver_profile_begin": (C.c_int, [P, C.c_int32]),
     "msdp_solver_profile_end": (C.c_int, [P, c_double_p, c_int64_p, c_double_p]),
+    "msdp_solver_set_stream": (C.c_int, [P, P]),
+    "msdp_solver_footprint": (C.c_int, [P, c_int64_p]),
+    "msdp_solver_release": (C.c_int, [P]),
+    "msdp_server_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int32, C.c_int64]),
+    "msdp_server_start": (C.c_int, [P]),
+    "msdp_server_submit": (C.c_int, [P, P, C.c_double, c_int64_p]),
+    "msdp_server_wait_all": (C.c_int, [P]),
+    "msdp_server_request": (C.c_int, [P, C.c_int64, C.POINTER(RequestInfo)]),
+    "msdp_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "msdp_server_destroy": (C.c_int, [P]),
     "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
     "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),

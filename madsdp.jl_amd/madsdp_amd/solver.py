@@ -304,6 +304,12 @@ class MadSDPSolver:
         L.check(self.lib.msdp_solver_take_step(self._h, a, b))
 
     # -- state --------------------------------------------------------------
+    def footprint(self) -> int:
+        """Device bytes the solver holds once initialized (estimate)."""
+        v = C.c_int64()
+        L.check(self.lib.msdp_solver_footprint(self._h, C.byref(v)))
+        return v.value
+
     def info(self) -> L.Info:
         inf = L.Info()
         L.check(self.lib.msdp_solver_get_info(self._h, C.byref(inf)))
