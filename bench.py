@@ -55,6 +55,78 @@ def build_model(workload):
     raise SystemExit(f"unknown workload {workload}")
 
 
+def run_stream(args):
+    """Serving stream (c4): Poisson arrivals of independent instances, half
+    max-cut with n ~ U{20..600}, half dense SDPs with n ~ U{16..200},
+    m ~ U{50..3000}, n_s = n.  The same arrivals are served twice: by the plain
+    FP64 path (one solve at a time) and by the continuous-batching loop
+    (`--slots` concurrent solves, mixed precision).  Returns the JSON record."""
+    import numpy as np
+    from madsdp_amd import MadSDPSolver, instances, serve
+
+    stream = instances.serving_stream(count=args.count, seed=4, rate=args.rate)
+    arrivals = [a - stream[0][0] for a, _, _ in stream]
+
+    t = time.perf_counter()
+    models = [instances.build_stream_instance(kind, params) for _, kind, params in stream]
+    print(f"generated {len(models)} instances in {time.perf_counter() - t:.1f} s", file=sys.stderr,
+          flush=True)
+
+    def one(slots, precision):
+        t = time.perf_counter()
+        solvers = [MadSDPSolver(model, device=0, precision=precision) for model in models]
+        prep = time.perf_counter() - t
+        res, summary = serve(solvers, arrivals, slots=slots, mem_budget=args.mem_budget << 30)
+        st = [int(s.status) for s in solvers]
+        its = [int(s.iter) for s in solvers]
+        objs = [s.primal_objective() for s in solvers]
+        summary.update(prepare_s=prep, succeeded=int(sum(v == 1 for v in st)),
+                       iterations_mean=float(np.mean(its)),
+                       not_succeeded=[(k, stream[k][1], stream[k][2].get("n"), stream[k][2].get("m"),
+                                       st[k], its[k]) for k in range(len(st)) if st[k] != 1])
+        print(f"{precision} slots={slots}: {summary}", file=sys.stderr, flush=True)
+        return summary, its, objs
+
+    if args.stream_paths == "plain":
+        plain, it_p, obj_p = one(1, "fp64")
+        batched, it_b, obj_b = plain, it_p, obj_p
+    elif args.stream_paths == "batched":
+        batched, it_b, obj_b = one(args.slots, "mixed")
+        plain, it_p, obj_p = batched, it_b, obj_b
+    else:
+        plain, it_p, obj_p = one(1, "fp64")
+        batched, it_b, obj_b = one(args.slots, "mixed")
+    dit = [b - a for a, b in zip(it_p, it_b)]
+    dobj = max(abs(a - b) / max(1.0, abs(a)) for a, b in zip(obj_p, obj_b))
+    return {
+        "metric": METRIC,
+        "value": batched["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.count,
+        "warmup": 0,
+        "ms_per_step": 1e3 / batched["solves_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded numpy PCG64 instances, Poisson arrivals)",
+        "config": {"workload": f"serving stream: first {args.count} of the 4096-instance seed-4 stream "
+                               f"(half max-cut n~U{{20..600}}, half dense n~U{{16..200}}, m~U{{50..3000}}), "
+                               f"Poisson rate {args.rate}/s, tol 1e-7",
+                   "slots": args.slots, "rate": args.rate, "count": args.count},
+        "p50_latency_ms": 1e3 * batched["p50_latency_s"],
+        "p99_latency_ms": 1e3 * batched["p99_latency_s"],
+        "plain_fp64": plain,
+        "batched_mixed": batched,
+        "speedup_vs_plain": batched["solves_per_s"] / plain["solves_per_s"],
+        "max_extra_iterations": int(max(dit)),
+        "max_rel_objective_diff": dobj,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def cpu_baseline(model, iters_gpu, budget_s):
     """The FP64 oracle on the same instance: initialization plus as many IPM
     iterations as fit in about `budget_s` seconds (at least one); solves/s is
@@ -99,11 +171,29 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--count", type=int, default=128, help="stream: number of requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="stream: Poisson arrival rate (1/s)")
+    ap.add_argument("--slots", type=int, default=16, help="stream: concurrent solves")
+    ap.add_argument("--mem-budget", type=int, default=128, help="stream: device GiB for admission")
+    ap.add_argument("--stream-paths", default="both", choices=["both", "plain", "batched"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(independent solves shard across GPUs with no exchange).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "stream":
+        # one hardware queue per concurrent solve (HIP's default is 4); set
+        # before the HIP runtime starts
+        prev = os.environ.get("GPU_MAX_HW_QUEUES")
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, args.slots)))
+        print(f"GPU_MAX_HW_QUEUES {prev} -> {os.environ['GPU_MAX_HW_QUEUES']}; "
+              f"{len(os.sched_getaffinity(0))} host cores", file=sys.stderr)
+        rec = run_stream(args)
+        print(json.dumps(rec), flush=True)
+        # every request must end with the same status on both paths
+        same = rec["plain_fp64"]["not_succeeded"] == rec["batched_mixed"]["not_succeeded"]
+        return 0 if same else 1
 
     import ctypes as C
     import numpy as np

@@ -157,6 +157,9 @@ int msdp_solver_profile_end(msdp_solver* s, double* total_ms, int64_t* launches,
  * non-blocking stream.  No counterpart in the reference: Julia runs one
  * solver per task, the serving loop below runs one per stream. */
 int msdp_solver_set_stream(msdp_solver* s, void* stream);
+/* Allocate the device state and upload the problem data now (otherwise done
+ * by the first msdp_solver_initialize).  No counterpart in the reference. */
+int msdp_solver_upload(msdp_solver* s);
 /* Device bytes the solver holds once initialized, estimated from the sizes
  * alone (admission by memory footprint; no counterpart in the reference). */
 int msdp_solver_footprint(const msdp_solver* s, int64_t* bytes);
@@ -178,7 +181,7 @@ int msdp_solver_release(msdp_solver* s);
  * until its request is done. */
 typedef struct msdp_server msdp_server;
 typedef struct {
-  double arrival, admitted, done;  /* seconds after msdp_server_start */
+  double arrival, admitted, uploaded, done;  /* seconds after msdp_server_start */
   int64_t footprint;               /* bytes */
   int32_t rc;                      /* msdp_solver_solve's return (0 = ok) */
   int32_t slot;

@@ -20,6 +20,11 @@ constexpr int WAVE = 64;
 void set_error(const std::string& msg);
 const char* last_error();
 int fail_hip(hipError_t e, const char* what, const char* file, int line);
+// Host wait for a stream.  By default the calling thread spins (lowest
+// latency for one solve); a thread that switched to blocking mode sleeps on a
+// blocking-sync event instead, leaving its core to other solves' threads.
+int stream_sync(hipStream_t s);
+void set_blocking_sync(bool on);
 #define MSDP_HIP(x)                                                      \
   do {                                                                   \
     hipError_t e_ = (x);                                                 \
@@ -140,6 +145,8 @@ struct EigBatch {
 };
 constexpr int JACOBI_VB_MAX = 128;
 int jacobi_vbatched(hipStream_t s, const EigBatch* items_dev, int count, bool vectors);
+// eigenvalues only (w ascending if non-null, else just *wmin), n <= 128
+int eigvals_vbatched(hipStream_t s, const EigBatch* items_dev, int count);
 // Batched Jacobi for many small matrices (n <= 64).
 int jacobi_batched(hipStream_t s, int n, const double* A, int64_t lda, int64_t strideA,
                    double* w, int64_t stridew, double* V, int64_t ldv, int64_t strideV,

@@ -14,6 +14,38 @@ namespace msdp {
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+struct SyncEvent {
+  hipEvent_t ev = nullptr;
+  int device = -1;
+  ~SyncEvent() {
+    if (ev) (void)hipEventDestroy(ev);
+  }
+};
+thread_local bool t_blocking = false;
+thread_local SyncEvent t_sync;
+}  // namespace
+
+void set_blocking_sync(bool on) { t_blocking = on; }
+
+int stream_sync(hipStream_t s) {
+  if (!t_blocking) {
+    MSDP_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+  int dev = 0;
+  MSDP_HIP(hipGetDevice(&dev));
+  if (!t_sync.ev || t_sync.device != dev) {
+    if (t_sync.ev) MSDP_HIP(hipEventDestroy(t_sync.ev));
+    t_sync.ev = nullptr;
+    MSDP_HIP(hipEventCreateWithFlags(&t_sync.ev, hipEventBlockingSync | hipEventDisableTiming));
+    t_sync.device = dev;
+  }
+  MSDP_HIP(hipEventRecord(t_sync.ev, s));
+  MSDP_HIP(hipEventSynchronize(t_sync.ev));
+  return 0;
+}
 const char* last_error() { return g_err.c_str(); }
 int fail_hip(hipError_t e, const char* what, const char* file, int line) {
   char buf[512];

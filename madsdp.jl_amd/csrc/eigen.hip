@@ -469,7 +469,172 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
   if (lane == 0) *out = acc;
 }
 
+// --------------------------------------------------------------------------
+// One-workgroup tridiagonalisation for n <= TRD1_MAX: the whole reduction in
+// a single launch (3n launches of the blocked scheme otherwise), the working
+// matrix in global memory (L2-resident at these sizes).  Unblocked Householder
+// (LAPACK's sytd2, lower) with the rank-2 update of step k fused with the
+// matrix-vector product of step k+1: column k+1 is updated first, its
+// reflector v' formed, then one pass over the trailing block applies
+// A -= v w' + w v' and accumulates p' = A v'.  Same outputs as the blocked
+// routine (V, tau, d, e).
+// --------------------------------------------------------------------------
+constexpr int TRD1_MAX = 640;
+constexpr int TRD1_NT = 1024;
+
+__device__ inline void trd1_reflector(int n, int c, double* A, double* V, double* tau, double* d,
+                                      double* e, double* vdst, double* red, double* s_tau,
+                                      double* s_scale) {
+  const int tid = threadIdx.x;
+  const double* col = A + (int64_t)c * n;
+  double ss = 0.0;
+  for (int i = c + 2 + tid; i < n; i += TRD1_NT) ss += col[i] * col[i];
+  ss = block_reduce<0>(ss, red);
+  if (tid == 0) {
+    const double alpha = col[c + 1];
+    double beta, t, sc;
+    if (ss == 0.0) {
+      beta = alpha; t = 0.0; sc = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+      t = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    d[c] = col[c];
+    e[c] = beta;
+    tau[c] = t;
+    *s_tau = t;
+    *s_scale = sc;
+  }
+  __syncthreads();
+  const double sc = *s_scale;
+  for (int i = c + 1 + tid; i < n; i += TRD1_NT) {
+    const double v = i == c + 1 ? 1.0 : col[i] * sc;
+    vdst[i] = v;
+    V[i + (int64_t)c * n] = v;
+  }
+  __syncthreads();
+}
+
+// rows/cols >= c0: if FUSED, A -= v w' + w v' first; p = A vn (unscaled).
+// Wave g (of 16) takes the columns c0 + g + 16 m, lane = row within a 64-row
+// chunk; loads of the next 8 columns are issued before the stores of the
+// current 8, so the stores' latency overlaps the loads'.  The 16 partial
+// sums of every row are added in a fixed order.
+constexpr int TRD1_W = TRD1_NT / 64;
+
+template <bool FUSED>
+__device__ inline void trd1_pass(int n, int c0, double* A, const double* v, const double* w,
+                                 const double* vn, double* p, double (*part)[TRD1_MAX]) {
+  constexpr int U = 8, S = TRD1_W;
+  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+  for (int rb = c0; rb < n; rb += 64) {
+    const int i = rb + lane;
+    double acc = 0.0;
+    if (i < n) {
+      const double vi = FUSED ? v[i] : 0.0, wi = FUSED ? w[i] : 0.0;
+      double* Ai = A + i;
+      int j = c0 + g;
+      double a[U], b[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] = j + S * u < n ? Ai[(int64_t)(j + S * u) * n] : 0.0;
+      for (; j < n; j += S * U) {
+        const int jn = j + S * U;
+#pragma unroll
+        for (int u = 0; u < U; ++u) b[u] = jn + S * u < n ? Ai[(int64_t)(jn + S * u) * n] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int jj = j + S * u;
+          if (jj < n) {
+            double x = a[u];
+            if (FUSED) {
+              x -= vi * w[jj] + wi * v[jj];
+              Ai[(int64_t)jj * n] = x;
+            }
+            acc += x * vn[jj];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = b[u];
+      }
+      part[g][i - c0] = acc;
+    }
+  }
+  __syncthreads();
+  for (int i = c0 + tid; i < n; i += TRD1_NT) {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < S; ++q) sum += part[q][i - c0];
+    p[i] = sum;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(TRD1_NT) void trd1_kernel(int n, const double* Ain, int64_t lda,
+                                                       double* A, double* V, double* tau,
+                                                       double* d, double* e) {
+  __shared__ double vs[TRD1_MAX], ws[TRD1_MAX], vn[TRD1_MAX], ps[TRD1_MAX];
+  __shared__ double part[TRD1_W][TRD1_MAX];
+  __shared__ double red[16];
+  __shared__ double s_tau, s_scale;
+  const int tid = threadIdx.x;
+  for (int64_t x = tid; x < (int64_t)n * n; x += TRD1_NT) {
+    const int i = (int)(x % n), j = (int)(x / n);
+    A[x] = Ain[i + (int64_t)j * lda];
+    V[x] = 0.0;
+  }
+  __syncthreads();
+  if (n <= 2) {
+    if (tid == 0) {
+      d[0] = A[0];
+      if (n == 2) {
+        e[0] = A[1];
+        d[1] = A[3];
+        tau[0] = 0.0;
+      }
+    }
+    return;
+  }
+  trd1_reflector(n, 0, A, V, tau, d, e, vs, red, &s_tau, &s_scale);
+  trd1_pass<false>(n, 1, A, nullptr, nullptr, vs, ps, part);
+  for (int k = 0;; ++k) {
+    const double t = s_tau;
+    double pv = 0.0;
+    for (int i = k + 1 + tid; i < n; i += TRD1_NT) pv += ps[i] * vs[i];
+    pv = block_reduce<0>(pv, red);
+    const double c2 = 0.5 * t * t * pv;
+    for (int i = k + 1 + tid; i < n; i += TRD1_NT) ws[i] = t * ps[i] - c2 * vs[i];
+    __syncthreads();
+    if (k == n - 3) {  // the trailing 2 x 2 block, then no reflector
+      if (tid == 0) {
+        const int a = n - 2, b = n - 1;
+        const double aa = A[a + (int64_t)a * n] - 2.0 * vs[a] * ws[a];
+        const double ba = A[b + (int64_t)a * n] - (vs[b] * ws[a] + ws[b] * vs[a]);
+        const double bb = A[b + (int64_t)b * n] - 2.0 * vs[b] * ws[b];
+        d[a] = aa;
+        e[a] = ba;
+        d[b] = bb;
+        tau[a] = 0.0;
+      }
+      return;
+    }
+    const int c = k + 1;
+    for (int i = c + tid; i < n; i += TRD1_NT)
+      A[i + (int64_t)c * n] -= vs[i] * ws[c] + ws[i] * vs[c];
+    __syncthreads();
+    trd1_reflector(n, c, A, V, tau, d, e, vn, red, &s_tau, &s_scale);
+    trd1_pass<true>(n, c + 1, A, vs, ws, vn, ps, part);
+    for (int i = c + 1 + tid; i < n; i += TRD1_NT) vs[i] = vn[i];
+    __syncthreads();
+  }
+}
+
 static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWork& t) {
+  if (n <= TRD1_MAX) {
+    trd1_kernel<<<1, TRD1_NT, 0, s>>>(n, A, lda, t.A, t.V, t.tau, t.d, t.e);
+    MSDP_HIP(hipGetLastError());
+    return 0;
+  }
   MSDP_TRY(copy_mat(s, n, n, A, lda, t.A, n));
   MSDP_TRY(fill(s, (int64_t)n * n, 0.0, t.V));
   const int last = n - 3;  // last column with a reflector
@@ -562,6 +727,141 @@ __global__ __launch_bounds__(256) void tridiag_min_kernel(int n, const double* d
     __syncthreads();
   }
   if (tid == 0) *out = 0.5 * (lo_s + hi_s);
+}
+
+// ==========================================================================
+// Eigenvalues only, n <= 128, a batch of matrices of different sizes (one
+// workgroup each): Householder tridiagonalisation with the matrix in LDS,
+// then Sturm bisection (all eigenvalues, ascending, one thread per index) or
+// multisection for the smallest.  Replaces the Jacobi sweeps where no
+// eigenvectors are needed (step lengths, the DualCentering predictor).
+// ==========================================================================
+constexpr int EV_MAX = 128;
+
+__global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) {
+  const EigBatch it = items[blockIdx.x];
+  const int n = it.n;
+  __shared__ double As[EV_MAX][EV_MAX + 1];
+  __shared__ double v[EV_MAX], p[EV_MAX], w[EV_MAX], d[EV_MAX], e[EV_MAX];
+  __shared__ double red[16];
+  __shared__ double s_tau, s_scale, lo_s, hi_s;
+  __shared__ int first;
+  const int tid = threadIdx.x, NT = 512;
+  for (int x = tid; x < n * n; x += NT) {
+    const int i = x % n, j = x / n;
+    As[i][j] = it.A[i + (int64_t)j * it.lda];
+  }
+  __syncthreads();
+  for (int k = 0; k < n - 2; ++k) {
+    double ss = 0.0;
+    for (int i = k + 2 + tid; i < n; i += NT) ss += As[i][k] * As[i][k];
+    ss = block_reduce<0>(ss, red);
+    if (tid == 0) {
+      const double alpha = As[k + 1][k];
+      double beta, t, sc;
+      if (ss == 0.0) {
+        beta = alpha; t = 0.0; sc = 0.0;
+      } else {
+        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+        t = (beta - alpha) / beta;
+        sc = 1.0 / (alpha - beta);
+      }
+      d[k] = As[k][k];
+      e[k] = beta;
+      s_tau = t;
+      s_scale = sc;
+    }
+    __syncthreads();
+    const double t = s_tau;
+    if (t == 0.0) continue;  // uniform: H_k = I
+    const double sc = s_scale;
+    for (int i = k + 1 + tid; i < n; i += NT) v[i] = i == k + 1 ? 1.0 : As[i][k] * sc;
+    __syncthreads();
+    const int L = n - k - 1;
+    {  // p = t A22 v, 4 lanes per row
+      const int r = tid >> 2, q = tid & 3, i = k + 1 + r;
+      double acc = 0.0;
+      if (r < L)
+        for (int j = k + 1 + q; j < n; j += 4) acc += As[i][j] * v[j];
+      acc += __shfl_xor(acc, 1);
+      acc += __shfl_xor(acc, 2);
+      if (q == 0 && r < L) p[i] = t * acc;
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int i = k + 1 + tid; i < n; i += NT) pv += p[i] * v[i];
+    pv = block_reduce<0>(pv, red);
+    const double c2 = 0.5 * t * pv;
+    for (int i = k + 1 + tid; i < n; i += NT) w[i] = p[i] - c2 * v[i];
+    __syncthreads();
+    for (int x = tid; x < L * L; x += NT) {
+      const int i = k + 1 + x % L, j = k + 1 + x / L;
+      As[i][j] -= v[i] * w[j] + w[i] * v[j];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      d[n - 2] = As[n - 2][n - 2];
+      e[n - 2] = As[n - 1][n - 2];
+    }
+    d[n - 1] = As[n - 1][n - 1];
+  }
+  __syncthreads();
+  // Gershgorin bounds
+  double gl = INFINITY, gu = -INFINITY, emax = 0.0;
+  for (int i = tid; i < n; i += NT) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+    gl = fmin(gl, d[i] - r);
+    gu = fmax(gu, d[i] + r);
+    if (i < n - 1) emax = fmax(emax, e[i] * e[i]);
+  }
+  gl = block_reduce<2>(gl, red);
+  gu = block_reduce<1>(gu, red);
+  emax = block_reduce<1>(emax, red);
+  const double pivmin = DBL_MIN * fmax(1.0, emax);
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double lo0 = gl - 2 * DEPS * tnorm - 2 * pivmin, hi0 = gu + 2 * DEPS * tnorm + 2 * pivmin;
+  if (it.w) {  // every eigenvalue by bisection on its index
+    if (tid < n) {
+      double lo = lo0, hi = hi0;
+      for (int s = 0; s < 128; ++s) {
+        if (hi - lo <= 2 * DEPS * fmax(fabs(lo), fabs(hi)) + 4 * pivmin) break;
+        const double mid = 0.5 * (lo + hi);
+        if (sturm_count(n, d, e, mid, pivmin) >= tid + 1) hi = mid;
+        else lo = mid;
+      }
+      const double lam = 0.5 * (lo + hi);
+      it.w[tid] = lam;
+      if (tid == 0 && it.wmin) *it.wmin = lam;
+    }
+    return;
+  }
+  if (tid == 0) { lo_s = lo0; hi_s = hi0; }
+  __syncthreads();
+  for (int round = 0; round < 12; ++round) {  // 513-way multisection for the smallest
+    const double lo = lo_s, hi = hi_s;
+    if (hi - lo <= 2 * DEPS * fmax(fabs(lo), fabs(hi)) + 4 * pivmin) break;
+    if (tid == 0) first = NT;
+    __syncthreads();
+    const double x = lo + (hi - lo) * (double)(tid + 1) / (NT + 1.0);
+    if (sturm_count(n, d, e, x, pivmin) >= 1) atomicMin(&first, tid);
+    __syncthreads();
+    if (tid == 0) {
+      const int f = first;
+      lo_s = f == 0 ? lo : lo + (hi - lo) * (double)f / (NT + 1.0);
+      hi_s = f == NT ? hi : lo + (hi - lo) * (double)(f + 1) / (NT + 1.0);
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && it.wmin) *it.wmin = 0.5 * (lo_s + hi_s);
+}
+
+int eigvals_vbatched(hipStream_t s, const EigBatch* items, int count) {
+  if (count <= 0) return 0;
+  eigvals_vb_kernel<<<count, 512, 0, s>>>(items);
+  MSDP_HIP(hipGetLastError());
+  return 0;
 }
 
 // ==========================================================================
@@ -996,7 +1296,7 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
     MSDP_HIP(hipGetLastError());
     std::vector<int> kp(nn);
     MSDP_HIP(hipMemcpyAsync(kp.data(), v.kp, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
-    MSDP_HIP(hipStreamSynchronize(s));
+    MSDP_TRY(stream_sync(s));
     // root -> (node, local) maps
     std::vector<int> rmap;
     std::vector<int> rloc;
@@ -1023,13 +1323,13 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
         MSDP_TRY(gemm(s, false, false, K, kp[q], kp[q], 1.0, Qg + level[q].qoff, K,
                       Ub + level[q].qoff, kp[q], 0.0, Qn + level[q].qoff, K));
       }
-      MSDP_HIP(hipStreamSynchronize(s));  // keep `both` alive until consumed
+      MSDP_TRY(stream_sync(s));  // keep `both` alive until consumed
     }
     dc_rank_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes_dev, v, lam, colsrc);
     dc_scatter_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes_dev, v, Qg, Qn, colsrc,
                                                                        lam, Z, n);
     MSDP_HIP(hipGetLastError());
-    MSDP_HIP(hipStreamSynchronize(s));  // `level` staging copy consumed
+    MSDP_TRY(stream_sync(s));  // `level` staging copy consumed
   }
   return 0;
 }
@@ -1352,7 +1652,7 @@ static int lanczos_min(hipStream_t s, int n, const double* A, int64_t lda, doubl
       lz_check_kernel<<<1, 256, 0, s>>>(k, al, be, scr, chk);
       MSDP_HIP(hipGetLastError());
       MSDP_HIP(hipMemcpyAsync(hv, chk, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-      MSDP_HIP(hipStreamSynchronize(s));
+      MSDP_TRY(stream_sync(s));
       anorm_est = std::max(anorm_est, std::fabs(hv[0]));
       const bool invariant = hv[2] <= 1e-14 * std::max(anorm_est, 1e-300);
       if (hv[1] <= LZ_RTOL * std::max(std::fabs(hv[0]), 1e-300) || invariant || k == n) break;

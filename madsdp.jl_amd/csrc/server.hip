@@ -22,7 +22,7 @@ namespace {
 
 struct Req {
   msdp_solver* s = nullptr;
-  double arrival = 0, admitted = -1, done = -1;
+  double arrival = 0, admitted = -1, uploaded = -1, done = -1;
   int64_t footprint = 0;
   int rc = 0, slot = -1;
   bool finished = false;
@@ -43,6 +43,7 @@ struct msdp_server {
   size_t next = 0;
   int64_t completed = 0;
   bool started = false, closing = false;
+  bool blocking = true;  // worker threads sleep while their solve's kernels run
   std::chrono::steady_clock::time_point t0;
 
   double elapsed() const {
@@ -53,6 +54,7 @@ struct msdp_server {
 
 void msdp_server::work(int slot) {
   (void)hipSetDevice(device);
+  msdp::set_blocking_sync(blocking);
   for (;;) {
     Req* r = nullptr;
     {
@@ -84,6 +86,8 @@ void msdp_server::work(int slot) {
       r->slot = slot;
     }
     int rc = msdp_solver_set_stream(r->s, streams[slot]);
+    if (rc == 0) rc = msdp_solver_upload(r->s);
+    const double t_up = elapsed();
     if (rc == 0) rc = msdp_solver_solve(r->s);
     std::string err = rc < 0 ? std::string(msdp_last_error()) : std::string();
     // always: frees the device state and drops the slot's stream
@@ -96,6 +100,7 @@ void msdp_server::work(int slot) {
       std::lock_guard<std::mutex> lk(mu);
       r->rc = rc;
       r->err = err;
+      r->uploaded = t_up;
       r->done = elapsed();
       r->finished = true;
       in_use -= r->footprint;
@@ -181,6 +186,7 @@ int msdp_server_request(const msdp_server* cv_, int64_t id, msdp_request_info* o
   const Req& r = v->reqs[id];
   out->arrival = r.arrival;
   out->admitted = r.admitted;
+  out->uploaded = r.uploaded;
   out->done = r.done;
   out->footprint = r.footprint;
   out->rc = r.rc;

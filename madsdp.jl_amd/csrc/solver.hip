@@ -95,7 +95,6 @@ struct msdp_solver {
   double* swork = nullptr;
   size_t swork_elems = 0;
   double* eigall = nullptr;  // DualCentering proximity eigenvalues, all blocks
-  double* eigw = nullptr;    // step-length eigenvalues of the small blocks
   // batched Jacobi descriptors of the small (n <= 128) blocks
   EigBatch *eb_nt = nullptr, *eb_step = nullptr, *eb_dc = nullptr;
   int nsmall = 0;
@@ -167,14 +166,14 @@ struct msdp_solver {
   int read_slots() {
     hsc.resize(nslots);
     MSDP_HIP(hipMemcpyAsync(hsc_pin, sc, sizeof(double) * nslots, hipMemcpyDeviceToHost, st));
-    MSDP_HIP(hipStreamSynchronize(st));
+    MSDP_TRY(stream_sync(st));
     std::memcpy(hsc.data(), hsc_pin, sizeof(double) * nslots);
     return 0;
   }
   int read_infos(std::vector<int>& out) {
     out.resize(4 * nb + 8);
     MSDP_HIP(hipMemcpyAsync(out.data(), info, sizeof(int) * out.size(), hipMemcpyDeviceToHost, st));
-    MSDP_HIP(hipStreamSynchronize(st));
+    MSDP_TRY(stream_sync(st));
     return 0;
   }
   int* binfo(int i, int k) { return info + 8 + 4 * i + k; }
@@ -330,7 +329,7 @@ int msdp_solver::finalize() {
       MSDP_HIP(hipMemcpyAsync(o.pos_lin, plin.data(), sizeof(int64_t) * plin.size(), hipMemcpyHostToDevice, st));
       MSDP_HIP(hipMemcpyAsync(o.pos_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice, st));
       MSDP_HIP(hipMemcpyAsync(o.pos_ent, pent.data(), sizeof(int) * pent.size(), hipMemcpyHostToDevice, st));
-      MSDP_HIP(hipStreamSynchronize(st));  // the host arrays above are locals
+      MSDP_TRY(stream_sync(st));  // the host arrays above are locals
     } else {
       o.kind = BlockOps::EMPTY;
     }
@@ -375,7 +374,6 @@ int msdp_solver::finalize() {
   int64_t tot = 0;
   for (auto& bb : blk) tot += bb.n;
   MSDP_TRY(alloc(&eigall, std::max<int64_t>(tot, 1)));
-  MSDP_TRY(alloc(&eigw, 2 * std::max<int64_t>(tot, 1)));
   {
     std::vector<EigBatch> nt_items, step_items, dc_items;
     int64_t off = 0;
@@ -383,8 +381,8 @@ int msdp_solver::finalize() {
       Blk& bk = blk[i];
       if (bk.n <= JACOBI_VB_MAX) {
         nt_items.push_back(EigBatch{bk.n, bk.Nm, bk.n, bk.lam, nullptr, bk.Vv, bk.n, bk.Mm});
-        step_items.push_back(EigBatch{bk.n, bk.tmp2, bk.n, eigw + 2 * off, sc + bslot(i, 3), nullptr, 0, nullptr});
-        step_items.push_back(EigBatch{bk.n, bk.buf2, bk.n, eigw + 2 * off + bk.n, sc + bslot(i, 4), nullptr, 0, nullptr});
+        step_items.push_back(EigBatch{bk.n, bk.tmp2, bk.n, nullptr, sc + bslot(i, 3), nullptr, 0, nullptr});
+        step_items.push_back(EigBatch{bk.n, bk.buf2, bk.n, nullptr, sc + bslot(i, 4), nullptr, 0, nullptr});
         dc_items.push_back(EigBatch{bk.n, bk.tmp2, bk.n, eigall + off, nullptr, nullptr, 0, nullptr});
       }
       off += bk.n;
@@ -397,7 +395,7 @@ int msdp_solver::finalize() {
       MSDP_HIP(hipMemcpyAsync(eb_nt, nt_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice, st));
       MSDP_HIP(hipMemcpyAsync(eb_step, step_items.data(), sizeof(EigBatch) * 2 * nsmall, hipMemcpyHostToDevice, st));
       MSDP_HIP(hipMemcpyAsync(eb_dc, dc_items.data(), sizeof(EigBatch) * nsmall, hipMemcpyHostToDevice, st));
-      MSDP_HIP(hipStreamSynchronize(st));
+      MSDP_TRY(stream_sync(st));
     }
   }
   // chunk buffers of the dense Schur assembly: up to 64 constraints or 256 MiB
@@ -500,7 +498,7 @@ int msdp_solver::keep_solution() {
     MSDP_HIP(hipMemcpyAsync(r_X[i].data(), blk[i].X, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
     MSDP_HIP(hipMemcpyAsync(r_S[i].data(), blk[i].S, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
   }
-  MSDP_HIP(hipStreamSynchronize(st));
+  MSDP_TRY(stream_sync(st));
   return 0;
 }
 
@@ -682,7 +680,7 @@ int msdp_solver::factorize_schur(bool& ok) {
     MSDP_TRY(potrf(st, (int)m, H, m, info, Hdinv));
     int hinfo = 0;
     MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
-    MSDP_HIP(hipStreamSynchronize(st));
+    MSDP_TRY(stream_sync(st));
     if (hinfo == 0) {
       MSDP_TRY(transpose(st, (int)m, (int)m, H, m, Ht, m));
       reg_used = reg > 0 ? reg : 0.0;
@@ -801,7 +799,7 @@ int msdp_solver::step_lengths(double& a, double& bt) {
           MSDP_TRY(eig_min(st, n, T[k], n, slot(bslot(i, 3 + k)), ework, eiwork));
       }
     }
-    if (nsmall > 0) MSDP_TRY(jacobi_vbatched(st, eb_step, 2 * nsmall, false));
+    if (nsmall > 0) MSDP_TRY(eigvals_vbatched(st, eb_step, 2 * nsmall));
     if (ns > 0) {
       MSDP_TRY(scalar_min_step(st, (int)ns, x, delx, slot(3)));
       MSDP_TRY(scalar_min_step(st, (int)ns, z, delz, slot(4)));
@@ -833,7 +831,7 @@ int msdp_solver::step_lengths(double& a, double& bt) {
         MSDP_TRY(potrf(st, n, bk.buf, n, info, trwork));
         int hinfo = 0;
         MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
-        MSDP_HIP(hipStreamSynchronize(st));
+        MSDP_TRY(stream_sync(st));
         if (hinfo == 0) break;
         al *= opt.shrink;
         if (al < EPS) { out = 0.0; return 0; }
@@ -1030,7 +1028,7 @@ int msdp_solver::initial_point() {
       else MSDP_HIP(hipMemcpyAsync(x, ip_x.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
     }
     MSDP_TRY(set_dual_slacks(y));
-    MSDP_HIP(hipStreamSynchronize(st));
+    MSDP_TRY(stream_sync(st));
     return 0;
   }
   // DualShift
@@ -1093,7 +1091,7 @@ int msdp_solver::solve() {
     if (status != REGULAR) break;
     MSDP_TRY(update_termination());
   }
-  MSDP_HIP(hipStreamSynchronize(st));
+  MSDP_TRY(stream_sync(st));
   return 0;
 }
 
@@ -1191,7 +1189,7 @@ int msdp_solver::predictor_step(bool& ok) {  // :232-264
     MSDP_TRY(dot(st, n2, bk.buf2, bk.buf2, slot(bslot(i, 6)), red));
     off += n;
   }
-  if (nsmall > 0) MSDP_TRY(jacobi_vbatched(st, eb_dc, nsmall, false));
+  if (nsmall > 0) MSDP_TRY(eigvals_vbatched(st, eb_dc, nsmall));
   std::vector<int> inf;
   MSDP_TRY(read_infos(inf));
   for (int i = 0; i < nb; ++i)
@@ -1403,7 +1401,7 @@ int msdp_solver::qr_factorize(bool& ok) {
   double rd[2] = {0, 0};
   MSDP_HIP(hipMemcpyAsync(&bad, qbad, sizeof(int), hipMemcpyDeviceToHost, st));
   MSDP_HIP(hipMemcpyAsync(rd, qrd, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-  MSDP_HIP(hipStreamSynchronize(st));
+  MSDP_TRY(stream_sync(st));
   if (bad) return 0;              // zero or non-finite column (qr_schur.jl:74-77)
   if (rows < m) return 0;         // more multipliers than rows (:83-85)
   // A dense Householder QR leaves roundoff where SPQR's exact zero pivot would
@@ -1571,7 +1569,7 @@ int msdp_solver_set_cost_block(msdp_solver* s, int32_t block, const double* C) {
   }
   MSDP_HIP(hipSetDevice(s->device));
   MSDP_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * n2, hipMemcpyHostToDevice, s->st));
-  MSDP_HIP(hipStreamSynchronize(s->st));
+  MSDP_TRY(stream_sync(s->st));
   return 0;
 }
 
@@ -1622,7 +1620,7 @@ int msdp_solver_set_rhs(msdp_solver* s, const double* bv) {
   MSDP_HIP(hipSetDevice(s->device));
   if (s->m > 0) {
     MSDP_HIP(hipMemcpyAsync(s->b, bv, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
-    MSDP_HIP(hipStreamSynchronize(s->st));
+    MSDP_TRY(stream_sync(s->st));
   }
   return 0;
 }
@@ -1640,7 +1638,7 @@ int msdp_solver_set_scalar_data(msdp_solver* s, const double* d_lin, const doubl
     MSDP_HIP(hipMemcpyAsync(s->dlin, d_lin, sizeof(double) * s->ns, hipMemcpyHostToDevice, s->st));
     if (s->m > 0)
       MSDP_HIP(hipMemcpyAsync(s->Clin, C_lin, sizeof(double) * s->ns * s->m, hipMemcpyHostToDevice, s->st));
-    MSDP_HIP(hipStreamSynchronize(s->st));
+    MSDP_TRY(stream_sync(s->st));
   }
   return 0;
 }
@@ -1675,7 +1673,7 @@ int msdp_solver_update_termination(msdp_solver* s) { GUARD_READY(s); MSDP_HIP(hi
 int msdp_solver_solve(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->solve(); }
 int msdp_solver_synchronize(msdp_solver* s) {
   GUARD0(s);
-  if (s->st && !s->released) MSDP_HIP(hipStreamSynchronize(s->st));
+  if (s->st && !s->released) MSDP_TRY(stream_sync(s->st));
   return 0;
 }
 int msdp_solver_set_stream(msdp_solver* s, void* stream) {
@@ -1693,13 +1691,18 @@ int msdp_solver_footprint(const msdp_solver* s, int64_t* bytes) {
   *bytes = s->footprint();
   return 0;
 }
+int msdp_solver_upload(msdp_solver* s) {
+  GUARD(s);
+  MSDP_HIP(hipSetDevice(s->device));
+  return s->finalize();
+}
 int msdp_solver_release(msdp_solver* s) {
   GUARD0(s);
   return s->release();
 }
 int msdp_solver_profile_begin(msdp_solver* s, int32_t cls) {
   GUARD_READY(s);
-  MSDP_HIP(hipStreamSynchronize(s->st));
+  MSDP_TRY(stream_sync(s->st));
   return prof_begin(cls, s->st);
 }
 int msdp_solver_profile_end(msdp_solver* s, double* ms, int64_t* launches, double* work) {
@@ -1757,7 +1760,7 @@ int msdp_solver_schur_solve(msdp_solver* s, double* rhs) {
   MSDP_HIP(hipMemcpyAsync(s->wm, rhs, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
   MSDP_TRY(potrs(s->st, (int)s->m, s->H, s->Ht, s->m, s->Hdinv, s->wm, s->wm2));
   MSDP_HIP(hipMemcpyAsync(rhs, s->wm, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->st));
-  MSDP_HIP(hipStreamSynchronize(s->st));
+  MSDP_TRY(stream_sync(s->st));
   return 0;
 }
 
@@ -1809,7 +1812,7 @@ int msdp_solver_get(const msdp_solver* cs, int32_t which, int32_t block, double*
   double* p; size_t c;
   if (arr(s, which, block, &p, &c) != 0) return as_err("bad array selector or block");
   MSDP_HIP(hipMemcpyAsync(host, p, sizeof(double) * c, hipMemcpyDeviceToHost, s->st));
-  MSDP_HIP(hipStreamSynchronize(s->st));
+  MSDP_TRY(stream_sync(s->st));
   return 0;
 }
 
@@ -1819,7 +1822,7 @@ int msdp_solver_set(msdp_solver* s, int32_t which, int32_t block, const double* 
   double* p; size_t c;
   if (arr(s, which, block, &p, &c) != 0) return as_err("bad array selector or block");
   MSDP_HIP(hipMemcpyAsync(p, host, sizeof(double) * c, hipMemcpyHostToDevice, s->st));
-  MSDP_HIP(hipStreamSynchronize(s->st));
+  MSDP_TRY(stream_sync(s->st));
   return 0;
 }
 

@@ -48,9 +48,11 @@ def maxcut(n: int, p: float, seed: int, weights: str = "unit") -> Model:
 
 
 def _sym_normal(rng, k, n):
+    """k symmetric n x n matrices: the upper triangle (diagonal included) of
+    iid N(0,1) draws, mirrored."""
     A = rng.standard_normal((k, n, n))
-    iu = np.triu_indices(n, 1)
-    A[:, iu[1], iu[0]] = A[:, iu[0], iu[1]]
+    low = np.tril(np.ones((n, n), dtype=bool), -1)
+    np.copyto(A, np.swapaxes(A, 1, 2), where=low)
     return A
 
 

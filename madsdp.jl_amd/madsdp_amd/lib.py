@@ -45,7 +45,8 @@ class Info(C.Structure):
 
 
 class RequestInfo(C.Structure):
-    _fields_ = [("arrival", C.c_double), ("admitted", C.c_double), ("done", C.c_double),
+    _fields_ = [("arrival", C.c_double), ("admitted", C.c_double), ("uploaded", C.c_double),
+                ("done", C.c_double),
                 ("footprint", C.c_int64), ("rc", C.c_int32), ("slot", C.c_int32),
                 ("finished", C.c_int32), ("pad", C.c_int32)]
 
@@ -92,6 +93,7 @@ SIGNATURES = {
     "msdp_solver_set_stream": (C.c_int, [P, P]),
     "msdp_solver_footprint": (C.c_int, [P, c_int64_p]),
     "msdp_solver_release": (C.c_int, [P]),
+    "msdp_solver_upload": (C.c_int, [P]),
     "msdp_server_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int32, C.c_int64]),
     "msdp_server_start": (C.c_int, [P]),
     "msdp_server_submit": (C.c_int, [P, P, C.c_double, c_int64_p]),

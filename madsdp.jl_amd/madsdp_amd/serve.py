@@ -26,6 +26,7 @@ from .solver import MadSDPSolver
 class RequestResult:
     arrival: float
     admitted: float
+    uploaded: float
     done: float
     footprint: int
     slot: int
@@ -66,7 +67,8 @@ class Server:
         info = L.RequestInfo()
         L.check(self.lib.msdp_server_request(self._h, rid, C.byref(info)), "msdp_server_request")
         L.check(info.rc, f"request {rid}")
-        return RequestResult(info.arrival, info.admitted, info.done, info.footprint, info.slot)
+        return RequestResult(info.arrival, info.admitted, info.uploaded, info.done, info.footprint,
+                             info.slot)
 
     def stats(self) -> L.ServerStats:
         st = L.ServerStats()
@@ -98,9 +100,12 @@ def serve(solvers: List[MadSDPSolver], arrivals, slots: int = 8, mem_budget: int
     st = srv.stats()
     srv.close()
     lat = np.array([r.latency for r in res])
+    service = np.array([r.done - r.admitted for r in res])
     span = max(r.done for r in res) - min(r.arrival for r in res)
     summary = dict(completed=len(res), span_s=span, solves_per_s=len(res) / span,
                    p50_latency_s=float(np.percentile(lat, 50)),
                    p99_latency_s=float(np.percentile(lat, 99)),
+                   mean_service_s=float(service.mean()), max_service_s=float(service.max()),
+                   mean_upload_s=float(np.mean([r.uploaded - r.admitted for r in res])),
                    peak_active=int(st.peak_active), peak_bytes=int(st.peak_bytes))
     return res, summary

@@ -215,7 +215,8 @@ class MadSDPSolver:
             L.check(lib.msdp_solver_set_cost_block(self._h, i, Cf.ctypes.data_as(L.c_double_p)))
             blk = model.A[i]
             if blk.kind == "dense":
-                mats = np.ascontiguousarray(np.transpose(blk.mats, (0, 2, 1)))  # column-major per matrix
+                # constraint matrices are symmetric: row- and column-major agree
+                mats = np.ascontiguousarray(blk.mats, dtype=np.float64)
                 idx = np.ascontiguousarray(blk.con_idx, dtype=np.int64)
                 L.check(lib.msdp_solver_set_constraints_dense(self._h, i, idx.size, L.iptr(idx), L.dptr(mats)))
             else:
