@@ -176,6 +176,9 @@ def main():
     ap.add_argument("--slots", type=int, default=16, help="stream: concurrent solves")
     ap.add_argument("--mem-budget", type=int, default=128, help="stream: device GiB for admission")
     ap.add_argument("--stream-paths", default="both", choices=["both", "plain", "batched"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run an elementwise kernel over MIB MiB three times (the "
+                         "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -199,6 +202,14 @@ def main():
     import numpy as np
     from madsdp_amd import MadSDPSolver, Status
 
+    if args.copy_calibration > 0:
+        import torch
+        a = torch.ones(args.copy_calibration << 17, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)  # an elementwise kernel: reads and writes MIB MiB
+        torch.cuda.synchronize()
+        del a, b
     model, config = build_model(args.workload)
     s = MadSDPSolver(model, device=0, precision=args.precision)
     for _ in range(args.warmup):
@@ -223,9 +234,14 @@ def main():
         if launches.value > 0 and ms.value > 0:
             if cls in (0, 2):
                 achieved = work.value / (ms.value * 1e-3) / 1e12
+                traffic = None
+                tf = os.path.join(ROOT, "profiles", f"gemm_traffic_{args.workload}.json")
+                if cls == 0 and args.precision == "fp64" and os.path.exists(tf):
+                    with open(tf) as fh:
+                        traffic = json.load(fh)["traffic_bytes_per_launch"]
                 peak = PEAKS["mfma"] if cls == 0 else PEAKS["mfma32"]
                 roof = {"bound": "mfma", "achieved": achieved, "peak": peak,
-                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                         "kernel": "gemm_f64_kernel (FP64 MFMA 16x16x4)" if cls == 0
                         else "gemm128_f32_kernel (FP32 MFMA 16x16x4)",
                         "launches": launches.value, "avg_launch_us": 1e3 * ms.value / launches.value,
