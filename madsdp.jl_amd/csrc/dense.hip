@@ -248,8 +248,10 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, i
                                                           const double* __restrict__ A,
                                                           int64_t lda,
                                                           const double* __restrict__ B,
-                                                          int64_t ldb, double* __restrict__ P) {
+                                                          int64_t ldb, double* __restrict__ P,
+                                                          int lower) {
   const int tm = blockIdx.x, tn = blockIdx.y, part = blockIdx.z;
+  if (lower && tn > tm) return;  // wholly above the diagonal: never read back
   const int kbeg = part * kc, kend = min(K, kbeg + kc);
   __shared__ double As[GK][GT + 1];
   __shared__ double Bs[GK][GT + 1];
@@ -309,12 +311,13 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, i
 }
 
 __global__ void splitk_reduce_kernel(int M, int N, int parts, const double* P, double alpha,
-                                     double beta, double* C, int64_t ldc) {
+                                     double beta, double* C, int64_t ldc, int lower) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)M * N) return;
+  const int i = (int)(e % M), j = (int)(e / M);
+  if (lower && i < j) return;
   double v = 0.0;
   for (int q = 0; q < parts; ++q) v += P[e + (int64_t)q * M * N];
-  const int i = (int)(e % M), j = (int)(e / M);
   double* c = C + i + (int64_t)j * ldc;
   *c = beta == 0.0 ? alpha * v : alpha * v + beta * (*c);
 }
@@ -338,7 +341,7 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   }
   const bool prof = prof_active(PROF_GEMM, s);
   const int64_t tiles64 = (int64_t)cdiv(M, GT) * cdiv(N, GT) * batch;
-  if (!lower && !guard && batch == 1 && tiles64 < 128 && K >= 1024) {
+  if (!guard && batch == 1 && tiles64 < 128 && K >= 1024) {
     // split K so that about 256 workgroups run
     int parts = (int)std::min<int64_t>(std::max<int64_t>(2, 256 / tiles64), K / 256);
     const int kc = ((K + parts - 1) / parts + GK - 1) / GK * GK;
@@ -348,12 +351,13 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     if (prof) prof_before(s);
     dim3 g3(cdiv(M, GT), cdiv(N, GT), parts);
     double* Pd = static_cast<double*>(P);
-    if (!ta && !tb) gemm_splitk_kernel<false, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
-    else if (ta && !tb) gemm_splitk_kernel<true, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
-    else if (!ta && tb) gemm_splitk_kernel<false, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
-    else gemm_splitk_kernel<true, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd);
-    splitk_reduce_kernel<<<cdiv((int64_t)M * N, 256), 256, 0, s>>>(M, N, parts, Pd, alpha, beta, C, ldc);
-    if (prof) prof_after(s, 2.0 * M * N * (double)K);
+    if (!ta && !tb) gemm_splitk_kernel<false, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
+    else if (ta && !tb) gemm_splitk_kernel<true, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
+    else if (!ta && tb) gemm_splitk_kernel<false, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
+    else gemm_splitk_kernel<true, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
+    splitk_reduce_kernel<<<cdiv((int64_t)M * N, 256), 256, 0, s>>>(M, N, parts, Pd, alpha, beta, C, ldc,
+                                                                     lower);
+    if (prof) prof_after(s, 2.0 * M * N * (double)K * (lower ? 0.5 : 1.0));
     MSDP_HIP(hipGetLastError());
     MSDP_HIP(hipFreeAsync(P, s));
     return 0;

@@ -123,9 +123,11 @@ def _spd_clustered(rng, n, clusters):
     return (Q * lam) @ Q.T
 
 
-@pytest.mark.parametrize("n", [2, 10, 64, 65, 100, 257, 600])
+@pytest.mark.parametrize("n", [2, 10, 64, 65, 100, 257, 600, 700, 1100])
 @pytest.mark.parametrize("kind", ["random", "clustered", "identity"])
 def test_syevd(env, n, kind):
+    """n <= 64 Jacobi; <= 640 one-workgroup tridiagonalisation; above, the
+    blocked one; then divide and conquer and the back-transform."""
     torch, lib, st = env
     rng = np.random.default_rng(n)
     if kind == "random":
