@@ -75,6 +75,16 @@ struct msdp_solver {
   std::vector<double> r_y, r_x, r_z;
   std::vector<std::vector<double>> r_X, r_S;
   double* hsc_pin = nullptr;
+  // Multi-block problems whose blocks are all small (n <= JACOBI_VB_MAX): the
+  // per-block work of a stage runs on NAUX auxiliary streams (block i on
+  // stream i % NAUX) between a fork and a join on the main stream, each with
+  // its own scratch.  Accumulations shared by blocks stay on the main stream.
+  static constexpr int NAUX = 8;
+  bool multi = false;
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> aux_done;
+  hipEvent_t forked = nullptr;
+  std::vector<double*> aux_trwork, aux_red, aux_swork;
   int nb = 0;
   int64_t m = 0, ns = 0;
   std::vector<Blk> blk;
@@ -128,6 +138,7 @@ struct msdp_solver {
 
   ~msdp_solver() {
     free_device();
+    destroy_aux();
     if (hsc_pin) (void)hipHostFree(hsc_pin);
     if (st && own_stream) (void)hipStreamDestroy(st);
   }
@@ -158,6 +169,54 @@ struct msdp_solver {
   }
   int release();
   int keep_solution();
+  void destroy_aux() {
+    for (hipStream_t q : aux) (void)hipStreamDestroy(q);
+    for (hipEvent_t ev : aux_done) (void)hipEventDestroy(ev);
+    if (forked) (void)hipEventDestroy(forked);
+    aux.clear();
+    aux_done.clear();
+    aux_trwork.clear();
+    aux_red.clear();
+    aux_swork.clear();
+    forked = nullptr;
+    multi = false;
+  }
+  int setup_aux();
+  int fork() {
+    if (!multi) return 0;
+    MSDP_HIP(hipEventRecord(forked, st));
+    for (hipStream_t q : aux) MSDP_HIP(hipStreamWaitEvent(q, forked, 0));
+    return 0;
+  }
+  int join() {
+    if (!multi) return 0;
+    for (int k = 0; k < NAUX; ++k) {
+      MSDP_HIP(hipEventRecord(aux_done[k], aux[k]));
+      MSDP_HIP(hipStreamWaitEvent(st, aux_done[k], 0));
+    }
+    return 0;
+  }
+  // block i's work on its auxiliary stream (and scratch) for the scope
+  struct OnBlock {
+    msdp_solver* s;
+    hipStream_t st0;
+    double *tw0, *red0, *sw0;
+    OnBlock(msdp_solver* s_, int i) : s(s_), st0(s_->st), tw0(s_->trwork), red0(s_->red), sw0(s_->swork) {
+      if (s->multi) {
+        const int k = i % NAUX;
+        s->st = s->aux[k];
+        s->trwork = s->aux_trwork[k];
+        s->red = s->aux_red[k];
+        s->swork = s->aux_swork[k];
+      }
+    }
+    ~OnBlock() {
+      s->st = st0;
+      s->trwork = tw0;
+      s->red = red0;
+      s->swork = sw0;
+    }
+  };
   int64_t footprint() const;
 
   double* slot(int k) { return sc + k; }
@@ -232,6 +291,34 @@ static int as_err(const char* msg) {
 // ==========================================================================
 // set-up
 // ==========================================================================
+int msdp_solver::setup_aux() {
+  if (nb < 2) return 0;
+  int nmax = 1;
+  for (auto& b : blk) {
+    if (b.n > JACOBI_VB_MAX) return 0;
+    nmax = std::max(nmax, b.n);
+  }
+  const size_t tw = std::max(trtri_work_elems(nmax), potrf_work_elems(nmax));
+  for (int k = 0; k < NAUX; ++k) {
+    hipStream_t q;
+    MSDP_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    aux.push_back(q);
+    hipEvent_t ev;
+    MSDP_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    aux_done.push_back(ev);
+    double *a, *b, *c = nullptr;
+    MSDP_TRY(alloc(&a, tw));
+    MSDP_TRY(alloc(&b, reduce_scratch_elems(0) + 64));
+    if (swork_elems) MSDP_TRY(alloc(&c, swork_elems));
+    aux_trwork.push_back(a);
+    aux_red.push_back(b);
+    aux_swork.push_back(c);
+  }
+  MSDP_HIP(hipEventCreateWithFlags(&forked, hipEventDisableTiming));
+  multi = true;
+  return 0;
+}
+
 int msdp_solver::finalize() {
   if (released) return as_err("solver released: its device state was freed after the solve");
   if (finalized) return 0;
@@ -408,6 +495,7 @@ int msdp_solver::finalize() {
     if (bb.ops.kind == BlockOps::DENSE) want = std::max(want, 2 * (size_t)bb.n * bb.n);
   swork_elems = want;
   if (want) MSDP_TRY(alloc(&swork, want));
+  MSDP_TRY(setup_aux());
   // norms of the data (src/solver.jl:119-120, 217-221, 322, 330)
   for (int i = 0; i < nb; ++i) MSDP_TRY(max_abs(st, (int64_t)blk[i].n * blk[i].n, blk[i].C, slot(bslot(i, 0)), red));
   if (m > 0) MSDP_TRY(max_abs(st, m, b, slot(0), red));
@@ -474,6 +562,7 @@ int msdp_solver::release() {
   int rc = 0;
   if (finalized) rc = keep_solution();
   free_device();
+  destroy_aux();
   released = finalized;
   if (!own_stream) st = nullptr;
   return rc;
@@ -555,13 +644,16 @@ int msdp_solver::update_residuals() {
     if (ns > 0) MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, x, ns, 1.0, Rp, m));
     MSDP_TRY(axpby(st, m, 1.0, b, -1.0, Rp, Rp));
   }
+  MSDP_TRY(fork());
   for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
     Blk& bk = blk[i];
     const int64_t n2 = (int64_t)bk.n * bk.n;
     MSDP_TRY(jtprod(st, bk.ops, y, bk.tmp2));
     MSDP_TRY(axpbypcz(st, n2, 1.0, bk.C, -1.0, bk.S, -1.0, bk.tmp2, bk.Rd));
     MSDP_TRY(max_abs(st, n2, bk.Rd, slot(bslot(i, 1)), red));
   }
+  MSDP_TRY(join());
   if (ns > 0) {
     MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, y, m, 0.0, ws1, ns));
     MSDP_TRY(axpbypcz(st, ns, 1.0, dlin, -1.0, z, -1.0, ws1, Rdlin));
@@ -579,7 +671,9 @@ int msdp_solver::update_residuals() {
 int msdp_solver::prepare(bool& ok) {
   MSDP_TRY(fill(st, 1, 0.0, slot(2)));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
+  MSDP_TRY(fork());
   for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
     Blk& bk = blk[i];
     const int n = bk.n;
     MSDP_TRY(prepare_block(bk, i));
@@ -598,11 +692,16 @@ int msdp_solver::prepare(bool& ok) {
       }
     }
   }
+  MSDP_TRY(join());
   if (nt() && nsmall > 0) {
     // eigen-decompositions of all small blocks in one launch
     MSDP_TRY(jacobi_vbatched(st, eb_nt, nsmall, true));
-    for (int i = 0; i < nb; ++i)
+    MSDP_TRY(fork());
+    for (int i = 0; i < nb; ++i) {
+      OnBlock on(this, i);
       if (blk[i].n <= JACOBI_VB_MAX) MSDP_TRY(nt_scaling(blk[i], i));
+    }
+    MSDP_TRY(join());
   }
   if (ns > 0) MSDP_TRY(scalar_op(st, SOP_ZINV, (int)ns, 0, nullptr, z, nullptr, nullptr, nullptr, zinv, nullptr, false));
   std::vector<int> inf;
@@ -720,7 +819,10 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
   // rhs
   if (m > 0) {
     MSDP_HIP(hipMemcpyAsync(dely, Rp, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-    for (auto& bk : blk) {
+    MSDP_TRY(fork());
+    for (int i = 0; i < nb; ++i) {
+      OnBlock on(this, i);
+      Blk& bk = blk[i];
       const int64_t n2 = (int64_t)bk.n * bk.n;
       MSDP_TRY(axpby(st, n2, 1.0, bk.X, -mu_, bk.Si, bk.buf));
       if (bk.rd_nonzero) {
@@ -728,8 +830,10 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
         MSDP_TRY(axpby(st, n2, 1.0, bk.buf, 1.0, bk.buf2, bk.buf));
       }
       if (corrected) MSDP_TRY(axpby(st, n2, 1.0, bk.buf, -1.0, bk.corr, bk.buf));
-      MSDP_TRY(jprod(st, bk.ops, bk.buf, dely));
     }
+    MSDP_TRY(join());
+    // A(.) accumulates every block into dely: in block order on the main stream
+    for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.buf, dely));
     if (ns > 0) {
       MSDP_TRY(scalar_op(st, SOP_RHS, (int)ns, mu_, x, z, zinv, Rdlin, corrlin, ws1, nullptr, corrected));
       MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, ws1, ns, 1.0, dely, m));
@@ -737,7 +841,10 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     MSDP_TRY(schur_solve(dely));
   }
   // directions
-  for (auto& bk : blk) {
+  MSDP_TRY(fork());
+  for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
+    Blk& bk = blk[i];
     const int n = bk.n;
     const int64_t n2 = (int64_t)n * n;
     MSDP_TRY(jtprod(st, bk.ops, dely, bk.tmp2));
@@ -753,6 +860,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     if (corrected) MSDP_TRY(axpby(st, n2, 1.0, bk.delX, 1.0, bk.corr, bk.delX));
     MSDP_TRY(symmetrize(st, n, bk.delX, n));
   }
+  MSDP_TRY(join());
   if (ns > 0) {
     MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, dely, m, 0.0, ws1, ns));
     if (corrected) MSDP_HIP(hipMemcpyAsync(delx, corrlin, sizeof(double) * ns, hipMemcpyDeviceToDevice, st));
@@ -763,7 +871,10 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
 
 // correction! (step.jl:263-271; directions.jl:90-100 NT, 123-127 HKM)
 int msdp_solver::correction() {
-  for (auto& bk : blk) {
+  MSDP_TRY(fork());
+  for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
+    Blk& bk = blk[i];
     const int n = bk.n;
     if (nt()) {
       MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Gi, n, bk.delX, n, 0.0, bk.tmp, n));
@@ -778,6 +889,7 @@ int msdp_solver::correction() {
     }
     MSDP_TRY(symmetrize(st, n, bk.corr, n));
   }
+  MSDP_TRY(join());
   if (ns > 0) MSDP_TRY(scalar_op(st, SOP_CORR, (int)ns, 0, nullptr, nullptr, zinv, delx, delz, corrlin, nullptr, false));
   return 0;
 }
@@ -785,7 +897,9 @@ int msdp_solver::correction() {
 // step_lengths (step.jl:186-198 EigenStep, 200-228 CholeskyBacktrack)
 int msdp_solver::step_lengths(double& a, double& bt) {
   if (opt.step == MSDP_STEP_EIGEN) {
+    MSDP_TRY(fork());
     for (int i = 0; i < nb; ++i) {
+      OnBlock on(this, i);
       Blk& bk = blk[i];
       const int n = bk.n;
       const double* Ls[2] = {bk.Lxinv, bk.Lsinv};
@@ -799,6 +913,7 @@ int msdp_solver::step_lengths(double& a, double& bt) {
           MSDP_TRY(eig_min(st, n, T[k], n, slot(bslot(i, 3 + k)), ework, eiwork));
       }
     }
+    MSDP_TRY(join());
     if (nsmall > 0) MSDP_TRY(eigvals_vbatched(st, eb_step, 2 * nsmall));
     if (ns > 0) {
       MSDP_TRY(scalar_min_step(st, (int)ns, x, delx, slot(3)));
@@ -853,7 +968,9 @@ int msdp_solver::step_lengths(double& a, double& bt) {
 
 // affine_complementarity (step.jl:231-243)
 int msdp_solver::affine_complementarity(double a, double bt, double& out) {
+  MSDP_TRY(fork());
   for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
     Blk& bk = blk[i];
     const int64_t n2 = (int64_t)bk.n * bk.n;
     MSDP_TRY(dot(st, n2, bk.X, bk.S, slot(bslot(i, 3)), red));
@@ -861,6 +978,7 @@ int msdp_solver::affine_complementarity(double a, double bt, double& out) {
     MSDP_TRY(dot(st, n2, bk.delX, bk.S, slot(bslot(i, 5)), red));
     MSDP_TRY(dot(st, n2, bk.delX, bk.delS, slot(bslot(i, 6)), red));
   }
+  MSDP_TRY(join());
   if (ns > 0) MSDP_TRY(scalar_affine(st, (int)ns, x, delx, z, delz, a, bt, slot(5)));
   MSDP_TRY(read_slots());
   double s = 0.0;
@@ -876,8 +994,12 @@ int msdp_solver::affine_complementarity(double a, double bt, double& out) {
 }
 
 int msdp_solver::total_complementarity(double& out) {
-  for (int i = 0; i < nb; ++i)
+  MSDP_TRY(fork());
+  for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
     MSDP_TRY(dot(st, (int64_t)blk[i].n * blk[i].n, blk[i].X, blk[i].S, slot(bslot(i, 3)), red));
+  }
+  MSDP_TRY(join());
   if (ns > 0) MSDP_TRY(dot(st, ns, x, z, slot(5), red));
   MSDP_TRY(read_slots());
   double s = 0.0;
@@ -891,13 +1013,17 @@ int msdp_solver::total_complementarity(double& out) {
 int msdp_solver::take_step(double a, double bt) {
   alpha = a;
   beta = bt;
-  for (auto& bk : blk) {
+  MSDP_TRY(fork());
+  for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
+    Blk& bk = blk[i];
     const int64_t n2 = (int64_t)bk.n * bk.n;
     MSDP_TRY(axpby(st, n2, 1.0, bk.X, a, bk.delX, bk.X));
     MSDP_TRY(symmetrize(st, bk.n, bk.X, bk.n));
     MSDP_TRY(axpby(st, n2, 1.0, bk.S, bt, bk.delS, bk.S));
     MSDP_TRY(symmetrize(st, bk.n, bk.S, bk.n));
   }
+  MSDP_TRY(join());
   if (m > 0) MSDP_TRY(axpby(st, m, 1.0, y, bt, dely, y));
   if (ns > 0) {
     MSDP_TRY(axpby(st, ns, 1.0, x, a, delx, x));
@@ -950,8 +1076,12 @@ int msdp_solver::iterate() {
 // update_termination! (primal_dual.jl:123-135, dual_centering.jl:293-303)
 int msdp_solver::update_termination() {
   if (dual_centering()) MSDP_TRY(update_residuals());
-  for (int i = 0; i < nb; ++i)
+  MSDP_TRY(fork());
+  for (int i = 0; i < nb; ++i) {
+    OnBlock on(this, i);
     MSDP_TRY(dot(st, (int64_t)blk[i].n * blk[i].n, blk[i].C, blk[i].X, slot(bslot(i, 7)), red));
+  }
+  MSDP_TRY(join());
   if (ns > 0) {
     MSDP_TRY(dot(st, ns, dlin, x, slot(6), red));
     MSDP_TRY(max_abs(st, ns, Rdlin, slot(7), red));
