@@ -189,7 +189,8 @@ int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, doubl
     MSDP_HIP(hipGetLastError());
     return 0;
   }
-  // dense: P_j = op(Lm) (A_j op(F2)); B[:, j] = svec(P_j) or vec(P_j); Hb = B'B
+  // dense: P_j = op(Lm) (A_j op(F2)); B[:, j] = svec(P_j) or vec(P_j); Hb = B'B,
+// scattered into H unless H is null (then the caller scatters b.Hbuf later)
   const int64_t n2 = (int64_t)n * n;
   const int64_t drow = sc.full_vec ? n2 : (int64_t)n * (n + 1) / 2;
   const int mi = b.ncon;
@@ -218,9 +219,7 @@ int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, doubl
     double* Hb = b.Hbuf;
     MSDP_TRY(gemm(st, true, false, mi, mi, (int)drow, 1.0, Bm, drow, Bm, drow, 0.0, Hb, mi, 1, 0,
                   0, 0, true));
-    dim3 g(cdiv(mi, 128), mi);
-    scatter_lower_kernel<<<g, 128, 0, st>>>(mi, b.con_list, Hb, H, m);
-    MSDP_HIP(hipGetLastError());
+    if (H) MSDP_TRY(scatter_lower_add(st, mi, b.con_list, Hb, H, m));
   }
   return 0;
 }

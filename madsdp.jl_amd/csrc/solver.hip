@@ -260,6 +260,7 @@ struct msdp_solver {
   int apply_H(const double* v, double* out);
   int schur_solve(double* v);
   int factorize_schur(bool& ok);
+  int schur_scaling(Blk& bk, SchurScaling& sc);
   int apply_E(Blk& b, const double* M, bool diag, double* out);
   int solve_directions(double mu_, bool corrected);
   int correction();
@@ -730,6 +731,27 @@ int msdp_solver::nt_scaling(Blk& bk, int i) {
   return 0;
 }
 
+// (L, R) of the Schur complement and the factored dense form for one block
+// (src/directions.jl:77,112; src/dual_centering.jl:66)
+int msdp_solver::schur_scaling(Blk& bk, SchurScaling& sc) {
+  if (dual_centering()) {
+    sc.L = bk.Si; sc.R = bk.Si;
+    sc.Lm = bk.Lsinv; sc.transL = false; sc.F2 = bk.Lsinv; sc.transF2 = true; sc.full_vec = false;
+  } else if (nt()) {
+    sc.L = bk.W; sc.R = bk.W;
+    sc.Lm = bk.G; sc.transL = true; sc.F2 = bk.G; sc.transF2 = false; sc.full_vec = false;
+  } else {
+    sc.L = bk.Si; sc.R = bk.X;
+    sc.Lm = bk.Lsinv; sc.transL = false; sc.F2 = bk.Lx; sc.transF2 = false; sc.full_vec = true;
+    if (bk.ops.kind == BlockOps::DENSE && opt.step != MSDP_STEP_EIGEN) {
+      // HKM needs chol(X) for the factored dense form
+      const int i = (int)(&bk - blk.data());
+      MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 3)));
+    }
+  }
+  return 0;
+}
+
 // factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
 int msdp_solver::factorize_schur(bool& ok) {
   ok = false;
@@ -740,24 +762,30 @@ int msdp_solver::factorize_schur(bool& ok) {
   bool any_lowp = false;
   if (lowp) MSDP_TRY(ensure_mixed());
   MSDP_TRY(fill(st, m * m, 0.0, Hs));
-  for (auto& bk : blk) {
+  // Multi-block: the per-block Schur contributions of dense blocks that touch
+  // a subset of the constraints are formed concurrently into their own
+  // buffers, then scattered into H in block order on the main stream.
+  const bool split = multi && !lowp;
+  auto deferred = [&](const Blk& bk) {
+    return split && bk.ops.kind == BlockOps::DENSE && !bk.ops.dcon_identity && bk.ops.ncon > 0;
+  };
+  MSDP_TRY(fork());
+  for (int i = 0; i < nb; ++i) {
+    Blk& bk = blk[i];
+    if (!deferred(bk)) continue;
+    OnBlock on(this, i);
     SchurScaling sc;
-    const int n = bk.n;
-    if (dual_centering()) {
-      sc.L = bk.Si; sc.R = bk.Si;
-      sc.Lm = bk.Lsinv; sc.transL = false; sc.F2 = bk.Lsinv; sc.transF2 = true; sc.full_vec = false;
-    } else if (nt()) {
-      sc.L = bk.W; sc.R = bk.W;
-      sc.Lm = bk.G; sc.transL = true; sc.F2 = bk.G; sc.transF2 = false; sc.full_vec = false;
-    } else {
-      sc.L = bk.Si; sc.R = bk.X;
-      sc.Lm = bk.Lsinv; sc.transL = false; sc.F2 = bk.Lx; sc.transF2 = false; sc.full_vec = true;
-      if (bk.ops.kind == BlockOps::DENSE && opt.step != MSDP_STEP_EIGEN) {
-        // HKM needs chol(X) for the factored dense form
-        MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(0, 3)));
-      }
+    MSDP_TRY(schur_scaling(bk, sc));
+    MSDP_TRY(schur_block(st, bk.ops, sc, nullptr, m, swork, swork_elems));
+  }
+  MSDP_TRY(join());
+  for (auto& bk : blk) {
+    if (deferred(bk)) {
+      MSDP_TRY(scatter_lower_add(st, bk.ops.ncon, bk.ops.con_list, bk.ops.Hbuf, Hs, m));
+      continue;
     }
-    (void)n;
+    SchurScaling sc;
+    MSDP_TRY(schur_scaling(bk, sc));
     if (lowp && bk.ops.kind == BlockOps::DENSE) {
       MSDP_TRY(schur_block_fp32(bk, sc));
       any_lowp = true;
