@@ -95,6 +95,10 @@ size_t trtri_work_elems(int n);
 int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch);
 // elementwise helpers
 int symmetrize(hipStream_t s, int n, double* A, int64_t lda);
+// L = chol((M + M')/2) with the flush and a zero upper triangle, n <= 64, one
+// launch; *info = 0, or the 1-based first non-positive pivot
+int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
+                   int* info);
 int mirror_lower(hipStream_t s, int n, double* A, int64_t lda);  // upper := lower'
 int copy_mat(hipStream_t s, int rows, int cols, const double* A, int64_t lda, double* B,
              int64_t ldb);

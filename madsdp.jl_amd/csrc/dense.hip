@@ -752,6 +752,70 @@ int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch) {
   return 0;
 }
 
+// _cholesky_factor! for n <= 64 in one launch of one wavefront: the
+// symmetrised copy, the factorisation (right-looking, the matrix in LDS, one
+// row per lane, so no workgroup barriers), the flush and the zeroed upper
+// triangle.  Same arithmetic per entry as the blocked path's diagonal kernel.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64) void chol_small_kernel(int n, const double* M, int64_t ldm,
+                                                        double* L, int64_t ldl, int* info) {
+  __shared__ double As[64][65];
+  const int lane = threadIdx.x;
+  for (int j = 0; j < n; ++j)
+    if (lane < n && lane >= j) {
+      const double a = M[lane + (int64_t)j * ldm], b = M[j + (int64_t)lane * ldm];
+      As[lane][j] = (a + b) / 2;
+    }
+  wave_lds_sync();
+  int fail = 0;
+  for (int k = 0; k < n; ++k) {
+    const double d = As[k][k];
+    if (!(d > 0.0) || !isfinite(d)) {  // the same value in every lane
+      fail = k + 1;
+      break;
+    }
+    const double lkk = sqrt(d);
+    wave_lds_sync();
+    if (lane == k) As[k][k] = lkk;
+    if (lane > k && lane < n) As[lane][k] = As[lane][k] / lkk;
+    wave_lds_sync();
+    if (lane > k && lane < n) {
+      const double lik = As[lane][k];
+      for (int j = k + 1; j <= lane; ++j) As[lane][j] -= lik * As[j][k];
+    }
+    wave_lds_sync();
+  }
+  if (lane == 0) *info = fail;
+  if (fail) return;
+  double dm = lane < n ? fabs(As[lane][lane]) : 0.0;
+  dm = wave_max(dm);
+  const double eps = 2.220446049250313e-16;
+  const double tol = eps * eps * dm;
+  for (int j = 0; j < n; ++j) {
+    if (lane >= n) continue;
+    double v = 0.0;
+    if (lane == j) v = As[lane][j];
+    else if (lane > j) v = fabs(As[lane][j]) < tol ? 0.0 : As[lane][j];
+    L[lane + (int64_t)j * ldl] = v;
+  }
+}
+
+int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
+                   int* info) {
+  if (n <= 0 || n > 64) {
+    set_error("cholesky_small: n must be in 1..64");
+    return -2;
+  }
+  chol_small_kernel<<<1, 64, 0, s>>>(n, M, ldm, L, ldl, info);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
 // ==========================================================================
 // elementwise
 // ==========================================================================

@@ -598,6 +598,7 @@ int msdp_solver::keep_solution() {
 // _cholesky_factor! (blocks.jl:94-113)
 int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) {
   const int n = bk.n;
+  if (n <= 64) return cholesky_small(st, n, M, n, L, n, inf);
   MSDP_TRY(copy_mat(st, n, n, M, n, L, n));
   MSDP_TRY(symmetrize(st, n, L, n));
   MSDP_TRY(potrf(st, n, L, n, inf, trwork));
@@ -2000,6 +2001,7 @@ size_t msdp_k_cholesky_factor_scratch(int32_t n) {
 int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm, double* L, int64_t ldl,
                            int32_t* info, double* scratch) {
   hipStream_t st = (hipStream_t)stream;
+  if (n >= 1 && n <= 64) return cholesky_small(st, n, M, ldm, L, ldl, info);
   MSDP_TRY(copy_mat(st, n, n, M, ldm, L, ldl));
   MSDP_TRY(symmetrize(st, n, L, ldl));
   MSDP_TRY(potrf(st, n, L, ldl, info, scratch + reduce_scratch_elems(0) + 64));
