@@ -338,7 +338,21 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
   if (i < n) {
     double pcv = j > 0 ? p[i] : 0.0;
     double bv = k >= 0 ? A[i + (int64_t)k * n] : 0.0;
-    for (int t = 0; t < ncor; ++t) {
+    int t = 0;
+    for (; t + 3 < ncor; t += 4) {  // loads of four panel columns issued together
+      double vt[4], wt[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        vt[u] = V[i + (int64_t)(k0 + t + u) * n];
+        wt[u] = W[i + (int64_t)(t + u) * n];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pcv -= vt[u] * hw[t + u] + wt[u] * hv[t + u];
+        if (k >= 0) bv -= vt[u] * rowW[t + u] + wt[u] * rowV[t + u];
+      }
+    }
+    for (; t < ncor; ++t) {
       const double vt = V[i + (int64_t)(k0 + t) * n], wt = W[i + (int64_t)t * n];
       pcv -= vt * hw[t] + wt * hv[t];
       if (k >= 0) bv -= vt * rowW[t] + wt * rowV[t];
@@ -357,8 +371,9 @@ __global__ __launch_bounds__(1024) void trd_reflector_kernel(
     int n, int k, int j, int k0, int nparts, const double* A, double* V, double* W,
     const double* pc, const double* base, const double* part, double* tau, double* d,
     double* e) {
+  constexpr int CACHE = 8;  // column entries kept in registers per thread (n <= 8192)
   __shared__ double red[16];
-  __shared__ double s_scale, s_wk, s_vk;
+  __shared__ double s_scale, s_wk, s_vk, s_tp, s_c2;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int prev = j - 1;
   const int kp = k0 + prev;
@@ -368,39 +383,74 @@ __global__ __launch_bounds__(1024) void trd_reflector_kernel(
     double sd = 0.0;
     for (int q = tid; q < nparts; q += nt) sd += part[q];
     sd = block_reduce<0>(sd, red);
-    const double tp = tau[kp];
-    const double c2 = 0.5 * tp * tp * sd;
-    for (int i = tid; i < n; i += nt) wp[i] = i > kp ? tp * pc[i] - c2 * vp[i] : 0.0;
-    if (tid == 0 && k >= 0) {
-      s_wk = tp * pc[k] - c2 * vp[k];
-      s_vk = vp[k];
+    if (tid == 0) {
+      const double tp = tau[kp];
+      const double c2 = 0.5 * tp * tp * sd;
+      s_tp = tp;
+      s_c2 = c2;
+      if (k >= 0) {
+        s_wk = tp * pc[k] - c2 * vp[k];
+        s_vk = vp[k];
+      }
     }
     __syncthreads();
   }
-  if (k < 0) return;
-  auto col = [&](int i) -> double {
-    double c = base[i];
-    if (j > 0) c -= vp[i] * s_wk + wp[i] * s_vk;
-    return c;
-  };
-  if (tid == 0) d[k] = col(k);
+  if (k < 0) {  // only w_{k-1}: column j-1 of W
+    const double tp = s_tp, c2 = s_c2;
+    for (int i = tid; i < n; i += nt) wp[i] = i > kp ? tp * pc[i] - c2 * vp[i] : 0.0;
+    return;
+  }
+  // one pass: w_{k-1} (stored), column k corrected with it, its squared norm
+  double cache[CACHE];
+  double ss = 0.0;
+  {
+    const double tp = j > 0 ? s_tp : 0.0, c2 = j > 0 ? s_c2 : 0.0;
+    const double wk = j > 0 ? s_wk : 0.0, vk = j > 0 ? s_vk : 0.0;
+    int c = 0;
+    for (int i = tid; i < n; i += nt, ++c) {
+      double col = base[i];
+      if (j > 0) {
+        const double vi = vp[i];
+        const double wi = i > kp ? tp * pc[i] - c2 * vi : 0.0;
+        wp[i] = wi;
+        col -= vi * wk + wi * vk;
+      }
+#pragma unroll
+      for (int q = 0; q < CACHE; ++q)
+        if (q == c) cache[q] = col;
+      if (i >= k + 2) ss += col * col;
+      if (i == k) d[k] = col;
+      if (k == n - 2 && i == n - 1) e[k] = col;
+    }
+  }
   if (k == n - 2) {
     if (tid == 0) {
-      e[k] = col(n - 1);
       tau[k] = 0.0;
       d[n - 1] = A[(n - 1) + (int64_t)(n - 1) * n];
     }
     for (int i = tid; i < n; i += nt) V[i + (int64_t)k * n] = 0.0;
     return;
   }
-  double ss = 0.0;
-  for (int i = k + 2 + tid; i < n; i += nt) {
-    const double c = col(i);
-    ss += c * c;
-  }
   ss = block_reduce<0>(ss, red);
+  // alpha = corrected column entry k+1, from the thread that holds it
+  __shared__ double s_alpha;
+  if ((k + 1) % nt == tid) {
+    const int c = (k + 1) / nt;
+    double col;
+    if (c < CACHE) {
+      col = cache[0];
+#pragma unroll
+      for (int q = 1; q < CACHE; ++q)
+        if (q == c) col = cache[q];
+    } else {
+      col = base[k + 1];
+      if (j > 0) col -= vp[k + 1] * s_wk + wp[k + 1] * s_vk;
+    }
+    s_alpha = col;
+  }
+  __syncthreads();
   if (tid == 0) {
-    const double alpha = col(k + 1);
+    const double alpha = s_alpha;
     double beta, t, sc;
     if (ss == 0.0) {
       beta = alpha; t = 0.0; sc = 0.0;
@@ -416,11 +466,24 @@ __global__ __launch_bounds__(1024) void trd_reflector_kernel(
   __syncthreads();
   double* vk = V + (int64_t)k * n;
   const double sc = s_scale;
-  for (int i = tid; i < n; i += nt) {
+  int c = 0;
+  for (int i = tid; i < n; i += nt, ++c) {
     double v;
     if (i <= k) v = 0.0;
     else if (i == k + 1) v = 1.0;
-    else v = col(i) * sc;
+    else {
+      double col;
+      if (c < CACHE) {
+        col = cache[0];
+#pragma unroll
+        for (int q = 1; q < CACHE; ++q)
+          if (q == c) col = cache[q];
+      } else {
+        col = base[i];
+        if (j > 0) col -= vp[i] * s_wk + wp[i] * s_vk;
+      }
+      v = col * sc;
+    }
     vk[i] = v;
   }
 }
@@ -463,8 +526,17 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
   } else {
     return;
   }
-  double acc = 0.0;
-  for (int i = base + lane; i < n; i += 64) acc += colp[i] * v[i];
+  // four independent partial sums keep four loads in flight per lane
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = base + lane;
+  for (; i + 192 < n; i += 256) {
+    a0 += colp[i] * v[i];
+    a1 += colp[i + 64] * v[i + 64];
+    a2 += colp[i + 128] * v[i + 128];
+    a3 += colp[i + 192] * v[i + 192];
+  }
+  for (; i < n; i += 64) a0 += colp[i] * v[i];
+  double acc = (a0 + a1) + (a2 + a3);
   acc = wave_sum(acc);
   if (lane == 0) *out = acc;
 }
