@@ -14,6 +14,7 @@ LIB = PKG / "libmadsdp_hip.so"
 SOURCES = ["dense.hip", "gemm32.hip", "eigen.hip", "ops.hip", "qr.hip", "solver.hip", "server.hip",
            "profile.hip"]
 ARCH = os.environ.get("MSDP_OFFLOAD_ARCH", "gfx950")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          f"-I{INCLUDE}", f"-I{CSRC}"]
@@ -44,7 +45,8 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(one, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
+           f"-L{ROCM}/lib", "-lrocblas", f"-Wl,-rpath,{ROCM}/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
