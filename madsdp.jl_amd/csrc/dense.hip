@@ -344,9 +344,26 @@ static bool vendor_gemm_shape(int M, int N, int K, int batch, bool lower, const 
   return batch == 1 && !lower && !guard && M >= 512 && N >= 512 && K >= 32;
 }
 
+
+
+static int blas_handle(hipStream_t s);
+
 static int vendor_gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                        double* C, int64_t ldc) {
+  MSDP_TRY(blas_handle(s));
+  const rocblas_status st = rocblas_dgemm(
+      t_blas.h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+      tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A, (int)lda, B,
+      (int)ldb, &beta, C, (int)ldc);
+  if (st != rocblas_status_success) {
+    set_error("rocblas_dgemm failed");
+    return -1;
+  }
+  return 0;
+}
+
+static int blas_handle(hipStream_t s) {
   int dev = 0;
   MSDP_HIP(hipGetDevice(&dev));
   if (!t_blas.h || t_blas.device != dev) {
@@ -362,14 +379,6 @@ static int vendor_gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, dou
   }
   if (rocblas_set_stream(t_blas.h, s) != rocblas_status_success) {
     set_error("rocblas_set_stream failed");
-    return -1;
-  }
-  const rocblas_status st = rocblas_dgemm(
-      t_blas.h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-      tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A, (int)lda, B,
-      (int)ldb, &beta, C, (int)ldc);
-  if (st != rocblas_status_success) {
-    set_error("rocblas_dgemm failed");
     return -1;
   }
   return 0;
