@@ -66,8 +66,11 @@ int scalar_affine(hipStream_t st, int ns, const double* x, const double* dx, con
 int nt_rnt(hipStream_t st, int n, const double* M, const double* D, double* R);
 int sqrt_min(hipStream_t st, int n, const double* lam, double* D, double* minlam);
 int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, double* out);
+// H y = b from H = L L' (L and its transpose Lt, the 64 x 64 diagonal-block
+// inverses dinv); with di512 (the inverses of the full 512 x 512 diagonal
+// blocks, consecutive) those blocks are applied as products.
 int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
-          const double* dinv, double* b, double* tmp);
+          const double* dinv, double* b, double* tmp, const double* di512 = nullptr);
 int scale_cols_vec(hipStream_t st, int rows, int cols, double* A, int64_t lda, const double* d);
 int vec_min(hipStream_t st, int n, const double* v, double* out);
 int schur_onehot_identity(hipStream_t st, int m, const double* L, int64_t ldl, const double* R,

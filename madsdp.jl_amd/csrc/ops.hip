@@ -442,14 +442,59 @@ __global__ __launch_bounds__(256) void trsv_panel_kernel(int r0, int rows, int K
     target[r0 + rr] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
+// out[K0 + i] = sum_{j <= i} D[i + 512 j] in[K0 + j], i < 512: the inverse of a
+// full 512 x 512 diagonal block applied to its part of the vector.  64 rows
+// per workgroup, the 4 waves split the columns, partial sums added in order.
+__global__ __launch_bounds__(256) void trsv_blockinv_kernel(int K0, const double* D,
+                                                            const double* in, double* out) {
+  __shared__ double xs[TRSV_SB];
+  __shared__ double part[4][64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int j = t; j < TRSV_SB; j += 256) xs[j] = in[K0 + j];
+  __syncthreads();
+  const int i = blockIdx.x * 64 + lane;
+  double a0 = 0.0, a1 = 0.0;
+  const int j1 = min(i + 1, (w + 1) * 128);
+  int j = w * 128;
+  for (; j + 1 < j1; j += 2) {
+    a0 += D[i + (int64_t)j * TRSV_SB] * xs[j];
+    a1 += D[i + (int64_t)(j + 1) * TRSV_SB] * xs[j + 1];
+  }
+  if (j < j1) a0 += D[i + (int64_t)j * TRSV_SB] * xs[j];
+  part[w][lane] = a0 + a1;
+  __syncthreads();
+  if (w == 0) out[K0 + i] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+// out[K0 + i] = sum_{j >= i} D[j + 512 i] in[K0 + j]: the transposed inverse,
+// one wavefront per output (column i of D).
+__global__ __launch_bounds__(256) void trsv_blockinv_t_kernel(int K0, const double* D,
+                                                              const double* in, double* out) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const double* col = D + (int64_t)i * TRSV_SB;
+  double a0 = 0.0, a1 = 0.0;
+  int j = i + lane;
+  for (; j + 64 < TRSV_SB; j += 128) {
+    a0 += col[j] * in[K0 + j];
+    a1 += col[j + 64] * in[K0 + j + 64];
+  }
+  if (j < TRSV_SB) a0 += col[j] * in[K0 + j];
+  const double acc = wave_sum(a0 + a1);
+  if (lane == 0) out[K0 + i] = acc;
+}
+
 int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
-          const double* dinv, double* b, double* tmp) {
+          const double* dinv, double* b, double* tmp, const double* di512) {
   if (m <= 0) return 0;
   const int nsb = cdiv(m, TRSV_SB);
   // forward: L x = b (x -> tmp, b overwritten by the updates)
   for (int k = 0; k < nsb; ++k) {
     const int K0 = k * TRSV_SB, KB = std::min(TRSV_SB, m - K0);
-    trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, L, ld, dinv + (size_t)(K0 / 64) * 4096, b, tmp, 0);
+    if (di512 && KB == TRSV_SB)
+      trsv_blockinv_kernel<<<TRSV_SB / 64, 256, 0, st>>>(K0, di512 + (size_t)k * TRSV_SB * TRSV_SB, b, tmp);
+    else
+      trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, L, ld, dinv + (size_t)(K0 / 64) * 4096, b, tmp, 0);
     const int rest = m - K0 - KB;
     if (rest > 0)
       trsv_panel_kernel<<<cdiv(rest, 64), 256, 0, st>>>(K0 + KB, rest, K0, KB, L, ld, tmp, b);
@@ -457,7 +502,10 @@ int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
   // backward: L' z = x (z -> b, tmp overwritten by the updates)
   for (int k = nsb - 1; k >= 0; --k) {
     const int K0 = k * TRSV_SB, KB = std::min(TRSV_SB, m - K0);
-    trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, Lt, ld, dinv + (size_t)(K0 / 64) * 4096, tmp, b, 1);
+    if (di512 && KB == TRSV_SB)
+      trsv_blockinv_t_kernel<<<TRSV_SB / 4, 256, 0, st>>>(K0, di512 + (size_t)k * TRSV_SB * TRSV_SB, tmp, b);
+    else
+      trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, Lt, ld, dinv + (size_t)(K0 / 64) * 4096, tmp, b, 1);
     if (K0 > 0) trsv_panel_kernel<<<cdiv(K0, 64), 256, 0, st>>>(0, K0, K0, KB, Lt, ld, b, tmp);
   }
   MSDP_HIP(hipGetLastError());

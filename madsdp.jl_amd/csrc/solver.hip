@@ -96,6 +96,19 @@ struct msdp_solver {
          *Rdlin = nullptr, *zinv = nullptr, *corrlin = nullptr, *wm = nullptr, *ws1 = nullptr,
          *ws2 = nullptr, *Cs = nullptr;
   double *H = nullptr, *Hs = nullptr, *Ht = nullptr, *Hdinv = nullptr, *wm2 = nullptr;
+  // inverses of the factor's full 512 x 512 diagonal blocks (triangular
+  // solves apply them as products) and the workspace that forms them
+  double *Hdi = nullptr, *Hdiw = nullptr;
+  bool hdi_valid = false;
+  // Forms the block inverses once per factorisation; worth it only when one
+  // factorisation serves many solves (iterative refinement).
+  int form_block_inverses() {
+    if (!Hdi || hdi_valid) return 0;
+    for (int64_t k = 0; k < m / 512; ++k)
+      MSDP_TRY(trtri_lower(st, 512, H + k * 512 * (m + 1), m, Hdi + k * 512 * 512, 512, Hdiw));
+    hdi_valid = true;
+    return 0;
+  }
   int* info = nullptr;    // per-block factorisation infos (4 per block) + 8
   double* sc = nullptr;   // scalar slots
   double* red = nullptr;  // reduction scratch
@@ -446,6 +459,10 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&Hs, mm * mm));
   MSDP_TRY(alloc(&Ht, mm * mm));
   MSDP_TRY(alloc(&Hdinv, potrf_work_elems((int)mm)));
+  if (m >= 512) {
+    MSDP_TRY(alloc(&Hdi, (size_t)(m / 512) * 512 * 512));
+    MSDP_TRY(alloc(&Hdiw, trtri_work_elems(512)));
+  }
   MSDP_TRY(alloc(&wm2, mm));
   MSDP_TRY(alloc(&rh, mm));
   MSDP_TRY(alloc(&rr, mm));
@@ -811,6 +828,7 @@ int msdp_solver::factorize_schur(bool& ok) {
     MSDP_TRY(stream_sync(st));
     if (hinfo == 0) {
       MSDP_TRY(transpose(st, (int)m, (int)m, H, m, Ht, m));
+      hdi_valid = false;
       reg_used = reg > 0 ? reg : 0.0;
       h_approx = any_lowp;
       if (any_lowp) ++lowp_count;
@@ -1480,9 +1498,11 @@ int msdp_solver::apply_H(const double* v, double* out) {
 // if it stalls above 1e-9 relative, H is re-assembled in FP64.
 int msdp_solver::schur_solve(double* v) {
   if (!h_approx) return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+  MSDP_TRY(form_block_inverses());
+  const double* di = hdi_valid ? Hdi : nullptr;
   MSDP_HIP(hipMemcpyAsync(rh, v, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
   MSDP_TRY(dot(st, m, rh, rh, slot(15), red));
-  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2));
+  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2, di));
   double prev = INFINITY, hn = 0.0, rn = 0.0;
   for (int it = 0;; ++it) {
     MSDP_TRY(apply_H(v, rr));
@@ -1493,7 +1513,7 @@ int msdp_solver::schur_solve(double* v) {
     rn = std::sqrt(hsc[16]);
     if (!(rn > 1e-12 * hn) || it >= opt.refine_max_iter || !(rn < 0.5 * prev)) break;
     prev = rn;
-    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, rr, wm2));
+    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, rr, wm2, di));
     MSDP_TRY(axpby(st, m, 1.0, v, 1.0, rr, v));
     ++refine_steps;
   }

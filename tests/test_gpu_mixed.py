@@ -115,3 +115,12 @@ def test_mixed_maxcut_is_fp64(gpu):
 def test_precision_option_checked():
     with pytest.raises(ValueError):
         M.MadSDPSolver(instances.maxcut(5, 0.5, 1), precision="fp16")
+
+
+def test_mixed_dense_m1100(gpu):
+    """m = 1100 (at most n(n+1)/2 + n_s, so H has full rank): the refined
+    solves apply the inverses of the factor's full 512-row diagonal blocks."""
+    a, b = _pair(instances.dense_sdp(48, 1100, 12, 13))
+    print(f"fp64 {a.iter} it, mixed {b.iter} it, lowp {b.info().lowp_factorizations}, "
+          f"refine {b.info().refine_steps}, fallbacks {b.info().fp64_fallbacks}", flush=True)
+    _check(a, b)

@@ -205,3 +205,19 @@ def test_blocks_small(gpu):
     s, ref = solve_both(model)
     assert s.status == M.Status.SOLVE_SUCCEEDED
     assert_parity(s, ref)
+
+
+def test_maxcut_1100_pm1(gpu):
+    """m = 1100: the Schur solves apply the inverses of two full 512-row
+    diagonal blocks and finish the last 76 rows by substitution; n = 1100
+    takes the blocked tridiagonalisation and Lanczos step lengths."""
+    s, ref = solve_both(instances.maxcut(1100, 0.01, 12, "pm1"))
+    assert s.status == M.Status.SOLVE_SUCCEEDED
+    assert_parity(s, ref)
+
+
+def test_dense_m1100(gpu):
+    """Dense constraints, m = 1100 > 2 x 512, with scalars."""
+    s, ref = solve_both(instances.dense_sdp(24, 1100, 12, 13))
+    assert s.status == M.Status.SOLVE_SUCCEEDED
+    assert_parity(s, ref)
