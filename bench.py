@@ -58,9 +58,10 @@ def build_model(workload):
 def run_stream(args):
     """Serving stream (c4): Poisson arrivals of independent instances, half
     max-cut with n ~ U{20..600}, half dense SDPs with n ~ U{16..200},
-    m ~ U{50..3000}, n_s = n.  The same arrivals are served twice: by the plain
-    FP64 path (one solve at a time) and by the continuous-batching loop
-    (`--slots` concurrent solves, mixed precision).  Returns the JSON record."""
+    m ~ U{50..3000}, n_s = n.  The same arrivals are served by the plain FP64
+    path (one solve at a time) and by the continuous-batching loop (`--slots`
+    concurrent solves) in `--batched-precision` and in the other precision.
+    Returns the JSON record."""
     import numpy as np
     from madsdp_amd import MadSDPSolver, instances, serve
 
@@ -90,12 +91,17 @@ def run_stream(args):
     if args.stream_paths == "plain":
         plain, it_p, obj_p = one(1, "fp64")
         batched, it_b, obj_b = plain, it_p, obj_p
+        other = None
     elif args.stream_paths == "batched":
-        batched, it_b, obj_b = one(args.slots, "mixed")
+        batched, it_b, obj_b = one(args.slots, args.batched_precision)
         plain, it_p, obj_p = batched, it_b, obj_b
+        other = None
     else:
         plain, it_p, obj_p = one(1, "fp64")
-        batched, it_b, obj_b = one(args.slots, "mixed")
+        batched, it_b, obj_b = one(args.slots, args.batched_precision)
+        alt = "mixed" if args.batched_precision == "fp64" else "fp64"
+        other = one(args.slots, alt)[0]
+        other["precision"] = alt
     dit = [b - a for a, b in zip(it_p, it_b)]
     dobj = max(abs(a - b) / max(1.0, abs(a)) for a, b in zip(obj_p, obj_b))
     return {
@@ -118,7 +124,8 @@ def run_stream(args):
         "p50_latency_ms": 1e3 * batched["p50_latency_s"],
         "p99_latency_ms": 1e3 * batched["p99_latency_s"],
         "plain_fp64": plain,
-        "batched_mixed": batched,
+        "batched": dict(batched, precision=args.batched_precision),
+        "batched_other_precision": other,
         "speedup_vs_plain": batched["solves_per_s"] / plain["solves_per_s"],
         "max_extra_iterations": int(max(dit)),
         "max_rel_objective_diff": dobj,
@@ -176,6 +183,8 @@ def main():
     ap.add_argument("--slots", type=int, default=16, help="stream: concurrent solves")
     ap.add_argument("--mem-budget", type=int, default=128, help="stream: device GiB for admission")
     ap.add_argument("--stream-paths", default="both", choices=["both", "plain", "batched"])
+    ap.add_argument("--batched-precision", default="fp64", choices=["mixed", "fp64"],
+                    help="stream: precision of the batched path")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run an elementwise kernel over MIB MiB three times (the "
                          "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -195,7 +204,7 @@ def main():
         rec = run_stream(args)
         print(json.dumps(rec), flush=True)
         # every request must end with the same status on both paths
-        same = rec["plain_fp64"]["not_succeeded"] == rec["batched_mixed"]["not_succeeded"]
+        same = rec["plain_fp64"]["not_succeeded"] == rec["batched"]["not_succeeded"]
         return 0 if same else 1
 
     if args.workload == "blocks96":
