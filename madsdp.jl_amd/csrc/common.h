@@ -149,6 +149,9 @@ struct EigBatch {
 };
 constexpr int JACOBI_VB_MAX = 128;
 int jacobi_vbatched(hipStream_t s, const EigBatch* items_dev, int count, bool vectors);
+// eigenvalues ascending in w and the matching vectors in V (Vtmp: n x n
+// scratch), n <= 128, one workgroup per item
+int syev_vbatched(hipStream_t s, const EigBatch* items_dev, int count);
 // eigenvalues only (w ascending if non-null, else just *wmin), n <= 128
 int eigvals_vbatched(hipStream_t s, const EigBatch* items_dev, int count);
 // Batched Jacobi for many small matrices (n <= 64).

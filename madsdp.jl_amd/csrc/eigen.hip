@@ -929,6 +929,221 @@ __global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) 
   if (tid == 0 && it.wmin) *it.wmin = 0.5 * (lo_s + hi_s);
 }
 
+// --------------------------------------------------------------------------
+// Eigen-decomposition with vectors, n <= 128, one workgroup per matrix:
+// Householder tridiagonalisation in LDS (reflectors kept in the item's Vtmp,
+// tau_k on its diagonal), implicit QL with Wilkinson-type shifts on the
+// tridiagonal (one thread forms each sweep's rotations, every thread applies
+// them to its row of Z held in the same LDS), the back-transformation
+// V = H_0 ... H_{n-3} Z, and the columns sorted by eigenvalue.  Replaces the
+// cyclic Jacobi sweeps for the NT scaling of small blocks.
+// --------------------------------------------------------------------------
+__device__ void syev_small(const EigBatch& it) {
+  const int n = it.n;
+  __shared__ double As[EV_MAX][EV_MAX + 1];
+  __shared__ double v[EV_MAX], p[EV_MAX], w[EV_MAX], d[EV_MAX], e[EV_MAX];
+  __shared__ double cs[EV_MAX], sn[EV_MAX];
+  __shared__ double red[16];
+  __shared__ double s_tau, s_scale;
+  __shared__ int s_cnt, s_top, s_state;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  double* Hv = it.Vtmp;
+  for (int x = tid; x < n * n; x += NT) {
+    const int i = x % n, j = x / n;
+    As[i][j] = it.A[i + (int64_t)j * it.lda];
+  }
+  __syncthreads();
+  for (int k = 0; k < n - 2; ++k) {
+    double ss = 0.0;
+    for (int i = k + 2 + tid; i < n; i += NT) ss += As[i][k] * As[i][k];
+    ss = block_reduce<0>(ss, red);
+    if (tid == 0) {
+      const double alpha = As[k + 1][k];
+      double beta, t, sc;
+      if (ss == 0.0) {
+        beta = alpha; t = 0.0; sc = 0.0;
+      } else {
+        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+        t = (beta - alpha) / beta;
+        sc = 1.0 / (alpha - beta);
+      }
+      d[k] = As[k][k];
+      e[k] = beta;
+      s_tau = t;
+      s_scale = sc;
+      Hv[k + (int64_t)k * n] = t;
+    }
+    __syncthreads();
+    const double t = s_tau, sc = s_scale;
+    for (int i = k + 1 + tid; i < n; i += NT) {
+      const double vi = i == k + 1 ? 1.0 : As[i][k] * sc;
+      v[i] = vi;
+      Hv[i + (int64_t)k * n] = vi;
+    }
+    __syncthreads();
+    if (t == 0.0) continue;  // uniform: H_k = I
+    const int L = n - k - 1;
+    {
+      const int r = tid >> 2, q = tid & 3, i = k + 1 + r;
+      double acc = 0.0;
+      if (r < L)
+        for (int j = k + 1 + q; j < n; j += 4) acc += As[i][j] * v[j];
+      acc += __shfl_xor(acc, 1);
+      acc += __shfl_xor(acc, 2);
+      if (q == 0 && r < L) p[i] = t * acc;
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int i = k + 1 + tid; i < n; i += NT) pv += p[i] * v[i];
+    pv = block_reduce<0>(pv, red);
+    const double c2 = 0.5 * t * pv;
+    for (int i = k + 1 + tid; i < n; i += NT) w[i] = p[i] - c2 * v[i];
+    __syncthreads();
+    for (int x = tid; x < L * L; x += NT) {
+      const int i = k + 1 + x % L, j = k + 1 + x / L;
+      As[i][j] -= v[i] * w[j] + w[i] * v[j];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      d[n - 2] = As[n - 2][n - 2];
+      e[n - 2] = As[n - 1][n - 2];
+    }
+    d[n - 1] = As[n - 1][n - 1];
+    e[n - 1] = 0.0;
+  }
+  __syncthreads();
+  // Z = I in the same LDS
+  for (int x = tid; x < n * n; x += NT) {
+    const int i = x % n, j = x / n;
+    As[i][j] = i == j ? 1.0 : 0.0;
+  }
+  // implicit QL (e[i] couples d[i] and d[i+1]); thread 0 keeps l, iter
+  int l = 0, iter = 0;
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) {
+      int state = 0, cnt = 0, top = 0;
+      for (;;) {
+        if (l >= n) { state = 2; break; }
+        int m = l;
+        for (; m < n - 1; ++m) {
+          const double dd = fabs(d[m]) + fabs(d[m + 1]);
+          if (fabs(e[m]) <= DEPS * dd) break;
+        }
+        if (m == l) {  // d[l] converged
+          ++l;
+          iter = 0;
+          continue;
+        }
+        if (++iter > 60) { state = 3; break; }
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+        double s = 1.0, c = 1.0, pp = 0.0;
+        int i = m - 1;
+        bool split = false;
+        for (; i >= l; --i) {
+          const double f = s * e[i], b = c * e[i];
+          r = hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= pp;
+            e[m] = 0.0;
+            split = true;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - pp;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          pp = s * r;
+          d[i + 1] = g + pp;
+          g = c * r - b;
+          cs[cnt] = c;
+          sn[cnt] = s;
+          ++cnt;
+        }
+        top = m - 1;
+        if (!split) {
+          d[l] -= pp;
+          e[l] = g;
+          e[m] = 0.0;
+        }
+        break;  // apply this sweep's rotations
+      }
+      s_state = state;
+      s_cnt = cnt;
+      s_top = top;
+    }
+    __syncthreads();
+    const int cnt = s_cnt, top = s_top;
+    if (tid < n) {
+      double* zr = As[tid];
+      for (int q = 0; q < cnt; ++q) {
+        const int i = top - q;
+        const double c = cs[q], s = sn[q];
+        const double f = zr[i + 1];
+        zr[i + 1] = s * zr[i] + c * f;
+        zr[i] = c * zr[i] - s * f;
+      }
+    }
+    if (s_state != 0) break;
+  }
+  __syncthreads();
+  // V = H_0 ... H_{n-3} Z
+  for (int k = n - 3; k >= 0; --k) {
+    const double t = Hv[k + (int64_t)k * n];
+    if (t == 0.0) continue;  // uniform
+    for (int i = k + 1 + tid; i < n; i += NT) v[i] = Hv[i + (int64_t)k * n];
+    __syncthreads();
+    {
+      const int c = tid >> 2, q = tid & 3;
+      double acc = 0.0;
+      if (c < n)
+        for (int i = k + 1 + q; i < n; i += 4) acc += v[i] * As[i][c];
+      acc += __shfl_xor(acc, 1);
+      acc += __shfl_xor(acc, 2);
+      if (q == 0 && c < n) w[c] = t * acc;
+    }
+    __syncthreads();
+    const int L = n - k - 1;
+    for (int x = tid; x < L * n; x += NT) {
+      const int i = k + 1 + x % L, c = x / L;
+      As[i][c] -= v[i] * w[c];
+    }
+    __syncthreads();
+  }
+  // sort by eigenvalue (ties by index) and write
+  for (int i = tid; i < n; i += NT) {
+    const double di = d[i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += (d[j] < di) || (d[j] == di && j < i);
+    it.w[r] = di;
+    if (r == 0 && it.wmin) *it.wmin = di;
+    p[i] = (double)r;
+  }
+  __syncthreads();
+  for (int x = tid; x < n * n; x += NT) {
+    const int row = x % n, col = x / n;
+    it.V[row + (int64_t)((int)p[col]) * it.ldv] = As[row][col];
+  }
+}
+
+__global__ __launch_bounds__(512) void syev_vb_kernel(const EigBatch* items) {
+  syev_small(items[blockIdx.x]);
+}
+
+__global__ __launch_bounds__(512) void syev_one_kernel(EigBatch it) { syev_small(it); }
+
+int syev_vbatched(hipStream_t s, const EigBatch* items, int count) {
+  if (count <= 0) return 0;
+  syev_vb_kernel<<<count, 512, 0, s>>>(items);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
 int eigvals_vbatched(hipStream_t s, const EigBatch* items, int count) {
   if (count <= 0) return 0;
   eigvals_vb_kernel<<<count, 512, 0, s>>>(items);
@@ -1475,6 +1690,11 @@ size_t syevd_iwork_elems(int n) { return dc_iwork_elems(n); }
 int syevd(hipStream_t s, int n, const double* A, int64_t lda, double* w, double* V, int64_t ldv,
           bool vectors, double* work, int* iwork) {
   if (n <= 0) return 0;
+  if (n <= EV_MAX && V) {
+    syev_one_kernel<<<1, 512, 0, s>>>(EigBatch{n, A, lda, w, nullptr, V, ldv, work});
+    MSDP_HIP(hipGetLastError());
+    return 0;
+  }
   if (n <= 64)
     return jacobi_batched(s, n, A, lda, 0, w, 0, V, ldv, 0, 1, vectors);
   TrdWork t = trd_carve(n, work);

@@ -714,7 +714,7 @@ int msdp_solver::prepare(bool& ok) {
   MSDP_TRY(join());
   if (nt() && nsmall > 0) {
     // eigen-decompositions of all small blocks in one launch
-    MSDP_TRY(jacobi_vbatched(st, eb_nt, nsmall, true));
+    MSDP_TRY(syev_vbatched(st, eb_nt, nsmall));
     MSDP_TRY(fork());
     for (int i = 0; i < nb; ++i) {
       OnBlock on(this, i);
