@@ -469,140 +469,115 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, con
 // ==========================================================================
 // Cholesky
 // ==========================================================================
-// Factor one diagonal block (n <= 64) with 256 threads: thread (ti, tc)
-// owns the 4 x 4 register tile of rows ti + 16a and columns tc + 16b (the
-// dynamic tile index is resolved by uniform predicates over static slots,
-// which keeps the tile in registers).  Each
-// column step: the pivot owner takes the square root, the owners of column j
-// scale it and publish it in LDS, every thread updates its tile (16 FMAs).
-// With `dinv`, L^-1 is formed the same way by right-looking forward
-// substitution over rows (64 x 64, ld 64, zero padded).
+// --------------------------------------------------------------------------
+// One-wavefront 64 x 64 kernels.  Lane i keeps row i (factorisation) or
+// column i (inverse) of the block in 64 registers; the loops over the block
+// are unrolled so the register arrays are indexed statically, the column
+// being eliminated is published in LDS, and no workgroup barrier is needed
+// (a wavefront's LDS accesses complete in program order).  The dependent
+// chain per column step is one square root / reciprocal and one FMA.
+// --------------------------------------------------------------------------
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double readlane_f64(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+constexpr int W64_LD = 65;  // LDS leading dimension of a 64 x 64 block
+
+// Right-looking Cholesky of the n x n (n <= 64) block whose row `lane`
+// (lower part; zeros right of the diagonal and at rows >= n) is in r.
+// Column j: the pivot's square root, the column divided by it, then
+// r[c] -= L[lane][j] L[c][j] for c > j, entries updated in increasing j as
+// in LAPACK's right-looking potrf.  Column j of L is left in
+// Lc[j * W64_LD + i] (zeros above the diagonal and at rows >= n).  Returns
+// 0, or 1 + the first column whose pivot is not positive and finite.
+__device__ __attribute__((always_inline)) inline int chol64_rows(double (&r)[64], int n,
+                                                                 double* Lc) {
+  const int lane = threadIdx.x & 63;
+  int fail = 0;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    if (j < n && fail == 0) {  // uniform; no early exit, so the loop unrolls fully
+      const double d = readlane_f64(r[j], j);
+      if (!(d > 0.0) || !isfinite(d)) {
+        fail = j + 1;
+      } else {
+        const double ljj = sqrt(d);
+        const double v = lane == j ? ljj : (lane > j ? r[j] / ljj : 0.0);
+        r[j] = v;
+        Lc[j * W64_LD + lane] = v;
+        wave_lds_sync();
+#pragma unroll
+        for (int c = j + 1; c < 64; ++c) r[c] -= v * Lc[j * W64_LD + c];
+      }
+    }
+  }
+  return fail;
+}
+
+// Column `lane` of L^-1 for the n x n lower factor in Lc (column-major, ld
+// W64_LD), right-looking: x starts at e_lane; step k scales x_k by 1 / L_kk
+// and subtracts L_ik x_k from every x_i below it.
+__device__ __attribute__((always_inline)) inline void trinv64_cols(const double* Lc, int n,
+                                                                   double (&x)[64]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) x[i] = i == lane ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    if (k < n) {
+      const double rd = 1.0 / Lc[k * W64_LD + k];
+      x[k] *= rd;
+#pragma unroll
+      for (int i = k + 1; i < 64; ++i) x[i] -= Lc[k * W64_LD + i] * x[k];
+    }
+  }
+}
+
+// Factor one diagonal block (n <= 64) in place (lower triangle; the upper is
+// not touched) with one wavefront.  With `dinv`, also L^-1 (64 x 64,
+// column-major, ld 64, zero outside the n x n lower triangle).
 // info[b] = info_base + first failing column.
-__global__ __launch_bounds__(256) void chol_diag_kernel(int n, double* A, int64_t lda,
-                                                        int64_t stride, int* info, int info_base,
-                                                        int per_batch_info, double* dinv) {
+__global__ __launch_bounds__(64) void chol_diag_kernel(int n, double* A, int64_t lda,
+                                                       int64_t stride, int* info, int info_base,
+                                                       int per_batch_info, double* dinv) {
   int* inf = info + (per_batch_info ? blockIdx.x : 0);
   if (!per_batch_info && *inf != 0) return;
   A += (int64_t)blockIdx.x * stride;
-  __shared__ double colv[64];
-  __shared__ double Lc[64][65];   // the factor, for the inverse
-  __shared__ double rdg[64];
-  __shared__ int fail;
-  const int tid = threadIdx.x, ti = tid & 15, tc = tid >> 4;
-  double t[4][4];
+  __shared__ double Lc[64 * W64_LD];
+  const int lane = threadIdx.x;
+  double r[64];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int i = ti + 16 * a, c = tc + 16 * b;
-      t[a][b] = (i < n && c <= i) ? A[i + (int64_t)c * lda] : 0.0;
-    }
-  if (tid == 0) fail = 0;
-  __syncthreads();
-#pragma unroll 1
-  for (int j = 0; j < n; ++j) {
-    const int bj = j >> 4, cj = j & 15;
-    if (tc == cj && ti == cj) {
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-        if (a == bj) {
-          const double d = t[a][a];
-          if (!(d > 0.0) || !isfinite(d)) {
-            fail = j + 1;
-          } else {
-            t[a][a] = sqrt(d);
-            colv[j] = t[a][a];
-            rdg[j] = 1.0 / t[a][a];
-          }
-        }
-    }
-    __syncthreads();
-    if (fail) break;
-    if (tc == cj) {
-      const double ljj = colv[j];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int i = ti + 16 * a;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (b == bj && i > j && i < n) {
-            t[a][b] = t[a][b] / ljj;
-            colv[i] = t[a][b];
-          }
-      }
-    }
-    __syncthreads();
-    double li[4], lc[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      li[a] = colv[ti + 16 * a];
-      lc[a] = colv[tc + 16 * a];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int i = ti + 16 * a, c = tc + 16 * b;
-        if (c > j && i >= c && i < n) t[a][b] -= li[a] * lc[b];
-      }
-  }
+  for (int c = 0; c < 64; ++c)
+    r[c] = (lane < n && c <= lane) ? A[lane + (int64_t)c * lda] : 0.0;
+  const int fail = chol64_rows(r, n, Lc);
   if (fail) {
-    if (tid == 0) atomicCAS(inf, 0, info_base + fail);
+    if (lane == 0) atomicCAS(inf, 0, info_base + fail);
     return;
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int i = ti + 16 * a, c = tc + 16 * b;
-      if (i < n && c <= i) A[i + (int64_t)c * lda] = t[a][b];
-      Lc[i][c] = (i < n && c <= i) ? t[a][b] : 0.0;
-    }
+  for (int c = 0; c < 64; ++c)
+    if (lane < n && c <= lane) A[lane + (int64_t)c * lda] = r[c];
   if (!dinv) return;
-  __syncthreads();
-  // X = L^-1: row k of X is final after rows < k are subtracted; acc starts at I
-  double x[4][4];
+  wave_lds_sync();
+  double x[64];
+  trinv64_cols(Lc, n, x);
+  wave_lds_sync();
+  // transpose through LDS so that the stores of dinv are coalesced
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) x[a][b] = (ti + 16 * a == tc + 16 * b) ? 1.0 : 0.0;
-  __shared__ double xrow[64];
-#pragma unroll 1
-  for (int k = 0; k < n; ++k) {
-    const int bk = k >> 4;
-    if (ti == (k & 15)) {
-      const double r = rdg[k];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-        if (a == bk) {
-#pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            x[a][b] *= r;
-            xrow[tc + 16 * b] = x[a][b];
-          }
-        }
-    }
-    __syncthreads();
-    double lr[4], xr[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      lr[a] = Lc[ti + 16 * a][k];
-      xr[a] = xrow[tc + 16 * a];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        if (ti + 16 * a > k) x[a][b] -= lr[a] * xr[b];
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int i = ti + 16 * a, c = tc + 16 * b;
-      dinv[i + 64 * c] = (i < n && c < n && c <= i) ? x[a][b] : 0.0;
-    }
+  for (int i = 0; i < 64; ++i) Lc[i * W64_LD + lane] = x[i];
+  wave_lds_sync();
+#pragma unroll 8
+  for (int c = 0; c < 64; ++c)
+    dinv[lane + 64 * c] = (lane < n && c < n && c <= lane) ? Lc[lane * W64_LD + c] : 0.0;
 }
 
 // A21 <- A21 * L11^-T in place, with L11^-1 from chol_diag_kernel; each
@@ -649,7 +624,7 @@ int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work)
     const int jb = std::min(64, n - j0);
     double* A11 = A + j0 + (int64_t)j0 * lda;
     double* dinv = work + (size_t)(j0 / 64) * 4096;
-    chol_diag_kernel<<<1, 256, 0, s>>>(jb, A11, lda, 0, info, j0, 0, dinv);
+    chol_diag_kernel<<<1, 64, 0, s>>>(jb, A11, lda, 0, info, j0, 0, dinv);
     MSDP_HIP(hipGetLastError());
     const int rest = n - j0 - jb;
     if (rest > 0) {
@@ -668,7 +643,7 @@ int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t st
                         int* info) {
   if (n > 64) { set_error("potrf_small_batched: n > 64"); return -2; }
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * batch, s));
-  chol_diag_kernel<<<batch, 256, 0, s>>>(n, A, lda, stride, info, 0, 1, nullptr);
+  chol_diag_kernel<<<batch, 64, 0, s>>>(n, A, lda, stride, info, 0, 1, nullptr);
   MSDP_HIP(hipGetLastError());
   return 0;
 }
@@ -679,31 +654,38 @@ int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t st
 // ==========================================================================
 __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, int64_t ldl,
                                                         double* Li, int64_t ldi) {
+  // Column c of T^-1 by forward substitution in lane c, right-looking: x_k
+  // is final at step k (1 / T_kk for k = c, x_k / T_kk below), then
+  // x_i -= T_ik x_k for i > k.
   const int b0 = blockIdx.x * 64;
   const int nb = min(64, n - b0);
-  __shared__ double T[64][65];
-  __shared__ double X[64][65];
-  const int c = threadIdx.x;
-  for (int e = c; e < 64 * 64; e += 64) {
-    const int i = e & 63, j = e >> 6;
-    T[i][j] = (i < nb && j < nb && i >= j) ? L[(b0 + i) + (int64_t)(b0 + j) * ldl] : 0.0;
-  }
-  __syncthreads();
-  if (c < nb) {
-    // column c of T^-1 by forward substitution
-    for (int i = 0; i < c; ++i) X[i][c] = 0.0;
-    X[c][c] = 1.0 / T[c][c];
-    for (int i = c + 1; i < nb; ++i) {
-      double v = 0.0;
-      for (int k = c; k < i; ++k) v -= T[i][k] * X[k][c];
-      X[i][c] = v / T[i][i];
+  __shared__ double T[64 * W64_LD];
+  const int lane = threadIdx.x;
+#pragma unroll 8
+  for (int j = 0; j < 64; ++j)
+    T[j * W64_LD + lane] =
+        (lane < nb && j < nb && lane >= j) ? L[(b0 + lane) + (int64_t)(b0 + j) * ldl] : 0.0;
+  wave_lds_sync();
+  double x[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) x[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    if (k < nb) {
+      const double tkk = T[k * W64_LD + k];
+      x[k] = k == lane ? 1.0 / tkk : x[k] / tkk;
+#pragma unroll
+      for (int i = k + 1; i < 64; ++i) x[i] -= T[k * W64_LD + i] * x[k];
     }
   }
-  __syncthreads();
-  for (int e = c; e < nb * nb; e += 64) {
-    const int i = e % nb, j = e / nb;
-    Li[(b0 + i) + (int64_t)(b0 + j) * ldi] = X[i][j];
-  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < 64; ++i) T[i * W64_LD + lane] = x[i];
+  wave_lds_sync();
+  if (lane >= nb) return;
+#pragma unroll 8
+  for (int c = 0; c < 64; ++c)
+    if (c < nb) Li[(b0 + lane) + (int64_t)(b0 + c) * ldi] = T[lane * W64_LD + c];
 }
 
 __global__ void zero_upper_kernel(int n, double* A, int64_t lda) {
@@ -821,55 +803,36 @@ int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch) {
 }
 
 // _cholesky_factor! for n <= 64 in one launch of one wavefront: the
-// symmetrised copy, the factorisation (right-looking, the matrix in LDS, one
-// row per lane, so no workgroup barriers), the flush and the zeroed upper
-// triangle.  Same arithmetic per entry as the blocked path's diagonal kernel.
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
+// symmetrised copy, the factorisation (chol64_rows: one row per lane in
+// registers), the flush and the zeroed upper triangle.  Same arithmetic per
+// entry as the blocked path's diagonal kernel.
 __global__ __launch_bounds__(64) void chol_small_kernel(int n, const double* M, int64_t ldm,
                                                         double* L, int64_t ldl, int* info) {
-  __shared__ double As[64][65];
+  __shared__ double Lc[64 * W64_LD];
   const int lane = threadIdx.x;
-  for (int j = 0; j < n; ++j)
-    if (lane < n && lane >= j) {
-      const double a = M[lane + (int64_t)j * ldm], b = M[j + (int64_t)lane * ldm];
-      As[lane][j] = (a + b) / 2;
-    }
-  wave_lds_sync();
-  int fail = 0;
-  for (int k = 0; k < n; ++k) {
-    const double d = As[k][k];
-    if (!(d > 0.0) || !isfinite(d)) {  // the same value in every lane
-      fail = k + 1;
-      break;
-    }
-    const double lkk = sqrt(d);
-    wave_lds_sync();
-    if (lane == k) As[k][k] = lkk;
-    if (lane > k && lane < n) As[lane][k] = As[lane][k] / lkk;
-    wave_lds_sync();
-    if (lane > k && lane < n) {
-      const double lik = As[lane][k];
-      for (int j = k + 1; j <= lane; ++j) As[lane][j] -= lik * As[j][k];
-    }
-    wave_lds_sync();
-  }
+  double r[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c)
+    r[c] = (lane < n && c <= lane)
+               ? (M[lane + (int64_t)c * ldm] + M[c + (int64_t)lane * ldm]) / 2
+               : 0.0;
+  const int fail = chol64_rows(r, n, Lc);
   if (lane == 0) *info = fail;
   if (fail) return;
-  double dm = lane < n ? fabs(As[lane][lane]) : 0.0;
+  wave_lds_sync();
+  double dm = lane < n ? fabs(Lc[lane * W64_LD + lane]) : 0.0;
   dm = wave_max(dm);
   const double eps = 2.220446049250313e-16;
   const double tol = eps * eps * dm;
-  for (int j = 0; j < n; ++j) {
-    if (lane >= n) continue;
-    double v = 0.0;
-    if (lane == j) v = As[lane][j];
-    else if (lane > j) v = fabs(As[lane][j]) < tol ? 0.0 : As[lane][j];
-    L[lane + (int64_t)j * ldl] = v;
+  if (lane >= n) return;
+#pragma unroll
+  for (int c = 0; c < 64; ++c) {
+    if (c < n) {
+      double v = 0.0;
+      if (lane == c) v = r[c];
+      else if (lane > c) v = fabs(r[c]) < tol ? 0.0 : r[c];
+      L[lane + (int64_t)c * ldl] = v;
+    }
   }
 }
 
