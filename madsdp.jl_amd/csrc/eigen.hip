@@ -277,7 +277,7 @@ int jacobi_vbatched(hipStream_t s, const EigBatch* items, int count, bool vector
 // Householder tridiagonalisation (lower), A = Q T Q',  Q = H_0 ... H_{n-3}
 // ==========================================================================
 struct TrdWork {
-  double *A, *V, *tau, *d, *e, *w, *p0, *p1, *W, *hv, *part;
+  double *A, *V, *tau, *d, *e, *w, *p0, *p1, *W, *hv, *part, *ppart;
 };
 
 constexpr int TRD_NB = 32;
@@ -298,7 +298,8 @@ static TrdWork trd_carve(int n, double* work) {
   t.p1 = t.p0 + n;
   t.W = t.p1 + n;
   t.hv = t.W + (size_t)n * TRD_NB;  // hv[0:nb], hw[nb:2nb]
-  t.part = t.hv + 2 * TRD_NB;       // per-workgroup partial dots (<= n / 256 + 1)
+  t.part = t.hv + 2 * TRD_NB;       // per-workgroup partial squares (<= n / 256 + 1)
+  t.ppart = t.part + n / 256 + 8;   // per-workgroup partial p . v (<= (n + 2 TRD_NB) / 4 + 1)
   return t;
 }
 
@@ -307,150 +308,142 @@ static TrdWork trd_carve(int n, double* work) {
 // the fly by the panel's reflectors V and their W = tau (A v - ...) vectors,
 // and after the panel A -= V W' + W V' is applied by two MFMA GEMMs.
 //
-// Column step of panel column j (global column k = k0 + j; k < 0: only the
-// completion of panel column j-1), in two kernels:
-//  trd_rows_kernel (grid, one row per thread): the corrections that involve
-//    the panel columns t < j-1:
-//      pc[i]   = p[i] - sum_t V[i,t] hw[t] + W[i,t] hv[t]      (A v_{k-1} now)
-//      base[i] = A[i,k] - sum_t V[i,t] W[k,t] + W[i,t] V[k,t]  (column k)
-//    and partial sums of pc . v_{k-1}.
-//  trd_reflector_kernel (one workgroup, O(n)): w_{k-1} = tau pc - c v_{k-1}
-//    (column j-1 of W), the last correction of column k with it, then the
-//    Householder reflector of column k (V[:, k]), d_k, e_k, tau_k.
-__global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int k0,
-                                                       const double* A, const double* V,
-                                                       const double* W, const double* p,
-                                                       const double* hv, double* pc,
-                                                       double* base, double* part) {
-  __shared__ double rowW[TRD_NB], rowV[TRD_NB];
-  __shared__ double red[4];
-  const int prev = j - 1;
-  const int start = j > 0 ? k0 + j : k;
-  const double* hw = hv + TRD_NB;
-  const int ncor = j > 0 ? prev : 0;
-  if (k >= 0 && threadIdx.x < ncor) {
-    rowW[threadIdx.x] = W[k + (int64_t)threadIdx.x * n];
-    rowV[threadIdx.x] = V[k + (int64_t)(k0 + threadIdx.x) * n];
-  }
-  __syncthreads();
-  const int i = start + blockIdx.x * 256 + threadIdx.x;
-  double sd = 0.0;
-  if (i < n) {
-    double pcv = j > 0 ? p[i] : 0.0;
-    double bv = k >= 0 ? A[i + (int64_t)k * n] : 0.0;
-    int t = 0;
-    for (; t + 3 < ncor; t += 4) {  // loads of four panel columns issued together
-      double vt[4], wt[4];
+// Column step of panel column j (global column k = k0 + j), two launches:
+//  trd_rows_kernel (grid, one row per thread):
+//    c2   = tau_{k-1}^2 / 2 * (pc . v_{k-1}), with
+//           pc . v_{k-1} = p . v_{k-1} - 2 hv . hw          (hv = V' v, hw = W' v)
+//    pc_i = p_i - sum_{t<j-1} V[i,t] hw[t] + W[i,t] hv[t]  (= A v_{k-1} now)
+//    w_i  = tau_{k-1} pc_i - c2 v_{k-1,i}                  (column j-1 of W)
+//    col_i = A[i,k] - sum_{t<j} V[i,t] W[k,t] + W[i,t] V[k,t]   (column k now)
+//    and per-workgroup partial sums of col_i^2 (i >= k+2); d_k = col_k.
+//  trd_panel_product_kernel (grid, one wavefront per trailing column): every
+//    workgroup forms the Householder reflector of column k from the partial
+//    squares (tau_k, e_k; v = [1, col_{k+2:} / (alpha - beta)]), stores its
+//    rows of V[:, k], and computes p = A_trailing v, hv = V_panel' v,
+//    hw = W_panel' v and per-workgroup partial sums of p . v.
+// The identity for pc . v needs no grid-wide reduction of pc, so one launch
+// per column is saved against forming pc first.  Deterministic throughout.
+__device__ inline double trd_pc_row(double pcv, int i, int n, int k0, int ncor,
+                                    const double* V, const double* W, const double* hv,
+                                    const double* hw) {
+  int t = 0;
+  for (; t + 3 < ncor; t += 4) {  // loads of four panel columns issued together
+    double vt[4], wt[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        vt[u] = V[i + (int64_t)(k0 + t + u) * n];
-        wt[u] = W[i + (int64_t)(t + u) * n];
-      }
+    for (int u = 0; u < 4; ++u) {
+      vt[u] = V[i + (int64_t)(k0 + t + u) * n];
+      wt[u] = W[i + (int64_t)(t + u) * n];
+    }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        pcv -= vt[u] * hw[t + u] + wt[u] * hv[t + u];
-        if (k >= 0) bv -= vt[u] * rowW[t + u] + wt[u] * rowV[t + u];
-      }
-    }
-    for (; t < ncor; ++t) {
-      const double vt = V[i + (int64_t)(k0 + t) * n], wt = W[i + (int64_t)t * n];
-      pcv -= vt * hw[t] + wt * hv[t];
-      if (k >= 0) bv -= vt * rowW[t] + wt * rowV[t];
-    }
-    if (j > 0) {
-      pc[i] = pcv;
-      sd = pcv * V[i + (int64_t)(k0 + prev) * n];
-    }
-    if (k >= 0) base[i] = bv;
+    for (int u = 0; u < 4; ++u) pcv -= vt[u] * hw[t + u] + wt[u] * hv[t + u];
   }
-  sd = block_reduce<0>(sd, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = sd;
+  for (; t < ncor; ++t) pcv -= V[i + (int64_t)(k0 + t) * n] * hw[t] + W[i + (int64_t)t * n] * hv[t];
+  return pcv;
 }
 
-__global__ __launch_bounds__(1024) void trd_reflector_kernel(
-    int n, int k, int j, int k0, int nparts, const double* A, double* V, double* W,
-    const double* pc, const double* base, const double* part, double* tau, double* d,
-    double* e) {
-  constexpr int CACHE = 8;  // column entries kept in registers per thread (n <= 8192)
-  __shared__ double red[16];
-  __shared__ double s_scale, s_wk, s_vk, s_tp, s_c2;
-  const int tid = threadIdx.x, nt = blockDim.x;
+__global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int k0, int npp,
+                                                       const double* A, const double* V,
+                                                       double* W, const double* p,
+                                                       const double* hv, const double* ppart,
+                                                       double* tau, double* colk, double* part,
+                                                       double* d, double* e) {
+  __shared__ double rowW[TRD_NB], rowV[TRD_NB];
+  __shared__ double red[4];
+  __shared__ double s_c2, s_wk, s_vk, s_tp;
+  const int tid = threadIdx.x;
   const int prev = j - 1;
-  const int kp = k0 + prev;
-  const double* vp = j > 0 ? V + (int64_t)kp * n : nullptr;
-  double* wp = j > 0 ? W + (int64_t)prev * n : nullptr;
+  const int kp = k0 + prev;             // column of the previous reflector (j > 0)
+  const int ncor = j > 0 ? prev : 0;    // panel columns whose terms pc_i still lacks
+  const int start = k >= 0 ? k : k0 + j;
+  const double* hw = hv + TRD_NB;
+  // column k needs the terms of every earlier panel column, the last one (t =
+  // j-1) through w_{k-1} below
+  if (k >= 0 && tid < ncor) {
+    rowW[tid] = W[k + (int64_t)tid * n];
+    rowV[tid] = V[k + (int64_t)(k0 + tid) * n];
+  }
+  double sp = 0.0;
+  if (j > 0)
+    for (int q = tid; q < npp; q += 256) sp += ppart[q];
+  __syncthreads();
+  const int i = start + blockIdx.x * 256 + tid;
+  double pcv = 0.0, bv = 0.0, vi = 0.0;
+  if (i < n) {
+    bv = k >= 0 ? A[i + (int64_t)k * n] : 0.0;
+    if (j > 0) {
+      pcv = trd_pc_row(p[i], i, n, k0, ncor, V, W, hv, hw);
+      vi = V[i + (int64_t)kp * n];
+    }
+    if (k >= 0)
+      for (int t = 0; t < ncor; ++t)
+        bv -= V[i + (int64_t)(k0 + t) * n] * rowW[t] + W[i + (int64_t)t * n] * rowV[t];
+  }
   if (j > 0) {
-    double sd = 0.0;
-    for (int q = tid; q < nparts; q += nt) sd += part[q];
-    sd = block_reduce<0>(sd, red);
+    sp = block_reduce<0>(sp, red);
     if (tid == 0) {
+      double hh = 0.0;
+      for (int t = 0; t < ncor; ++t) hh += hv[t] * hw[t];
       const double tp = tau[kp];
-      const double c2 = 0.5 * tp * tp * sd;
+      const double c2 = 0.5 * tp * tp * (sp - 2.0 * hh);
       s_tp = tp;
       s_c2 = c2;
       if (k >= 0) {
-        s_wk = tp * pc[k] - c2 * vp[k];
-        s_vk = vp[k];
+        const double pck = trd_pc_row(p[k], k, n, k0, ncor, V, W, hv, hw);
+        const double vpk = V[k + (int64_t)kp * n];
+        s_wk = tp * pck - c2 * vpk;
+        s_vk = vpk;
       }
     }
     __syncthreads();
   }
-  if (k < 0) {  // only w_{k-1}: column j-1 of W
-    const double tp = s_tp, c2 = s_c2;
-    for (int i = tid; i < n; i += nt) wp[i] = i > kp ? tp * pc[i] - c2 * vp[i] : 0.0;
-    return;
-  }
-  // one pass: w_{k-1} (stored), column k corrected with it, its squared norm
-  double cache[CACHE];
   double ss = 0.0;
-  {
-    const double tp = j > 0 ? s_tp : 0.0, c2 = j > 0 ? s_c2 : 0.0;
-    const double wk = j > 0 ? s_wk : 0.0, vk = j > 0 ? s_vk : 0.0;
-    int c = 0;
-    for (int i = tid; i < n; i += nt, ++c) {
-      double col = base[i];
-      if (j > 0) {
-        const double vi = vp[i];
-        const double wi = i > kp ? tp * pc[i] - c2 * vi : 0.0;
-        wp[i] = wi;
-        col -= vi * wk + wi * vk;
-      }
-#pragma unroll
-      for (int q = 0; q < CACHE; ++q)
-        if (q == c) cache[q] = col;
-      if (i >= k + 2) ss += col * col;
-      if (i == k) d[k] = col;
-      if (k == n - 2 && i == n - 1) e[k] = col;
+  if (i < n) {
+    if (j > 0) {
+      const double wi = i > kp ? s_tp * pcv - s_c2 * vi : 0.0;
+      W[i + (int64_t)prev * n] = wi;
+      if (k >= 0) bv -= vi * s_wk + wi * s_vk;
+    }
+    if (k >= 0) {
+      colk[i] = bv;
+      if (i >= k + 2) ss = bv * bv;
+      if (i == k) d[k] = bv;
+      if (k == n - 2 && i == n - 1) e[k] = bv;
     }
   }
-  if (k == n - 2) {
-    if (tid == 0) {
+  if (k < 0) return;
+  if (k == n - 2) {  // the trailing 2 x 2 block: no reflector
+    if (blockIdx.x == 0 && tid == 0) {
       tau[k] = 0.0;
       d[n - 1] = A[(n - 1) + (int64_t)(n - 1) * n];
     }
-    for (int i = tid; i < n; i += nt) V[i + (int64_t)k * n] = 0.0;
     return;
   }
   ss = block_reduce<0>(ss, red);
-  // alpha = corrected column entry k+1, from the thread that holds it
-  __shared__ double s_alpha;
-  if ((k + 1) % nt == tid) {
-    const int c = (k + 1) / nt;
-    double col;
-    if (c < CACHE) {
-      col = cache[0];
-#pragma unroll
-      for (int q = 1; q < CACHE; ++q)
-        if (q == c) col = cache[q];
-    } else {
-      col = base[k + 1];
-      if (j > 0) col -= vp[k + 1] * s_wk + wp[k + 1] * s_vk;
-    }
-    s_alpha = col;
-  }
-  __syncthreads();
+  if (tid == 0) part[blockIdx.x] = ss;
+}
+
+// p[c] = A[base:, c]' v (c >= base) for the trailing matrix (A symmetric,
+// full storage), one wavefront per column; then hv = V_panel' v and
+// hw = W_panel' v for the j panel columns before k.  Deterministic.
+__global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, int j, int k0,
+                                                                int nparts, const double* A,
+                                                                double* V, const double* W,
+                                                                const double* colk,
+                                                                const double* part, double* tau,
+                                                                double* e, double* p,
+                                                                double* hv, double* ppart) {
+  __shared__ double red[4];
+  __shared__ double s_sc;
+  __shared__ double pvw[4];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = k + 1;
+  const int L = n - base;
+  // the reflector of column k (the same arithmetic in every workgroup)
+  double ss = 0.0;
+  for (int q = tid; q < nparts; q += 256) ss += part[q];
+  ss = block_reduce<0>(ss, red);
   if (tid == 0) {
-    const double alpha = s_alpha;
+    const double alpha = colk[base];
     double beta, t, sc;
     if (ss == 0.0) {
       beta = alpha; t = 0.0; sc = 0.0;
@@ -459,61 +452,17 @@ __global__ __launch_bounds__(1024) void trd_reflector_kernel(
       t = (beta - alpha) / beta;
       sc = 1.0 / (alpha - beta);
     }
-    s_scale = sc;
-    tau[k] = t;
-    e[k] = beta;
+    s_sc = sc;
+    if (blockIdx.x == 0) {
+      tau[k] = t;
+      e[k] = beta;
+    }
   }
   __syncthreads();
-  double* vk = V + (int64_t)k * n;
-  const double sc = s_scale;
-  int c = 0;
-  for (int i = tid; i < n; i += nt, ++c) {
-    double v;
-    if (i <= k) v = 0.0;
-    else if (i == k + 1) v = 1.0;
-    else {
-      double col;
-      if (c < CACHE) {
-        col = cache[0];
-#pragma unroll
-        for (int q = 1; q < CACHE; ++q)
-          if (q == c) col = cache[q];
-      } else {
-        col = base[i];
-        if (j > 0) col -= vp[i] * s_wk + wp[i] * s_vk;
-      }
-      v = col * sc;
-    }
-    vk[i] = v;
-  }
-}
-
-static int trd_column(hipStream_t s, int n, int k, int j, int k0, const TrdWork& t) {
-  const int start = j > 0 ? k0 + j : k;
-  const int nparts = std::max(1, cdiv(n - start, 256));
-  trd_rows_kernel<<<nparts, 256, 0, s>>>(n, k, j, k0, t.A, t.V, t.W, t.p0, t.hv, t.w, t.p1,
-                                          t.part);
-  trd_reflector_kernel<<<1, 1024, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.w, t.p1, t.part,
-                                          t.tau, t.d, t.e);
-  MSDP_HIP(hipGetLastError());
-  return 0;
-}
-
-// p[c] = A[base:, c]' v (c >= base) for the trailing matrix (A symmetric,
-// full storage), one wavefront per column; then hv = V_panel' v and
-// hw = W_panel' v for the j panel columns before k.  Deterministic.
-__global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, int j, int k0,
-                                                                const double* A,
-                                                                const double* V,
-                                                                const double* W, double* p,
-                                                                double* hv) {
-  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const int base = k + 1;
-  const int L = n - base;
-  const double* v = V + (int64_t)k * n;
-  const double* colp;
-  double* out;
+  const double sc = s_sc;
+  const int gw = blockIdx.x * 4 + wave;
+  const double* colp = nullptr;
+  double* out = nullptr;
   if (gw < L) {
     colp = A + (int64_t)(base + gw) * n;
     out = p + base + gw;
@@ -523,22 +472,34 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
   } else if (gw < L + 2 * j) {
     colp = W + (int64_t)(gw - L - j) * n;
     out = hv + TRD_NB + (gw - L - j);
-  } else {
-    return;
   }
-  // four independent partial sums keep four loads in flight per lane
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int i = base + lane;
-  for (; i + 192 < n; i += 256) {
-    a0 += colp[i] * v[i];
-    a1 += colp[i + 64] * v[i + 64];
-    a2 += colp[i + 128] * v[i + 128];
-    a3 += colp[i + 192] * v[i + 192];
+  double acc = 0.0;
+  if (colp) {
+    // four independent partial sums keep four loads in flight per lane
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = base + lane;
+    for (; i + 192 < n; i += 256) {
+      const double v0 = i == base ? 1.0 : colk[i] * sc;
+      a0 += colp[i] * v0;
+      a1 += colp[i + 64] * (colk[i + 64] * sc);
+      a2 += colp[i + 128] * (colk[i + 128] * sc);
+      a3 += colp[i + 192] * (colk[i + 192] * sc);
+    }
+    for (; i < n; i += 64) a0 += colp[i] * (i == base ? 1.0 : colk[i] * sc);
+    acc = (a0 + a1) + (a2 + a3);
+    acc = wave_sum(acc);
+    if (lane == 0) *out = acc;
   }
-  for (; i < n; i += 64) a0 += colp[i] * v[i];
-  double acc = (a0 + a1) + (a2 + a3);
-  acc = wave_sum(acc);
-  if (lane == 0) *out = acc;
+  double pv = 0.0;
+  if (gw < L) {
+    const int c = base + gw;
+    const double vc = c == base ? 1.0 : colk[c] * sc;
+    pv = acc * vc;
+    if (lane == 0) V[c + (int64_t)k * n] = vc;
+  }
+  if (lane == 0) pvw[wave] = pv;
+  __syncthreads();
+  if (tid == 0) ppart[blockIdx.x] = (pvw[0] + pvw[1]) + (pvw[2] + pvw[3]);
 }
 
 // --------------------------------------------------------------------------
@@ -709,22 +670,35 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
   }
   MSDP_TRY(copy_mat(s, n, n, A, lda, t.A, n));
   MSDP_TRY(fill(s, (int64_t)n * n, 0.0, t.V));
+  // column step: rows (w_{k-1}, column k) then the reflector and products
+  auto rows = [&](int k, int j, int k0, int npp) -> int {
+    const int start = k >= 0 ? k : k0 + j;
+    const int nparts = std::max(1, cdiv(n - start, 256));
+    trd_rows_kernel<<<nparts, 256, 0, s>>>(n, k, j, k0, npp, t.A, t.V, t.W, t.p0, t.hv, t.ppart,
+                                           t.tau, t.p1, t.part, t.d, t.e);
+    MSDP_HIP(hipGetLastError());
+    return nparts;
+  };
   const int last = n - 3;  // last column with a reflector
   for (int k0 = 0; k0 <= last; k0 += TRD_NB) {
     const int jb = std::min(TRD_NB, last - k0 + 1);
+    int npp = 0;
     for (int j = 0; j < jb; ++j) {
       const int k = k0 + j;
-      MSDP_TRY(trd_column(s, n, k, j, k0, t));
+      const int nparts = rows(k, j, k0, npp);
+      if (nparts < 0) return nparts;
       const int L = n - k - 1;
+      npp = cdiv(L + 2 * j, 4);
       const bool prof = prof_active(PROF_TRD, s);
       if (prof) prof_before(s);
-      trd_panel_product_kernel<<<cdiv(L + 2 * j, 4), 256, 0, s>>>(n, k, j, k0, t.A, t.V, t.W,
-                                                                   t.p0, t.hv);
+      trd_panel_product_kernel<<<npp, 256, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.p1,
+                                                   t.part, t.tau, t.e, t.p0, t.hv, t.ppart);
       // algorithmic bytes: the trailing L x L matrix read once
       if (prof) prof_after(s, 8.0 * L * (double)L);
+      MSDP_HIP(hipGetLastError());
     }
     // w of the panel's last column
-    MSDP_TRY(trd_column(s, n, -1, jb, k0, t));
+    MSDP_TRY(rows(-1, jb, k0, npp));
     const int r0 = k0 + jb;
     const int L2 = n - r0;
     if (L2 > 0) {
@@ -736,7 +710,7 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
     }
   }
   // the last 2 x 2 block: d_{n-2}, e_{n-2}, d_{n-1} (no reflector)
-  MSDP_TRY(trd_column(s, n, n - 2, 0, std::max(0, n - 2), t));
+  MSDP_TRY(rows(n - 2, 0, std::max(0, n - 2), 0));
   return 0;
 }
 
