@@ -497,56 +497,84 @@ constexpr int W64_LD = 65;  // LDS leading dimension of a 64 x 64 block
 // Column j: the pivot's square root, the column divided by it, then
 // r[c] -= L[lane][j] L[c][j] for c > j, entries updated in increasing j as
 // in LAPACK's right-looking potrf.  Column j of L is left in
-// Lc[j * W64_LD + i] (zeros above the diagonal and at rows >= n).  Returns
-// 0, or 1 + the first column whose pivot is not positive and finite.
-__device__ __attribute__((always_inline)) inline int chol64_rows(double (&r)[64], int n,
-                                                                 double* Lc) {
+// Lc[j * W64_LD + i] (zeros above the diagonal and at rows >= n).  The
+// column steps are template instances (compile-time j), so every register
+// index is static; the published column is read in chunks of 16 whose loads
+// are issued together (one LDS round trip per chunk).
+template <int J>
+__device__ __attribute__((always_inline)) inline void chol64_step(double (&r)[64], int n,
+                                                                  double* Lc, int& fail) {
   const int lane = threadIdx.x & 63;
-  int fail = 0;
+  if (J < n && fail == 0) {  // uniform
+    const double d = readlane_f64(r[J], J);
+    if (!(d > 0.0) || !isfinite(d)) {
+      fail = J + 1;
+    } else {
+      const double ljj = sqrt(d);
+      const double v = lane == J ? ljj : (lane > J ? r[J] / ljj : 0.0);
+      r[J] = v;
+      Lc[J * W64_LD + lane] = v;
+      wave_lds_sync();
 #pragma unroll
-  for (int j = 0; j < 64; ++j) {
-    if (j < n && fail == 0) {  // uniform; no early exit, so the loop unrolls fully
-      const double d = readlane_f64(r[j], j);
-      if (!(d > 0.0) || !isfinite(d)) {
-        fail = j + 1;
-      } else {
-        const double ljj = sqrt(d);
-        const double v = lane == j ? ljj : (lane > j ? r[j] / ljj : 0.0);
-        r[j] = v;
-        Lc[j * W64_LD + lane] = v;
-        wave_lds_sync();
+      for (int g = ((J + 1) / 16) * 16; g < 64; g += 16) {
+        double lc[16];
 #pragma unroll
-        for (int c = j + 1; c < 64; ++c) r[c] -= v * Lc[j * W64_LD + c];
+        for (int q = 0; q < 16; ++q)
+          if (g + q > J) lc[q] = Lc[J * W64_LD + g + q];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (g + q > J) r[g + q] -= v * lc[q];
       }
     }
   }
+  if constexpr (J + 1 < 64) chol64_step<J + 1>(r, n, Lc, fail);
+}
+
+__device__ __attribute__((always_inline)) inline int chol64_rows(double (&r)[64], int n,
+                                                                 double* Lc) {
+  int fail = 0;
+  chol64_step<0>(r, n, Lc, fail);
   return fail;
 }
 
 // Column `lane` of L^-1 for the n x n lower factor in Lc (column-major, ld
 // W64_LD), right-looking: x starts at e_lane; step k scales x_k by 1 / L_kk
 // and subtracts L_ik x_k from every x_i below it.
+template <int K>
+__device__ __attribute__((always_inline)) inline void trinv64_step(const double* Lc, int n,
+                                                                   double (&x)[64]) {
+  if (K < n) {
+    const double rd = 1.0 / Lc[K * W64_LD + K];
+    x[K] *= rd;
+#pragma unroll
+    for (int g = ((K + 1) / 16) * 16; g < 64; g += 16) {
+      double lc[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (g + q > K) lc[q] = Lc[K * W64_LD + g + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (g + q > K) x[g + q] -= lc[q] * x[K];
+    }
+  }
+  if constexpr (K + 1 < 64) trinv64_step<K + 1>(Lc, n, x);
+}
+
 __device__ __attribute__((always_inline)) inline void trinv64_cols(const double* Lc, int n,
                                                                    double (&x)[64]) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int i = 0; i < 64; ++i) x[i] = i == lane ? 1.0 : 0.0;
-#pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    if (k < n) {
-      const double rd = 1.0 / Lc[k * W64_LD + k];
-      x[k] *= rd;
-#pragma unroll
-      for (int i = k + 1; i < 64; ++i) x[i] -= Lc[k * W64_LD + i] * x[k];
-    }
-  }
+  trinv64_step<0>(Lc, n, x);
 }
 
 // Factor one diagonal block (n <= 64) in place (lower triangle; the upper is
 // not touched) with one wavefront.  With `dinv`, also L^-1 (64 x 64,
 // column-major, ld 64, zero outside the n x n lower triangle).
 // info[b] = info_base + first failing column.
-__global__ __launch_bounds__(64) void chol_diag_kernel(int n, double* A, int64_t lda,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void chol_diag_kernel(int n, double* A, int64_t lda,
                                                        int64_t stride, int* info, int info_base,
                                                        int per_batch_info, double* dinv) {
   int* inf = info + (per_batch_info ? blockIdx.x : 0);
@@ -652,6 +680,28 @@ int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t st
 // Triangular inverse (lower), recursive: diagonal 64-blocks inverted in LDS,
 // off-diagonal blocks X21 = -L22^-1 L21 L11^-1 by two GEMMs.
 // ==========================================================================
+template <int K>
+__device__ __attribute__((always_inline)) inline void trtri64_step(const double* T, int nb,
+                                                                   double (&x)[64]) {
+  const int lane = threadIdx.x & 63;
+  if (K < nb) {
+    const double tkk = T[K * W64_LD + K];
+    x[K] = K == lane ? 1.0 / tkk : x[K] / tkk;
+#pragma unroll
+    for (int g = ((K + 1) / 16) * 16; g < 64; g += 16) {
+      double lc[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (g + q > K) lc[q] = T[K * W64_LD + g + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (g + q > K) x[g + q] -= lc[q] * x[K];
+    }
+  }
+  if constexpr (K + 1 < 64) trtri64_step<K + 1>(T, nb, x);
+}
+
 __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, int64_t ldl,
                                                         double* Li, int64_t ldi) {
   // Column c of T^-1 by forward substitution in lane c, right-looking: x_k
@@ -661,23 +711,17 @@ __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, 
   const int nb = min(64, n - b0);
   __shared__ double T[64 * W64_LD];
   const int lane = threadIdx.x;
-#pragma unroll 8
-  for (int j = 0; j < 64; ++j)
-    T[j * W64_LD + lane] =
-        (lane < nb && j < nb && lane >= j) ? L[(b0 + lane) + (int64_t)(b0 + j) * ldl] : 0.0;
-  wave_lds_sync();
   double x[64];
+  // all 64 loads in flight before the first LDS store
+#pragma unroll
+  for (int j = 0; j < 64; ++j)
+    x[j] = (lane < nb && j < nb && lane >= j) ? L[(b0 + lane) + (int64_t)(b0 + j) * ldl] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) T[j * W64_LD + lane] = x[j];
+  wave_lds_sync();
 #pragma unroll
   for (int i = 0; i < 64; ++i) x[i] = 0.0;
-#pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    if (k < nb) {
-      const double tkk = T[k * W64_LD + k];
-      x[k] = k == lane ? 1.0 / tkk : x[k] / tkk;
-#pragma unroll
-      for (int i = k + 1; i < 64; ++i) x[i] -= T[k * W64_LD + i] * x[k];
-    }
-  }
+  trtri64_step<0>(T, nb, x);
   wave_lds_sync();
 #pragma unroll
   for (int i = 0; i < 64; ++i) T[i * W64_LD + lane] = x[i];

@@ -299,7 +299,7 @@ static TrdWork trd_carve(int n, double* work) {
   t.W = t.p1 + n;
   t.hv = t.W + (size_t)n * TRD_NB;  // hv[0:nb], hw[nb:2nb]
   t.part = t.hv + 2 * TRD_NB;       // per-workgroup partial squares (<= n / 256 + 1)
-  t.ppart = t.part + n / 256 + 8;   // per-workgroup partial p . v (<= (n + 2 TRD_NB) / 4 + 1)
+  t.ppart = t.part + n / 256 + 8;   // per-workgroup p_c v_c (<= n + 2 TRD_NB)
   return t;
 }
 
@@ -309,45 +309,28 @@ static TrdWork trd_carve(int n, double* work) {
 // and after the panel A -= V W' + W V' is applied by two MFMA GEMMs.
 //
 // Column step of panel column j (global column k = k0 + j), two launches:
-//  trd_rows_kernel (grid, one row per thread):
+//  trd_rows_kernel (grid, one row per thread; the row's panel entries are
+//    loaded together, the scalars of row k come from LDS):
 //    c2   = tau_{k-1}^2 / 2 * (pc . v_{k-1}), with
 //           pc . v_{k-1} = p . v_{k-1} - 2 hv . hw          (hv = V' v, hw = W' v)
 //    pc_i = p_i - sum_{t<j-1} V[i,t] hw[t] + W[i,t] hv[t]  (= A v_{k-1} now)
 //    w_i  = tau_{k-1} pc_i - c2 v_{k-1,i}                  (column j-1 of W)
 //    col_i = A[i,k] - sum_{t<j} V[i,t] W[k,t] + W[i,t] V[k,t]   (column k now)
 //    and per-workgroup partial sums of col_i^2 (i >= k+2); d_k = col_k.
-//  trd_panel_product_kernel (grid, one wavefront per trailing column): every
+//  trd_panel_product_kernel (grid, one workgroup per trailing column): every
 //    workgroup forms the Householder reflector of column k from the partial
 //    squares (tau_k, e_k; v = [1, col_{k+2:} / (alpha - beta)]), stores its
-//    rows of V[:, k], and computes p = A_trailing v, hv = V_panel' v,
-//    hw = W_panel' v and per-workgroup partial sums of p . v.
+//    entry of V[:, k], and computes its entry of p = A_trailing v (or of
+//    hv = V_panel' v, hw = W_panel' v) and the product p_c v_c.
 // The identity for pc . v needs no grid-wide reduction of pc, so one launch
 // per column is saved against forming pc first.  Deterministic throughout.
-__device__ inline double trd_pc_row(double pcv, int i, int n, int k0, int ncor,
-                                    const double* V, const double* W, const double* hv,
-                                    const double* hw) {
-  int t = 0;
-  for (; t + 3 < ncor; t += 4) {  // loads of four panel columns issued together
-    double vt[4], wt[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      vt[u] = V[i + (int64_t)(k0 + t + u) * n];
-      wt[u] = W[i + (int64_t)(t + u) * n];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) pcv -= vt[u] * hw[t + u] + wt[u] * hv[t + u];
-  }
-  for (; t < ncor; ++t) pcv -= V[i + (int64_t)(k0 + t) * n] * hw[t] + W[i + (int64_t)t * n] * hv[t];
-  return pcv;
-}
-
 __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int k0, int npp,
                                                        const double* A, const double* V,
                                                        double* W, const double* p,
                                                        const double* hv, const double* ppart,
                                                        double* tau, double* colk, double* part,
                                                        double* d, double* e) {
-  __shared__ double rowW[TRD_NB], rowV[TRD_NB];
+  __shared__ double rowW[TRD_NB], rowV[TRD_NB], hvs[TRD_NB], hws[TRD_NB];
   __shared__ double red[4];
   __shared__ double s_c2, s_wk, s_vk, s_tp;
   const int tid = threadIdx.x;
@@ -355,49 +338,84 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
   const int kp = k0 + prev;             // column of the previous reflector (j > 0)
   const int ncor = j > 0 ? prev : 0;    // panel columns whose terms pc_i still lacks
   const int start = k >= 0 ? k : k0 + j;
-  const double* hw = hv + TRD_NB;
-  // column k needs the terms of every earlier panel column, the last one (t =
-  // j-1) through w_{k-1} below
-  if (k >= 0 && tid < ncor) {
-    rowW[tid] = W[k + (int64_t)tid * n];
-    rowV[tid] = V[k + (int64_t)(k0 + tid) * n];
+  // every load of this launch is issued before the first wait
+  if (tid < ncor) {
+    hvs[tid] = hv[tid];
+    hws[tid] = hv[TRD_NB + tid];
+    if (k >= 0) {
+      rowW[tid] = W[k + (int64_t)tid * n];
+      rowV[tid] = V[k + (int64_t)(k0 + tid) * n];
+    }
   }
   double sp = 0.0;
-  if (j > 0)
-    for (int q = tid; q < npp; q += 256) sp += ppart[q];
-  __syncthreads();
+  if (j > 0) {  // four independent sums keep four loads in flight
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int q = tid;
+    for (; q + 768 < npp; q += 1024) {
+      s0 += ppart[q];
+      s1 += ppart[q + 256];
+      s2 += ppart[q + 512];
+      s3 += ppart[q + 768];
+    }
+    for (; q < npp; q += 256) s0 += ppart[q];
+    sp = (s0 + s1) + (s2 + s3);
+  }
   const int i = start + blockIdx.x * 256 + tid;
-  double pcv = 0.0, bv = 0.0, vi = 0.0;
-  if (i < n) {
-    bv = k >= 0 ? A[i + (int64_t)k * n] : 0.0;
+  const bool live = i < n;
+  double pi = 0.0, bv = 0.0, vi = 0.0;
+  double vr[TRD_NB], wr[TRD_NB];
+  if (live) {
+    if (k >= 0) bv = A[i + (int64_t)k * n];
     if (j > 0) {
-      pcv = trd_pc_row(p[i], i, n, k0, ncor, V, W, hv, hw);
+      pi = p[i];
       vi = V[i + (int64_t)kp * n];
     }
-    if (k >= 0)
-      for (int t = 0; t < ncor; ++t)
-        bv -= V[i + (int64_t)(k0 + t) * n] * rowW[t] + W[i + (int64_t)t * n] * rowV[t];
+#pragma unroll
+    for (int t = 0; t < TRD_NB; ++t) {
+      vr[t] = t < ncor ? V[i + (int64_t)(k0 + t) * n] : 0.0;
+      wr[t] = t < ncor ? W[i + (int64_t)t * n] : 0.0;
+    }
+  }
+  double pk = 0.0, vpk = 0.0, tp = 0.0;
+  if (tid == 0 && j > 0) {
+    tp = tau[kp];
+    if (k >= 0) {
+      pk = p[k];
+      vpk = V[k + (int64_t)kp * n];
+    }
+  }
+  __syncthreads();
+  double pcv = pi;
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < TRD_NB; ++t)
+      if (t < ncor) {
+        pcv -= vr[t] * hws[t] + wr[t] * hvs[t];
+        if (k >= 0) bv -= vr[t] * rowW[t] + wr[t] * rowV[t];
+      }
   }
   if (j > 0) {
     sp = block_reduce<0>(sp, red);
-    if (tid == 0) {
-      double hh = 0.0;
-      for (int t = 0; t < ncor; ++t) hh += hv[t] * hw[t];
-      const double tp = tau[kp];
-      const double c2 = 0.5 * tp * tp * (sp - 2.0 * hh);
-      s_tp = tp;
-      s_c2 = c2;
-      if (k >= 0) {
-        const double pck = trd_pc_row(p[k], k, n, k0, ncor, V, W, hv, hw);
-        const double vpk = V[k + (int64_t)kp * n];
-        s_wk = tp * pck - c2 * vpk;
-        s_vk = vpk;
+    if (tid < 64) {  // wave 0: the scalars of column k
+      const int t = tid;
+      double hh = t < ncor ? hvs[t] * hws[t] : 0.0;
+      double ck = (k >= 0 && t < ncor) ? rowV[t] * hws[t] + rowW[t] * hvs[t] : 0.0;
+      hh = wave_sum(hh);
+      ck = wave_sum(ck);
+      if (tid == 0) {
+        const double c2 = 0.5 * tp * tp * (sp - 2.0 * hh);
+        s_tp = tp;
+        s_c2 = c2;
+        if (k >= 0) {
+          s_wk = tp * (pk - ck) - c2 * vpk;
+          s_vk = vpk;
+        }
       }
     }
     __syncthreads();
   }
   double ss = 0.0;
-  if (i < n) {
+  if (live) {
     if (j > 0) {
       const double wi = i > kp ? s_tp * pcv - s_c2 * vi : 0.0;
       W[i + (int64_t)prev * n] = wi;
@@ -422,9 +440,13 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
   if (tid == 0) part[blockIdx.x] = ss;
 }
 
-// p[c] = A[base:, c]' v (c >= base) for the trailing matrix (A symmetric,
-// full storage), one wavefront per column; then hv = V_panel' v and
-// hw = W_panel' v for the j panel columns before k.  Deterministic.
+// One workgroup per output: p[c] = A[base:, c]' v for the trailing columns
+// c >= base (A symmetric, full storage; workgroups 0..L-1), then
+// hv = V_panel' v and hw = W_panel' v for the j panel columns before k.
+// Each thread keeps TRD_PR rows of the column and of v's numerators in
+// flight at once; the partial sums are combined in a fixed order.
+constexpr int TRD_PR = 12;
+
 __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, int j, int k0,
                                                                 int nparts, const double* A,
                                                                 double* V, const double* W,
@@ -434,10 +456,30 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
                                                                 double* hv, double* ppart) {
   __shared__ double red[4];
   __shared__ double s_sc;
-  __shared__ double pvw[4];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x;
   const int base = k + 1;
   const int L = n - base;
+  const int c = blockIdx.x;
+  const double* colp;
+  double* out;
+  if (c < L) {
+    colp = A + (int64_t)(base + c) * n;
+    out = p + base + c;
+  } else if (c < L + j) {
+    colp = V + (int64_t)(k0 + c - L) * n;
+    out = hv + (c - L);
+  } else {
+    colp = W + (int64_t)(c - L - j) * n;
+    out = hv + TRD_NB + (c - L - j);
+  }
+  // first batch of rows, issued before the reflector's reduction waits
+  double av[TRD_PR], cv[TRD_PR];
+#pragma unroll
+  for (int s = 0; s < TRD_PR; ++s) {
+    const int i = base + tid + 256 * s;
+    av[s] = i < n ? colp[i] : 0.0;
+    cv[s] = i < n ? colk[i] : 0.0;
+  }
   // the reflector of column k (the same arithmetic in every workgroup)
   double ss = 0.0;
   for (int q = tid; q < nparts; q += 256) ss += part[q];
@@ -453,53 +495,31 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
       sc = 1.0 / (alpha - beta);
     }
     s_sc = sc;
-    if (blockIdx.x == 0) {
+    if (c == 0) {
       tau[k] = t;
       e[k] = beta;
     }
   }
   __syncthreads();
   const double sc = s_sc;
-  const int gw = blockIdx.x * 4 + wave;
-  const double* colp = nullptr;
-  double* out = nullptr;
-  if (gw < L) {
-    colp = A + (int64_t)(base + gw) * n;
-    out = p + base + gw;
-  } else if (gw < L + j) {
-    colp = V + (int64_t)(k0 + gw - L) * n;
-    out = hv + (gw - L);
-  } else if (gw < L + 2 * j) {
-    colp = W + (int64_t)(gw - L - j) * n;
-    out = hv + TRD_NB + (gw - L - j);
-  }
   double acc = 0.0;
-  if (colp) {
-    // four independent partial sums keep four loads in flight per lane
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int i = base + lane;
-    for (; i + 192 < n; i += 256) {
-      const double v0 = i == base ? 1.0 : colk[i] * sc;
-      a0 += colp[i] * v0;
-      a1 += colp[i + 64] * (colk[i + 64] * sc);
-      a2 += colp[i + 128] * (colk[i + 128] * sc);
-      a3 += colp[i + 192] * (colk[i + 192] * sc);
+#pragma unroll
+  for (int s = 0; s < TRD_PR; ++s) {
+    const int i = base + tid + 256 * s;
+    acc += av[s] * (i == base ? 1.0 : cv[s] * sc);
+  }
+  for (int i = base + tid + 256 * TRD_PR; i < n; i += 256) acc += colp[i] * (colk[i] * sc);
+  acc = block_reduce<0>(acc, red);
+  if (tid == 0) {
+    *out = acc;
+    double pv = 0.0;
+    if (c < L) {
+      const double vc = c == 0 ? 1.0 : colk[base + c] * sc;
+      V[(base + c) + (int64_t)k * n] = vc;
+      pv = acc * vc;
     }
-    for (; i < n; i += 64) a0 += colp[i] * (i == base ? 1.0 : colk[i] * sc);
-    acc = (a0 + a1) + (a2 + a3);
-    acc = wave_sum(acc);
-    if (lane == 0) *out = acc;
+    ppart[c] = pv;
   }
-  double pv = 0.0;
-  if (gw < L) {
-    const int c = base + gw;
-    const double vc = c == base ? 1.0 : colk[c] * sc;
-    pv = acc * vc;
-    if (lane == 0) V[c + (int64_t)k * n] = vc;
-  }
-  if (lane == 0) pvw[wave] = pv;
-  __syncthreads();
-  if (tid == 0) ppart[blockIdx.x] = (pvw[0] + pvw[1]) + (pvw[2] + pvw[3]);
 }
 
 // --------------------------------------------------------------------------
@@ -688,7 +708,7 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
       const int nparts = rows(k, j, k0, npp);
       if (nparts < 0) return nparts;
       const int L = n - k - 1;
-      npp = cdiv(L + 2 * j, 4);
+      npp = L + 2 * j;
       const bool prof = prof_active(PROF_TRD, s);
       if (prof) prof_before(s);
       trd_panel_product_kernel<<<npp, 256, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.p1,
