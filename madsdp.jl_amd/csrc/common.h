@@ -54,6 +54,20 @@ __device__ inline double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
   return v;
 }
+// Orders a wavefront's LDS stores before its later LDS loads (no workgroup
+// barrier: the wavefront's LDS accesses complete in program order).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// The value of v in lane l (l uniform), through two v_readlane_b32.
+__device__ inline double readlane_f64(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 // Block-wide reduction for blockDim.x a multiple of 64 (<= 1024).
 template <int OP>  // 0 sum, 1 max, 2 min
 __device__ inline double block_reduce(double v, double* sh) {

@@ -431,7 +431,11 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     return 0;
   }
   if (prof) prof_before(s);
-  if (M >= BT && N >= BT) {
+  // 128 x 128 tiles only when they still fill the chip (lower: about half
+  // the tiles run); otherwise the 64 x 64 kernel's four times as many
+  // workgroups finish sooner
+  const int64_t tiles128 = (int64_t)cdiv(M, BT) * cdiv(N, BT) * batch;
+  if (M >= BT && N >= BT && (lower ? tiles128 / 2 : tiles128) >= 256) {
     dim3 g2(cdiv(M, BT), cdiv(N, BT), batch);
     if (!ta && !tb)
       gemm128_f64_kernel<false, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
@@ -477,19 +481,6 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, con
 // (a wavefront's LDS accesses complete in program order).  The dependent
 // chain per column step is one square root / reciprocal and one FMA.
 // --------------------------------------------------------------------------
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline double readlane_f64(double v, int l) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 constexpr int W64_LD = 65;  // LDS leading dimension of a 64 x 64 block
 
 // Right-looking Cholesky of the n x n (n <= 64) block whose row `lane`

@@ -283,7 +283,7 @@ struct TrdWork {
 constexpr int TRD_NB = 32;
 
 static size_t trd_work_elems(int n) {
-  return 2 * (size_t)n * n + 8 * (size_t)n + (size_t)n * TRD_NB + 2 * TRD_NB + 64;
+  return 2 * (size_t)n * n + 8 * (size_t)n + 3 * (size_t)n * TRD_NB + 2 * TRD_NB + 64;
 }
 
 static TrdWork trd_carve(int n, double* work) {
@@ -296,8 +296,11 @@ static TrdWork trd_carve(int n, double* work) {
   t.w = t.e + n;
   t.p0 = t.w + n;
   t.p1 = t.p0 + n;
-  t.W = t.p1 + n;
-  t.hv = t.W + (size_t)n * TRD_NB;  // hv[0:nb], hw[nb:2nb]
+  // W sits in the middle of an n x 3 TRD_NB buffer: the panel's V is copied
+  // into the columns just left and just right of W's, so that [V W] and
+  // [W V] are contiguous and A -= V W' + W V' is one GEMM of depth 2 jb
+  t.W = t.p1 + n + (size_t)n * TRD_NB;
+  t.hv = t.W + 2 * (size_t)n * TRD_NB;  // hv[0:nb], hw[nb:2nb]
   t.part = t.hv + 2 * TRD_NB;       // per-workgroup partial squares (<= n / 256 + 1)
   t.ppart = t.part + n / 256 + 8;   // per-workgroup p_c v_c (<= n + 2 TRD_NB)
   return t;
@@ -453,7 +456,8 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
                                                                 const double* colk,
                                                                 const double* part, double* tau,
                                                                 double* e, double* p,
-                                                                double* hv, double* ppart) {
+                                                                double* hv, double* ppart,
+                                                                double* vl, double* vr) {
   __shared__ double red[4];
   __shared__ double s_sc;
   const int tid = threadIdx.x;
@@ -516,6 +520,8 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
     if (c < L) {
       const double vc = c == 0 ? 1.0 : colk[base + c] * sc;
       V[(base + c) + (int64_t)k * n] = vc;
+      vl[base + c] = vc;  // the copies left and right of W (rows > k)
+      vr[base + c] = vc;
       pv = acc * vc;
     }
     ppart[c] = pv;
@@ -711,8 +717,11 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
       npp = L + 2 * j;
       const bool prof = prof_active(PROF_TRD, s);
       if (prof) prof_before(s);
+      double* vl = t.W - (int64_t)(jb - j) * n;  // [V_panel W_panel V_panel] around W
+      double* vr = t.W + (int64_t)(jb + j) * n;
       trd_panel_product_kernel<<<npp, 256, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.p1,
-                                                   t.part, t.tau, t.e, t.p0, t.hv, t.ppart);
+                                                   t.part, t.tau, t.e, t.p0, t.hv, t.ppart, vl,
+                                                   vr);
       // algorithmic bytes: the trailing L x L matrix read once
       if (prof) prof_after(s, 8.0 * L * (double)L);
       MSDP_HIP(hipGetLastError());
@@ -721,12 +730,11 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
     MSDP_TRY(rows(-1, jb, k0, npp));
     const int r0 = k0 + jb;
     const int L2 = n - r0;
-    if (L2 > 0) {
-      const double* Vp = t.V + r0 + (int64_t)k0 * n;
-      const double* Wp = t.W + r0;
+    if (L2 > 0) {  // A22 -= [V W] [W V]'
+      const double* VW = t.W - (int64_t)jb * n + r0;
+      const double* WV = t.W + r0;
       double* A22 = t.A + r0 + (int64_t)r0 * n;
-      MSDP_TRY(gemm(s, false, true, L2, L2, jb, -1.0, Vp, n, Wp, n, 1.0, A22, n));
-      MSDP_TRY(gemm(s, false, true, L2, L2, jb, -1.0, Wp, n, Vp, n, 1.0, A22, n));
+      MSDP_TRY(gemm(s, false, true, L2, L2, 2 * jb, -1.0, VW, n, WV, n, 1.0, A22, n));
     }
   }
   // the last 2 x 2 block: d_{n-2}, e_{n-2}, d_{n-1} (no reflector)
@@ -1620,28 +1628,51 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
 // ==========================================================================
 // T (nb x nb, upper) of a panel of reflectors from their Gram matrix
 // G = V'V: T_ii = tau_i, T[0:i, i] = -tau_i T[0:i, 0:i] G[0:i, i].
+// One wavefront; lane r keeps row r of T in registers (rows are independent:
+// T[r, i] = -tau_i sum_{k<i} T[r, k] G[k, i]), the columns of G are read
+// from LDS in batches, and the column index is a template parameter so every
+// register index is static.
+template <int I>
+__device__ __attribute__((always_inline)) inline void larft_step(int nb, const double* Gs,
+                                                                 double taur, double (&tr)[64]) {
+  const int r = threadIdx.x & 63;
+  if (I < nb) {
+    const double ti = readlane_f64(taur, I);
+    double v = 0.0;
+#pragma unroll
+    for (int g = 0; g < I; g += 16) {
+      double gc[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (g + q < I) gc[q] = Gs[I * 65 + g + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (g + q < I) v += tr[g + q] * gc[q];
+    }
+    tr[I] = r < I ? -ti * v : (r == I ? ti : 0.0);
+  }
+  if constexpr (I + 1 < 64) larft_step<I + 1>(nb, Gs, taur, tr);
+}
+
 __global__ __launch_bounds__(64) void larft_kernel(int nb, const double* G, const double* tau,
                                                    double* T) {
-  __shared__ double Ts[64][65];
-  __shared__ double Gs[64][65];
-  const int t = threadIdx.x;
-  for (int e = t; e < 64 * 64; e += 64) {
-    Ts[e & 63][e >> 6] = 0.0;
-    Gs[e & 63][e >> 6] = (e & 63) < nb && (e >> 6) < nb ? G[(e & 63) + (int64_t)(e >> 6) * nb] : 0.0;
-  }
-  __syncthreads();
-  for (int i = 0; i < nb; ++i) {
-    const double ti = tau[i];
-    // column i: x = G[0:i, i]; T[0:i, i] = -ti * T[0:i,0:i] x
-    if (t < i) {
-      double v = 0.0;
-      for (int k = t; k < i; ++k) v += Ts[t][k] * Gs[k][i];
-      Ts[t][i] = -ti * v;
-    }
-    if (t == 0) Ts[i][i] = ti;
-    __syncthreads();
-  }
-  for (int e = t; e < nb * nb; e += 64) T[(e % nb) + (int64_t)(e / nb) * nb] = Ts[e % nb][e / nb];
+  __shared__ double Gs[64 * 65];  // column i of G at Gs[i * 65 + k]
+  const int r = threadIdx.x;
+  double tr[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) tr[i] = (r < nb && i < nb) ? G[r + (int64_t)i * nb] : 0.0;
+  const double taur = r < nb ? tau[r] : 0.0;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) Gs[i * 65 + r] = tr[i];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < 64; ++i) tr[i] = 0.0;
+  larft_step<0>(nb, Gs, taur, tr);
+  if (r >= nb) return;
+#pragma unroll
+  for (int i = 0; i < 64; ++i)
+    if (i < nb) T[r + (int64_t)i * nb] = tr[i];
 }
 
 static int ormtr(hipStream_t s, int n, const double* V, const double* tau, double* Z, int64_t ldz,
