@@ -207,6 +207,11 @@ int msdp_server_destroy(msdp_server* v);
 int msdp_k_gemm(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                 double beta, double* C, int64_t ldc);
+/* lower triangle of C = alpha op(A)' op(A) + beta C, upper triangle untouched
+ * (BLAS dsyrk; the Schur assembly H = B'B behind LRO.schur_complement!,
+ * src/solver.jl:304); trans = 1: C = A' A with A K x N; trans = 0: C = A A' with A N x K */
+int msdp_k_syrk_lower(void* stream, int32_t trans, int32_t N, int32_t K, double alpha,
+                      const double* A, int64_t lda, double beta, double* C, int64_t ldc);
 /* lower Cholesky in place; *info_dev = 0 or first failing pivot (LAPACK potrf) */
 int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info_dev,
                  double* work);

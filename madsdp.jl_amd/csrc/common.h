@@ -68,6 +68,31 @@ __device__ inline double readlane_f64(double v, int l) {
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+// Lower-triangle tiles of a square nt x nt tile grid on a 1-D launch of
+// lower_tile_blocks(nt) workgroups.  Consecutive workgroups go to different
+// XCDs (blockIdx % 8), so each XCD is given a contiguous run of tiles in
+// column-major lower order: the runs share their operand panels in that
+// XCD's L2 instead of every XCD streaming all of them.  Returns false for
+// the padding workgroups.
+__host__ __device__ inline int lower_tile_blocks(int nt) {
+  const int T = nt * (nt + 1) / 2;
+  return 8 * ((T + 7) / 8);
+}
+__device__ inline bool lower_tile_xcd(int nt, int& tm, int& tn) {
+  const int T = nt * (nt + 1) / 2;
+  const int per = (T + 7) / 8;
+  const int w = blockIdx.x;
+  const int t = (w & 7) * per + (w >> 3);
+  if (t >= T) return false;
+  int c = 0, rem = t;
+  while (rem >= nt - c) {
+    rem -= nt - c;
+    ++c;
+  }
+  tn = c;
+  tm = c + rem;
+  return true;
+}
 // Block-wide reduction for blockDim.x a multiple of 64 (<= 1024).
 template <int OP>  // 0 sum, 1 max, 2 min
 __device__ inline double block_reduce(double v, double* sh) {
@@ -191,13 +216,18 @@ int svec_pack32(hipStream_t s, int n, int cb, const float* P, float* B, int64_t 
 namespace msdp {
 // --------------------------------------------------------------------------
 // Live kernel timing: when enabled for a kernel class on a stream, every
-// launch of that class is bracketed by HIP events on that stream and its
-// algorithmic work (flops or bytes) is accumulated.
+// launch of that class is timed with HIP events on that stream (recorded
+// around the launch, or stamped on the dispatch by hipExtLaunchKernel) and
+// its algorithmic work (flops or bytes) is accumulated.
 // --------------------------------------------------------------------------
 enum ProfClass { PROF_GEMM = 0, PROF_TRD = 1, PROF_GEMM32 = 2, PROF_NONE = -1 };
 bool prof_active(int cls, hipStream_t s);
 void prof_before(hipStream_t s);
 void prof_after(hipStream_t s, double work);
+// For a launch through hipExtLaunchKernel: a start/stop event pair that the
+// runtime stamps on the dispatch itself (no marker packets around it), with
+// the launch's algorithmic work.
+void prof_dispatch_events(hipEvent_t* start, hipEvent_t* stop, double work);
 int prof_begin(int cls, hipStream_t s);
 int prof_end(double* total_ms, int64_t* launches, double* work);
 }  // namespace msdp

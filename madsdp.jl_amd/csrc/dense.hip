@@ -156,8 +156,12 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
     int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
   if (guard && *guard != 0) return;
-  const int tm = blockIdx.x, tn = blockIdx.y;
-  if (lower && tn > tm) return;
+  int tm = blockIdx.x, tn = blockIdx.y;
+  if (lower == 2) {  // square lower triangle on a 1-D, XCD-grouped grid
+    if (!lower_tile_xcd((M + BT - 1) / BT, tm, tn)) return;
+  } else if (lower && tn > tm) {
+    return;
+  }
   const int64_t bz = blockIdx.z;
   A += bz * sA;
   B += bz * sB;
@@ -432,19 +436,25 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   }
   if (prof) prof_before(s);
   // 128 x 128 tiles only when they still fill the chip (lower: about half
-  // the tiles run); otherwise the 64 x 64 kernel's four times as many
-  // workgroups finish sooner
+  // the tiles run), or at least a third of it for deep products; otherwise
+  // the 64 x 64 kernel's four times as many workgroups finish sooner
   const int64_t tiles128 = (int64_t)cdiv(M, BT) * cdiv(N, BT) * batch;
-  if (M >= BT && N >= BT && (lower ? tiles128 / 2 : tiles128) >= 256) {
+  const int64_t run128 = lower ? tiles128 / 2 : tiles128;
+  if (M >= BT && N >= BT && (run128 >= 256 || (K >= 1024 && run128 >= 96))) {
     dim3 g2(cdiv(M, BT), cdiv(N, BT), batch);
+    int lmode = lower ? 1 : 0;
+    if (lower && batch == 1 && M == N) {
+      g2 = dim3(lower_tile_blocks(cdiv(M, BT)), 1, 1);
+      lmode = 2;
+    }
     if (!ta && !tb)
-      gemm128_f64_kernel<false, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+      gemm128_f64_kernel<false, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
     else if (ta && !tb)
-      gemm128_f64_kernel<true, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+      gemm128_f64_kernel<true, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
     else if (!ta && tb)
-      gemm128_f64_kernel<false, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+      gemm128_f64_kernel<false, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
     else
-      gemm128_f64_kernel<true, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+      gemm128_f64_kernel<true, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
     if (prof) prof_after(s, 2.0 * M * N * (double)K * batch * (lower ? 0.5 : 1.0));
     MSDP_HIP(hipGetLastError());
     return 0;

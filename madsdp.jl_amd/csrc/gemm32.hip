@@ -23,8 +23,12 @@ __global__ __launch_bounds__(256, 2) void gemm128_f32_kernel(
     int M, int N, int K, float alpha, const float* __restrict__ A, int64_t lda, int64_t sA,
     const float* __restrict__ B, int64_t ldb, int64_t sB, double beta, OutT* __restrict__ C,
     int64_t ldc, int64_t sC, int lower) {
-  const int tm = blockIdx.x, tn = blockIdx.y;
-  if (lower && tn > tm) return;
+  int tm = blockIdx.x, tn = blockIdx.y;
+  if (lower == 2) {  // square lower triangle on a 1-D, XCD-grouped grid
+    if (!lower_tile_xcd((M + 127) / 128, tm, tn)) return;
+  } else if (lower && tn > tm) {
+    return;
+  }
   const int64_t bz = blockIdx.z;
   A += bz * sA;
   B += bz * sB;
@@ -117,14 +121,19 @@ static int gemm32_t(hipStream_t s, bool ta, bool tb, int M, int N, int K, float 
                     bool lower) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
   dim3 g(cdiv(M, 128), cdiv(N, 128), batch);
+  int lmode = lower ? 1 : 0;
+  if (lower && batch == 1 && M == N) {
+    g = dim3(lower_tile_blocks(cdiv(M, 128)), 1, 1);
+    lmode = 2;
+  }
   if (!ta && !tb)
-    gemm128_f32_kernel<false, false, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower);
+    gemm128_f32_kernel<false, false, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode);
   else if (ta && !tb)
-    gemm128_f32_kernel<true, false, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower);
+    gemm128_f32_kernel<true, false, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode);
   else if (!ta && tb)
-    gemm128_f32_kernel<false, true, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower);
+    gemm128_f32_kernel<false, true, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode);
   else
-    gemm128_f32_kernel<true, true, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower);
+    gemm128_f32_kernel<true, true, OutT><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode);
   MSDP_HIP(hipGetLastError());
   return 0;
 }

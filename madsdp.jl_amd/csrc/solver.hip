@@ -2011,6 +2011,12 @@ int msdp_k_gemm(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int3
                 int64_t ldc) {
   return gemm((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
+int msdp_k_syrk_lower(void* stream, int32_t trans, int32_t N, int32_t K, double alpha,
+                      const double* A, int64_t lda, double beta, double* C, int64_t ldc) {
+  // C = alpha X X' + beta C with X = op(A)' (N x K): trans -> X = A' (A is K x N)
+  return gemm((hipStream_t)stream, trans != 0, trans == 0, N, N, K, alpha, A, lda, A, lda, beta, C,
+              ldc, 1, 0, 0, 0, true);
+}
 int msdp_k_potrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* info, double* work) {
   return potrf((hipStream_t)stream, n, A, lda, info, work);
 }

@@ -103,6 +103,8 @@ SIGNATURES = {
     "msdp_server_destroy": (C.c_int, [P]),
     "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
+    "msdp_k_syrk_lower": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_double, P, C.c_int64,
+                                    C.c_double, P, C.c_int64]),
     "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),
     "msdp_k_potrf_work": (C.c_size_t, [C.c_int32]),
     "msdp_k_cholesky_factor": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, P]),

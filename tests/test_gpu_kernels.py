@@ -54,6 +54,29 @@ def test_gemm(env, ta, tb, M, N, K):
     np.testing.assert_allclose(host(dC, (M, N)), ref, rtol=0, atol=1e-12 * max(1, K))
 
 
+@pytest.mark.parametrize("trans,N,K", [(1, 300, 700), (0, 257, 64), (1, 2100, 1100),
+                                       (0, 2304, 1030)])
+def test_syrk_lower(env, trans, N, K):
+    """Lower triangle of A'A (trans) or AA'; the upper triangle is untouched.
+    The two large cases take the block-row path (off-diagonal rectangles as
+    plain products, diagonal blocks batched)."""
+    torch, lib, st = env
+    rng = np.random.default_rng(N + K)
+    A = rng.standard_normal((K, N) if trans else (N, K))
+    Cm = rng.standard_normal((N, N))
+    dA, dC = dev(torch, A), dev(torch, Cm)
+    print(f"syrk trans={trans} N={N} K={K}", flush=True)
+    assert lib.msdp_k_syrk_lower(st, trans, N, K, 0.75, P(dA), A.shape[0], 2.0, P(dC), N) == 0
+    torch.cuda.synchronize()
+    X = A.T if trans else A
+    ref = 0.75 * X @ X.T + 2.0 * Cm
+    got = host(dC, (N, N))
+    low = np.tril_indices(N)
+    np.testing.assert_allclose(got[low], ref[low], rtol=0, atol=1e-12 * K)
+    up = np.triu_indices(N, 1)
+    assert np.array_equal(got[up], Cm[up])
+
+
 @pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 300])
 def test_cholesky_factor(env, n):
     torch, lib, st = env
