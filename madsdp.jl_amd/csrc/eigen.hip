@@ -15,6 +15,8 @@
 //  lambda_min : tridiagonalisation + Sturm-sequence multisection.
 #include "common.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cfloat>
 #include <vector>
@@ -715,15 +717,22 @@ static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWo
       if (nparts < 0) return nparts;
       const int L = n - k - 1;
       npp = L + 2 * j;
-      const bool prof = prof_active(PROF_TRD, s);
-      if (prof) prof_before(s);
       double* vl = t.W - (int64_t)(jb - j) * n;  // [V_panel W_panel V_panel] around W
       double* vr = t.W + (int64_t)(jb + j) * n;
-      trd_panel_product_kernel<<<npp, 256, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.p1,
-                                                   t.part, t.tau, t.e, t.p0, t.hv, t.ppart, vl,
-                                                   vr);
-      // algorithmic bytes: the trailing L x L matrix read once
-      if (prof) prof_after(s, 8.0 * L * (double)L);
+      if (prof_active(PROF_TRD, s)) {
+        // timed on the dispatch itself; algorithmic bytes: the trailing
+        // L x L matrix read once
+        hipEvent_t e0, e1;
+        prof_dispatch_events(&e0, &e1, 8.0 * L * (double)L);
+        hipExtLaunchKernelGGL(trd_panel_product_kernel, dim3(npp), dim3(256), 0, s, e0, e1, 0, n, k,
+                              j, k0, nparts, (const double*)t.A, t.V, (const double*)t.W,
+                              (const double*)t.p1, (const double*)t.part, t.tau, t.e, t.p0, t.hv,
+                              t.ppart, vl, vr);
+      } else {
+        trd_panel_product_kernel<<<npp, 256, 0, s>>>(n, k, j, k0, nparts, t.A, t.V, t.W, t.p1,
+                                                     t.part, t.tau, t.e, t.p0, t.hv, t.ppart, vl,
+                                                     vr);
+      }
       MSDP_HIP(hipGetLastError());
     }
     // w of the panel's last column

@@ -1,4 +1,5 @@
-// Per-launch HIP-event timing of one kernel class (see common.h).
+// Per-launch HIP-event timing of one kernel class (see common.h): events
+// recorded around the launch, or stamped on the dispatch by hipExtLaunchKernel.
 #include "common.h"
 
 #include <vector>
@@ -35,6 +36,12 @@ void prof_before(hipStream_t s) {
 void prof_after(hipStream_t s, double work) {
   hipEvent_t e = next_event();
   if (e) (void)hipEventRecord(e, s);
+  g_prof.work.push_back(work);
+}
+
+void prof_dispatch_events(hipEvent_t* start, hipEvent_t* stop, double work) {
+  *start = next_event();
+  *stop = next_event();
   g_prof.work.push_back(work);
 }
 
