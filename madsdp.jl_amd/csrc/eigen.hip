@@ -1837,28 +1837,23 @@ __global__ __launch_bounds__(1024) void lz_finish_kernel(int n, int j, const dou
   }
 }
 
-// Smallest eigenvalue theta of the k x k Lanczos tridiagonal (alpha, beta)
-// by Sturm multisection, and the residual bound beta_{k-1} |y_{k-1}| of its
-// normalized eigenvector y (inverse iteration with partial pivoting).
-// out[0] = theta, out[1] = residual bound, out[2] = |beta_{k-1}|.
-__global__ __launch_bounds__(256) void lz_check_kernel(int k, const double* alpha_g,
-                                                       const double* beta_g, double* scratch_g,
-                                                       double* out) {
-  __shared__ double red[4];
-  // the tridiagonal and the inverse-iteration arrays live in LDS (k <= 1024)
-  __shared__ double alpha[1024], beta[1024];
-  __shared__ double scratch[7 * 1024];
-  (void)scratch_g;
-  for (int i = threadIdx.x; i < k; i += blockDim.x) {
-    alpha[i] = alpha_g[i];
-    beta[i] = beta_g[i];
-  }
-  __syncthreads();
-  __shared__ double lo_s, hi_s;
-  __shared__ int first;
+// Smallest eigenvalue theta of the k x k Lanczos tridiagonal (alpha, beta,
+// in LDS) by Sturm multisection over NT points per round, and the residual
+// bound beta_{k-1} |y_{k-1}| of its normalized eigenvector y (inverse
+// iteration with partial pivoting, one thread).  Called by every thread of
+// the workgroup; out[0] = theta, out[1] = residual bound, out[2] =
+// |beta_{k-1}| are written by thread 0 (LDS or global).
+struct LzShared {
+  double lo, hi;
+  int first;
+};
+
+template <int NT>
+__device__ void lz_ritz_min(int k, const double* alpha, const double* beta, double* scratch,
+                            double* red, LzShared& sh, double* out) {
   const int tid = threadIdx.x;
   double gl = INFINITY, gu = -INFINITY, emax = 0.0;
-  for (int i = tid; i < k; i += 256) {
+  for (int i = tid; i < k; i += NT) {
     const double r = (i > 0 ? fabs(beta[i - 1]) : 0.0) + (i < k - 1 ? fabs(beta[i]) : 0.0);
     gl = fmin(gl, alpha[i] - r);
     gu = fmax(gu, alpha[i] + r);
@@ -1870,83 +1865,204 @@ __global__ __launch_bounds__(256) void lz_check_kernel(int k, const double* alph
   const double pivmin = DBL_MIN * fmax(1.0, emax);
   const double tnorm = fmax(fabs(gl), fabs(gu));
   if (tid == 0) {
-    lo_s = gl - 2 * DEPS * tnorm - 2 * pivmin;
-    hi_s = gu + 2 * DEPS * tnorm + 2 * pivmin;
+    sh.lo = gl - 2 * DEPS * tnorm - 2 * pivmin;
+    sh.hi = gu + 2 * DEPS * tnorm + 2 * pivmin;
   }
   __syncthreads();
   for (int round = 0; round < 12; ++round) {
-    const double lo = lo_s, hi = hi_s;
+    const double lo = sh.lo, hi = sh.hi;
     if (hi - lo <= 2 * DEPS * fmax(fabs(lo), fabs(hi)) + 4 * pivmin) break;
-    if (tid == 0) first = 256;
     __syncthreads();
-    const double x = lo + (hi - lo) * (double)(tid + 1) / 257.0;
-    if (sturm_count(k, alpha, beta, x, pivmin) >= 1) atomicMin(&first, tid);
+    if (tid == 0) sh.first = NT;
+    __syncthreads();
+    const double x = lo + (hi - lo) * (double)(tid + 1) / (double)(NT + 1);
+    if (sturm_count(k, alpha, beta, x, pivmin) >= 1) atomicMin(&sh.first, tid);
     __syncthreads();
     if (tid == 0) {
-      const int f = first;
-      lo_s = f == 0 ? lo : lo + (hi - lo) * (double)f / 257.0;
-      hi_s = f == 256 ? hi : lo + (hi - lo) * (double)(f + 1) / 257.0;
+      const int f = sh.first;
+      sh.lo = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)(NT + 1);
+      sh.hi = f == NT ? hi : lo + (hi - lo) * (double)(f + 1) / (double)(NT + 1);
     }
     __syncthreads();
   }
-  if (tid != 0) return;
-  const double theta = 0.5 * (lo_s + hi_s);
-  // inverse iteration on (T - theta I) y = b with partial pivoting
-  double* u0 = scratch;           // k
-  double* u1 = scratch + k;       // k
-  double* u2 = scratch + 2 * k;   // k
-  double* y = scratch + 3 * k;    // k
-  double* bb = scratch + 4 * k;   // k
-  double* l = scratch + 5 * k;    // multipliers k
-  int* sw = reinterpret_cast<int*>(scratch + 6 * k);  // swaps k
-  const double tiny = DEPS * fmax(tnorm, DBL_MIN);
-  for (int i = 0; i < k; ++i) y[i] = 1.0;
-  for (int pass = 0; pass < 3; ++pass) {
-    // factor
-    double dcur = alpha[0] - theta, ucur = k > 1 ? beta[0] : 0.0;
-    for (int i = 0; i < k; ++i) bb[i] = y[i];
-    for (int i = 0; i < k - 1; ++i) {
-      const double sub = beta[i];
-      const double dn = alpha[i + 1] - theta;
-      const double un = i + 1 < k - 1 ? beta[i + 1] : 0.0;
-      if (fabs(dcur) >= fabs(sub)) {
-        const double f = dcur != 0.0 ? sub / dcur : 0.0;
-        u0[i] = dcur; u1[i] = ucur; u2[i] = 0.0; l[i] = f; sw[i] = 0;
-        dcur = dn - f * ucur;
-        ucur = un;
-      } else {
-        const double f = dcur / sub;
-        u0[i] = sub; u1[i] = dn; u2[i] = un; l[i] = f; sw[i] = 1;
-        dcur = ucur - f * dn;
-        ucur = -f * un;
+  if (tid == 0) {
+    const double theta = 0.5 * (sh.lo + sh.hi);
+    // inverse iteration on (T - theta I) y = b with partial pivoting
+    double* u0 = scratch;           // k
+    double* u1 = scratch + k;       // k
+    double* u2 = scratch + 2 * k;   // k
+    double* y = scratch + 3 * k;    // k
+    double* bb = scratch + 4 * k;   // k
+    double* l = scratch + 5 * k;    // multipliers k
+    int* sw = reinterpret_cast<int*>(scratch + 6 * k);  // swaps k
+    const double tiny = DEPS * fmax(tnorm, DBL_MIN);
+    for (int i = 0; i < k; ++i) y[i] = 1.0;
+    for (int pass = 0; pass < 3; ++pass) {
+      // factor
+      double dcur = alpha[0] - theta, ucur = k > 1 ? beta[0] : 0.0;
+      for (int i = 0; i < k; ++i) bb[i] = y[i];
+      for (int i = 0; i < k - 1; ++i) {
+        const double sub = beta[i];
+        const double dn = alpha[i + 1] - theta;
+        const double un = i + 1 < k - 1 ? beta[i + 1] : 0.0;
+        if (fabs(dcur) >= fabs(sub)) {
+          const double f = dcur != 0.0 ? sub / dcur : 0.0;
+          u0[i] = dcur; u1[i] = ucur; u2[i] = 0.0; l[i] = f; sw[i] = 0;
+          dcur = dn - f * ucur;
+          ucur = un;
+        } else {
+          const double f = dcur / sub;
+          u0[i] = sub; u1[i] = dn; u2[i] = un; l[i] = f; sw[i] = 1;
+          dcur = ucur - f * dn;
+          ucur = -f * un;
+        }
       }
+      u0[k - 1] = dcur;
+      // forward
+      for (int i = 0; i < k - 1; ++i) {
+        if (sw[i]) { const double t = bb[i]; bb[i] = bb[i + 1]; bb[i + 1] = t - l[i] * bb[i]; }
+        else bb[i + 1] -= l[i] * bb[i];
+      }
+      // backward
+      for (int i = k - 1; i >= 0; --i) {
+        double v = bb[i];
+        if (i + 1 < k) v -= u1[i] * y[i + 1];
+        if (i + 2 < k) v -= u2[i] * y[i + 2];
+        double piv = u0[i];
+        if (fabs(piv) < tiny) piv = piv < 0 ? -tiny : tiny;
+        y[i] = v / piv;
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < k; ++i) nrm = fmax(nrm, fabs(y[i]));
+      if (!(nrm > 0) || !isfinite(nrm)) break;
+      for (int i = 0; i < k; ++i) y[i] /= nrm;
     }
-    u0[k - 1] = dcur;
-    // forward
-    for (int i = 0; i < k - 1; ++i) {
-      if (sw[i]) { const double t = bb[i]; bb[i] = bb[i + 1]; bb[i + 1] = t - l[i] * bb[i]; }
-      else bb[i + 1] -= l[i] * bb[i];
-    }
-    // backward
-    for (int i = k - 1; i >= 0; --i) {
-      double v = bb[i];
-      if (i + 1 < k) v -= u1[i] * y[i + 1];
-      if (i + 2 < k) v -= u2[i] * y[i + 2];
-      double piv = u0[i];
-      if (fabs(piv) < tiny) piv = piv < 0 ? -tiny : tiny;
-      y[i] = v / piv;
-    }
-    double nrm = 0.0;
-    for (int i = 0; i < k; ++i) nrm = fmax(nrm, fabs(y[i]));
-    if (!(nrm > 0) || !isfinite(nrm)) break;
-    for (int i = 0; i < k; ++i) y[i] /= nrm;
+    double n2 = 0.0;
+    for (int i = 0; i < k; ++i) n2 += y[i] * y[i];
+    const double ylast = fabs(y[k - 1]) / sqrt(n2);
+    out[0] = theta;
+    out[1] = isfinite(ylast) ? fabs(beta[k - 1]) * ylast : INFINITY;
+    out[2] = fabs(beta[k - 1]);
   }
-  double n2 = 0.0;
-  for (int i = 0; i < k; ++i) n2 += y[i] * y[i];
-  const double ylast = fabs(y[k - 1]) / sqrt(n2);
-  out[0] = theta;
-  out[1] = isfinite(ylast) ? fabs(beta[k - 1]) * ylast : INFINITY;
-  out[2] = fabs(beta[k - 1]);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void lz_check_kernel(int k, const double* alpha_g,
+                                                       const double* beta_g, double* scratch_g,
+                                                       double* out) {
+  __shared__ double red[4];
+  // the tridiagonal and the inverse-iteration arrays live in LDS (k <= 1024)
+  __shared__ double alpha[1024], beta[1024];
+  __shared__ double scratch[7 * 1024];
+  __shared__ LzShared sh;
+  (void)scratch_g;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    alpha[i] = alpha_g[i];
+    beta[i] = beta_g[i];
+  }
+  __syncthreads();
+  lz_ritz_min<256>(k, alpha, beta, scratch, red, sh, out);
+}
+
+// The whole Lanczos lambda_min of lanczos_min (same start vector, full
+// reorthogonalisation by classical Gram-Schmidt applied twice, the same
+// check every LZ_CHECK steps and the same stopping rules) in one workgroup,
+// for n <= LZ1_MAX: one launch and no host round trips instead of six
+// launches per step and a host read per check.  The Krylov basis Q (n x
+// (n + 1)) stays in global memory (L2-resident at these sizes), the current
+// vectors and the tridiagonal in LDS.
+constexpr int LZ1_MAX = 640;
+constexpr int LZ1_NT = 1024;
+
+__global__ __launch_bounds__(LZ1_NT) void lz1_kernel(int n, const double* A, int64_t lda,
+                                                    double* Q, double* out) {
+  __shared__ double w[LZ1_MAX], q[LZ1_MAX], h[LZ1_MAX + 1];
+  __shared__ double alpha[LZ1_MAX], beta[LZ1_MAX];
+  __shared__ double scratch[7 * LZ1_MAX];
+  __shared__ double red[16];
+  __shared__ double chk[3];
+  __shared__ LzShared sh;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = LZ1_NT / 64;
+  const int kmax = n;
+  // q_0: the hashed start vector, normalised
+  double ss = 0.0;
+  for (int i = tid; i < n; i += LZ1_NT) {
+    const double v = lz_hash(i);
+    q[i] = v;
+    ss += v * v;
+  }
+  ss = block_reduce<0>(ss, red);
+  {
+    const double inv = 1.0 / sqrt(ss);
+    for (int i = tid; i < n; i += LZ1_NT) {
+      q[i] *= inv;
+      Q[i] = q[i];
+    }
+  }
+  __syncthreads();
+  double anorm_est = 0.0;
+  double theta = 0.0;
+  for (int j = 0; j < kmax; ++j) {
+    // w = A q_j (A symmetric: row i = column i), one row per thread
+    for (int i = tid; i < n; i += LZ1_NT) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int c = 0;
+      for (; c + 3 < n; c += 4) {
+        a0 += A[i + (int64_t)c * lda] * q[c];
+        a1 += A[i + (int64_t)(c + 1) * lda] * q[c + 1];
+        a2 += A[i + (int64_t)(c + 2) * lda] * q[c + 2];
+        a3 += A[i + (int64_t)(c + 3) * lda] * q[c + 3];
+      }
+      for (; c < n; ++c) a0 += A[i + (int64_t)c * lda] * q[c];
+      w[i] = (a0 + a1) + (a2 + a3);
+    }
+    __syncthreads();
+    double hj = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      // h = Q[:, 0:j+1]' w, one wavefront per column
+      for (int c = wave; c <= j; c += NW) {
+        const double* col = Q + (int64_t)c * n;
+        double v = 0.0;
+        for (int i = lane; i < n; i += 64) v += col[i] * w[i];
+        v = wave_sum(v);
+        if (lane == 0) h[c] = v;
+      }
+      __syncthreads();
+      // w -= Q h
+      for (int i = tid; i < n; i += LZ1_NT) {
+        double v = 0.0;
+        for (int c = 0; c <= j; ++c) v += Q[i + (int64_t)c * n] * h[c];
+        w[i] -= v;
+      }
+      hj += h[j];
+      __syncthreads();
+    }
+    // alpha_j, beta_j = ||w||, q_{j+1} = w / beta_j
+    double s2 = 0.0;
+    for (int i = tid; i < n; i += LZ1_NT) s2 += w[i] * w[i];
+    s2 = block_reduce<0>(s2, red);
+    const double b = sqrt(s2);
+    const double inv = b > 0 ? 1.0 / b : 0.0;
+    if (tid == 0) {
+      alpha[j] = hj;
+      beta[j] = b;
+    }
+    for (int i = tid; i < n; i += LZ1_NT) {
+      q[i] = w[i] * inv;
+      Q[i + (int64_t)(j + 1) * n] = q[i];
+    }
+    __syncthreads();
+    const int k = j + 1;
+    if (k % LZ_CHECK == 0 || k == kmax) {
+      lz_ritz_min<LZ1_NT>(k, alpha, beta, scratch, red, sh, chk);
+      theta = chk[0];
+      anorm_est = fmax(anorm_est, fabs(theta));
+      const bool invariant = chk[2] <= 1e-14 * fmax(anorm_est, 1e-300);
+      if (chk[1] <= LZ_RTOL * fmax(fabs(theta), 1e-300) || invariant || k == n) break;
+    }
+  }
+  if (tid == 0) out[0] = theta;
 }
 
 static int lanczos_min(hipStream_t s, int n, const double* A, int64_t lda, double* out,
@@ -1999,6 +2115,11 @@ int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out, dou
     double* wv = work;
     MSDP_TRY(jacobi_batched(s, n, A, lda, 0, wv, 0, nullptr, 0, 0, 1, false));
     MSDP_HIP(hipMemcpyAsync(out, wv, sizeof(double), hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
+  if (n >= LZ_MIN_N && n <= LZ1_MAX) {
+    lz1_kernel<<<1, LZ1_NT, 0, s>>>(n, A, lda, work, out);
+    MSDP_HIP(hipGetLastError());
     return 0;
   }
   if (n >= LZ_MIN_N) return lanczos_min(s, n, A, lda, out, work);

@@ -179,10 +179,12 @@ def test_syevd(env, n, kind):
 
 
 @pytest.mark.parametrize("n,kind", [(3, "random"), (64, "random"), (65, "random"), (200, "random"),
-                                    (500, "random"), (1000, "random"), (1000, "cluster"),
+                                    (400, "step"), (500, "random"), (640, "cluster"),
+                                    (641, "random"), (1000, "random"), (1000, "cluster"),
                                     (1500, "step")])
 def test_eig_min(env, n, kind):
-    """Below 400 by tridiagonalisation + Sturm, above by Lanczos."""
+    """Below 400 by tridiagonalisation + Sturm; 400..640 by the one-workgroup
+    Lanczos kernel, above by the multi-launch Lanczos."""
     torch, lib, st = env
     rng = np.random.default_rng(n + 3)
     if kind == "random":
