@@ -129,6 +129,11 @@ int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t st
 int trtri_lower(hipStream_t s, int n, const double* L, int64_t ldl, double* Linv,
                 int64_t ldi, double* work);
 size_t trtri_work_elems(int n);
+// count matrices at strides sL / sLi; work holds count x trtri_work_elems(n)
+// at stride sW
+int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_t ldl,
+                        int64_t sL, double* Li, int64_t ldi, int64_t sLi, double* work,
+                        int64_t sW);
 // Cholesky flush of the factor: zero the upper triangle, zero entries below
 // eps^2 * max |L_ii| (src/blocks.jl:101-111).
 int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch);

@@ -704,10 +704,13 @@ __device__ __attribute__((always_inline)) inline void trtri64_step(const double*
 }
 
 __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, int64_t ldl,
-                                                        double* Li, int64_t ldi) {
+                                                        int64_t sL, double* Li, int64_t ldi,
+                                                        int64_t sLi) {
   // Column c of T^-1 by forward substitution in lane c, right-looking: x_k
   // is final at step k (1 / T_kk for k = c, x_k / T_kk below), then
-  // x_i -= T_ik x_k for i > k.
+  // x_i -= T_ik x_k for i > k.  blockIdx.y: the matrix of a batch.
+  L += (int64_t)blockIdx.y * sL;
+  Li += (int64_t)blockIdx.y * sLi;
   const int b0 = blockIdx.x * 64;
   const int nb = min(64, n - b0);
   __shared__ double T[64 * W64_LD];
@@ -733,39 +736,48 @@ __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, 
     if (c < nb) Li[(b0 + lane) + (int64_t)(b0 + c) * ldi] = T[lane * W64_LD + c];
 }
 
-__global__ void zero_upper_kernel(int n, double* A, int64_t lda) {
+__global__ void zero_upper_kernel(int n, double* A, int64_t lda, int64_t sA) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  A += (int64_t)blockIdx.z * sA;
   if (i < n && i < j) A[i + (int64_t)j * lda] = 0.0;
 }
 
-static int trtri_rec(hipStream_t s, int lo, int hi, const double* L, int64_t ldl, double* Li,
-                     int64_t ldi, double* work) {
+// One level of the recursion for every matrix of the batch at once (the
+// batch strides sL, sLi, sW apply to L, Li and the work buffer).
+static int trtri_rec(hipStream_t s, int lo, int hi, const double* L, int64_t ldl, int64_t sL,
+                     double* Li, int64_t ldi, int64_t sLi, double* work, int64_t sW, int count) {
   const int nblk = cdiv(hi - lo, 64);
   if (nblk <= 1) return 0;
   const int mid = lo + 64 * ((nblk + 1) / 2);
-  MSDP_TRY(trtri_rec(s, lo, mid, L, ldl, Li, ldi, work));
-  MSDP_TRY(trtri_rec(s, mid, hi, L, ldl, Li, ldi, work));
+  MSDP_TRY(trtri_rec(s, lo, mid, L, ldl, sL, Li, ldi, sLi, work, sW, count));
+  MSDP_TRY(trtri_rec(s, mid, hi, L, ldl, sL, Li, ldi, sLi, work, sW, count));
   const int r = hi - mid, c = mid - lo;
   // work = L21 * L11inv   (r x c)
   MSDP_TRY(gemm(s, false, false, r, c, c, 1.0, L + mid + (int64_t)lo * ldl, ldl,
-                Li + lo + (int64_t)lo * ldi, ldi, 0.0, work, r));
+                Li + lo + (int64_t)lo * ldi, ldi, 0.0, work, r, count, sL, sLi, sW));
   // X21 = -L22inv * work
   MSDP_TRY(gemm(s, false, false, r, c, r, -1.0, Li + mid + (int64_t)mid * ldi, ldi, work, r, 0.0,
-                Li + mid + (int64_t)lo * ldi, ldi));
+                Li + mid + (int64_t)lo * ldi, ldi, count, sLi, sW, sLi));
   return 0;
 }
 
 size_t trtri_work_elems(int n) { return (size_t)(n / 2 + 64) * (size_t)(n / 2 + 64); }
 
+int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_t ldl,
+                        int64_t sL, double* Li, int64_t ldi, int64_t sLi, double* work,
+                        int64_t sW) {
+  if (n <= 0 || count <= 0) return 0;
+  dim3 g(cdiv(n, 256), n, count);
+  zero_upper_kernel<<<g, 256, 0, s>>>(n, Li, ldi, sLi);
+  MSDP_HIP(hipGetLastError());
+  trtri_diag_kernel<<<dim3(cdiv(n, 64), count), 64, 0, s>>>(n, L, ldl, sL, Li, ldi, sLi);
+  MSDP_HIP(hipGetLastError());
+  return trtri_rec(s, 0, n, L, ldl, sL, Li, ldi, sLi, work, sW, count);
+}
+
 int trtri_lower(hipStream_t s, int n, const double* L, int64_t ldl, double* Li, int64_t ldi,
                 double* work) {
-  if (n <= 0) return 0;
-  dim3 g(cdiv(n, 256), n);
-  zero_upper_kernel<<<g, 256, 0, s>>>(n, Li, ldi);
-  MSDP_HIP(hipGetLastError());
-  trtri_diag_kernel<<<cdiv(n, 64), 64, 0, s>>>(n, L, ldl, Li, ldi);
-  MSDP_HIP(hipGetLastError());
-  return trtri_rec(s, 0, n, L, ldl, Li, ldi, work);
+  return trtri_lower_batched(s, n, 1, L, ldl, 0, Li, ldi, 0, work, 0);
 }
 
 // ==========================================================================

@@ -104,8 +104,9 @@ struct msdp_solver {
   // factorisation serves many solves (iterative refinement).
   int form_block_inverses() {
     if (!Hdi || hdi_valid) return 0;
-    for (int64_t k = 0; k < m / 512; ++k)
-      MSDP_TRY(trtri_lower(st, 512, H + k * 512 * (m + 1), m, Hdi + k * 512 * 512, 512, Hdiw));
+    // all full 512-row diagonal blocks in one batched triangular inverse
+    MSDP_TRY(trtri_lower_batched(st, 512, (int)(m / 512), H, m, 512 * (m + 1), Hdi, 512,
+                                 512 * 512, Hdiw, (int64_t)trtri_work_elems(512)));
     hdi_valid = true;
     return 0;
   }
@@ -461,7 +462,7 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&Hdinv, potrf_work_elems((int)mm)));
   if (m >= 512) {
     MSDP_TRY(alloc(&Hdi, (size_t)(m / 512) * 512 * 512));
-    MSDP_TRY(alloc(&Hdiw, trtri_work_elems(512)));
+    MSDP_TRY(alloc(&Hdiw, (size_t)(m / 512) * trtri_work_elems(512)));
   }
   MSDP_TRY(alloc(&wm2, mm));
   MSDP_TRY(alloc(&rh, mm));
@@ -569,6 +570,7 @@ int64_t msdp_solver::footprint() const {
   bytes += std::min<int64_t>(swork_b, 256ll << 20);
   bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
+  bytes += (mm / 512) * (512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdiw
   bytes += ((int64_t)syevd_work_elems((int)nmax) + trtri_work_elems((int)nmax) + 3 * tot) * d;
   return bytes + (1 << 20);
 }
@@ -835,6 +837,10 @@ int msdp_solver::factorize_schur(bool& ok) {
       ok = true;
       return 0;
     }
+    // an FP32-assembled H that is not numerically PD at the reference's
+    // first regularisation is off by its assembly error, which the ladder's
+    // absolute shifts (<= 1e-5) do not cover: FP64 from here on at once
+    if (any_lowp) break;
     reg = reg == 0 ? 1e-12 : 10 * reg;
   }
   if (any_lowp) {
