@@ -3,6 +3,9 @@
 // (Julia's `Matrix{Float64}` layout), leading dimension given explicitly.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <functional>
+#include <vector>
 #include <cstdint>
 #include <cmath>
 #include <string>
@@ -192,6 +195,91 @@ struct EigBatch {
   double* Vtmp;
 };
 constexpr int JACOBI_VB_MAX = 128;
+// --------------------------------------------------------------------------
+// Per-block op recorder.  Inside a recorded region the device ops below do
+// not launch: each appends itself (a batchable descriptor and a closure that
+// launches it alone) to the current block's list.  The region's flush then
+// walks the lists position by position: the same kind of op of many blocks
+// at one position becomes one variable-size batched launch (64 x 64 MFMA
+// tiles, elementwise, reductions, small Cholesky), everything else replays
+// per block in order.  A block's ops keep their order and each batched
+// launch does exactly the arithmetic of the separate launches, so results
+// are bit-identical with or without recording.
+// --------------------------------------------------------------------------
+enum RecKind {
+  REC_OPAQUE = 0, REC_GEMM_NN, REC_GEMM_TN, REC_GEMM_NT, REC_GEMM_TT, REC_EW, REC_RED,
+  REC_CHOL64, REC_NKINDS
+};
+struct GemmItem {
+  int M, N, K, lower;
+  double alpha, beta;
+  const double* A;
+  const double* B;
+  double* C;
+  int64_t lda, ldb, ldc;
+};
+enum EwOp {
+  EW_AXPBY = 0, EW_AXPBYPCZ, EW_FILL, EW_COPY, EW_SYM, EW_MIRROR, EW_TRANSPOSE, EW_SCALE_COLS,
+  EW_SCALE_ROWS, EW_RNT, EW_SCALE_BY_DIAG
+};
+struct EwItem {
+  int op, rows, cols, p;
+  int64_t count, lda, ldb;
+  double a, b, c;
+  const double* X;
+  const double* Y;
+  const double* V;
+  double* Z;
+};
+enum RedOp { RED_DOT = 0, RED_MAXABS, RED_SQRTMIN };
+struct RedItem {
+  int op, pad;
+  int64_t count;
+  const double* X;
+  const double* Y;
+  double* out;
+  double* D;
+};
+struct CholItem {  // _cholesky_factor! of an n <= 64 block (cholesky_small)
+  int n, pad;
+  const double* M;
+  double* L;
+  int* info;
+  int64_t ldm, ldl;
+};
+struct RecOp {
+  int kind = REC_OPAQUE;
+  GemmItem g{};
+  EwItem e{};
+  RedItem r{};
+  CholItem ch{};
+  std::function<int()> run;
+};
+struct Recorder {
+  std::vector<std::vector<RecOp>> ops;  // per block
+  int cur = 0;
+};
+// the recorder of the calling thread's current region (null: launch at once)
+Recorder*& rec_active();
+// ops without a recorded form refuse to run inside a recorded region (the
+// caller wraps them as opaque closures instead)
+inline int rec_forbid(const char* what) {
+  if (!rec_active()) return 0;
+  set_error(std::string(what) + " called inside a recorded region");
+  return -3;
+}
+inline int rec_add(RecOp&& op) {
+  rec_active()->ops[rec_active()->cur].push_back(std::move(op));
+  return 0;
+}
+// A batched launch's arguments: rec_pack writes the items of one kind (and
+// the prefix of their workgroup counts) to host memory and returns the byte
+// count and the grid size; rec_launch runs them from the same bytes copied
+// to device memory.
+size_t rec_bytes(int kind, size_t count);
+size_t rec_pack(int kind, const std::vector<const RecOp*>& items, char* host, int* grid);
+int rec_launch(hipStream_t s, int kind, int count, int grid, const char* dev);
+
 int jacobi_vbatched(hipStream_t s, const EigBatch* items_dev, int count, bool vectors);
 // eigenvalues ascending in w and the matching vectors in V (Vtmp: n x n
 // scratch), n <= 128, one workgroup per item

@@ -10,12 +10,18 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 
 namespace msdp {
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+
+Recorder*& rec_active() {
+  static thread_local Recorder* r = nullptr;
+  return r;
+}
 
 namespace {
 struct SyncEvent {
@@ -72,18 +78,13 @@ static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 constexpr int GT = 64;   // tile edge
 constexpr int GK = 16;   // k depth per LDS stage
 
+// One 64 x 64 output tile (tm, tn) of C = alpha op(A) op(B) + beta C.
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(
-    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
-    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
-    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
-  if (guard && *guard != 0) return;
-  const int tm = blockIdx.x, tn = blockIdx.y;
+__device__ __attribute__((always_inline)) inline void gemm64_tile(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda,
+    const double* __restrict__ B, int64_t ldb, double beta, double* __restrict__ C, int64_t ldc,
+    int lower, int tm, int tn) {
   if (lower && tn > tm) return;
-  const int64_t bz = blockIdx.z;
-  A += bz * sA;
-  B += bz * sB;
-  C += bz * sC;
   __shared__ double As[GK][GT + 1];
   __shared__ double Bs[GK][GT + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -141,6 +142,35 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(
       *c = beta == 0.0 ? v : v + beta * (*c);
     }
   }
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
+    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
+    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
+  if (guard && *guard != 0) return;
+  const int64_t bz = blockIdx.z;
+  gemm64_tile<TA, TB>(M, N, K, alpha, A + bz * sA, lda, B + bz * sB, ldb, beta, C + bz * sC, ldc,
+                      lower, blockIdx.x, blockIdx.y);
+}
+
+// Variable-size batch of 64-tile products (recorded per-block GEMMs): the
+// workgroup's item is found in the prefix of the items' tile counts.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_vb_kernel(const GemmItem* __restrict__ items,
+                                                      const int* __restrict__ start, int count) {
+  int lo = 0, hi = count - 1;
+  const int w = blockIdx.x;
+  while (lo < hi) {  // last item with start <= w
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  const GemmItem it = items[lo];
+  const int t = w - start[lo];
+  const int ntm = (it.M + GT - 1) / GT;
+  gemm64_tile<TA, TB>(it.M, it.N, it.K, it.alpha, it.A, it.lda, it.B, it.ldb, it.beta, it.C,
+                      it.ldc, it.lower, t % ntm, t / ntm);
 }
 
 // Large-tile GEMM: 128x128 output tile per workgroup of 4 waves, each wave
@@ -476,6 +506,19 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
 int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, const double* A,
          int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
          int batch, int64_t sA, int64_t sB, int64_t sC, bool lower) {
+  if (rec_active()) {
+    RecOp o;
+    // batched only where a lone launch takes the 64-tile kernel as well
+    if (batch == 1 && M > 0 && N > 0 && K > 0 && std::max(M, std::max(N, K)) < 512) {
+      o.kind = ta ? (tb ? REC_GEMM_TT : REC_GEMM_TN) : (tb ? REC_GEMM_NT : REC_GEMM_NN);
+      o.g = GemmItem{M, N, K, lower ? 1 : 0, alpha, beta, A, B, C, lda, ldb, ldc};
+    }
+    o.run = [=] {
+      return gemm_guarded(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB,
+                          sC, lower, nullptr);
+    };
+    return rec_add(std::move(o));
+  }
   return gemm_guarded(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC,
                       lower, nullptr);
 }
@@ -648,6 +691,7 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(int rows, double* A21, 
 size_t potrf_work_elems(int n) { return (size_t)cdiv(std::max(n, 1), 64) * 4096; }
 
 int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work) {
+  MSDP_TRY(rec_forbid("potrf"));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int jb = std::min(64, n - j0);
@@ -766,6 +810,7 @@ size_t trtri_work_elems(int n) { return (size_t)(n / 2 + 64) * (size_t)(n / 2 + 
 int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_t ldl,
                         int64_t sL, double* Li, int64_t ldi, int64_t sLi, double* work,
                         int64_t sW) {
+  MSDP_TRY(rec_forbid("trtri"));
   if (n <= 0 || count <= 0) return 0;
   dim3 g(cdiv(n, 256), n, count);
   zero_upper_kernel<<<g, 256, 0, s>>>(n, Li, ldi, sLi);
@@ -813,6 +858,15 @@ size_t reduce_scratch_elems(int64_t) { return RED_BLOCKS; }
 
 int dot(hipStream_t s, int64_t count, const double* X, const double* Y, double* out,
         double* scratch) {
+  if (rec_active()) {
+    RecOp o;
+    if (count > 0 && count <= (int64_t)RED_BLOCKS * 1024) {
+      o.kind = REC_RED;
+      o.r = RedItem{RED_DOT, 0, count, X, Y, out, nullptr};
+    }
+    o.run = [=] { return dot(s, count, X, Y, out, scratch); };
+    return rec_add(std::move(o));
+  }
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (count + 1023) / 1024));
   reduce_pass1<0><<<g, 256, 0, s>>>(count, X, Y, scratch);
   reduce_pass2<0><<<1, 256, 0, s>>>(g, scratch, out);
@@ -821,6 +875,15 @@ int dot(hipStream_t s, int64_t count, const double* X, const double* Y, double* 
 }
 
 int max_abs(hipStream_t s, int64_t count, const double* X, double* out, double* scratch) {
+  if (rec_active()) {
+    RecOp o;
+    if (count > 0 && count <= (int64_t)RED_BLOCKS * 1024) {
+      o.kind = REC_RED;
+      o.r = RedItem{RED_MAXABS, 0, count, X, nullptr, out, nullptr};
+    }
+    o.run = [=] { return max_abs(s, count, X, out, scratch); };
+    return rec_add(std::move(o));
+  }
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (count + 1023) / 1024));
   reduce_pass1<1><<<g, 256, 0, s>>>(count, X, nullptr, scratch);
   reduce_pass2<1><<<1, 256, 0, s>>>(g, scratch, out);
@@ -851,6 +914,7 @@ __global__ void flush_kernel(int n, double* L, int64_t ld, const double* dmax) {
 }
 
 int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch) {
+  MSDP_TRY(rec_forbid("chol_flush"));
   if (n <= 0) return 0;
   diag_maxabs_kernel<<<1, 256, 0, s>>>(n, L, ld, scratch);
   dim3 g(cdiv(n, 256), n);
@@ -863,8 +927,9 @@ int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch) {
 // symmetrised copy, the factorisation (chol64_rows: one row per lane in
 // registers), the flush and the zeroed upper triangle.  Same arithmetic per
 // entry as the blocked path's diagonal kernel.
-__global__ __launch_bounds__(64) void chol_small_kernel(int n, const double* M, int64_t ldm,
-                                                        double* L, int64_t ldl, int* info) {
+__device__ __attribute__((always_inline)) inline void chol_small_body(int n, const double* M,
+                                                                      int64_t ldm, double* L,
+                                                                      int64_t ldl, int* info) {
   __shared__ double Lc[64 * W64_LD];
   const int lane = threadIdx.x;
   double r[64];
@@ -893,11 +958,29 @@ __global__ __launch_bounds__(64) void chol_small_kernel(int n, const double* M, 
   }
 }
 
+__global__ __launch_bounds__(64) void chol_small_kernel(int n, const double* M, int64_t ldm,
+                                                        double* L, int64_t ldl, int* info) {
+  chol_small_body(n, M, ldm, L, ldl, info);
+}
+
+// a batch of blocks of different sizes, one wavefront each
+__global__ __launch_bounds__(64) void chol_small_vb_kernel(const CholItem* __restrict__ items) {
+  const CholItem it = items[blockIdx.x];
+  chol_small_body(it.n, it.M, it.ldm, it.L, it.ldl, it.info);
+}
+
 int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
                    int* info) {
   if (n <= 0 || n > 64) {
     set_error("cholesky_small: n must be in 1..64");
     return -2;
+  }
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_CHOL64;
+    o.ch = CholItem{n, 0, M, L, info, ldm, ldl};
+    o.run = [=] { return cholesky_small(s, n, M, ldm, L, ldl, info); };
+    return rec_add(std::move(o));
   }
   chol_small_kernel<<<1, 64, 0, s>>>(n, M, ldm, L, ldl, info);
   MSDP_HIP(hipGetLastError());
@@ -919,6 +1002,13 @@ __global__ void symmetrize_kernel(int n, double* A, int64_t lda) {
 
 int symmetrize(hipStream_t s, int n, double* A, int64_t lda) {
   if (n <= 1) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_SYM, n, n, 0, 0, lda, 0, 0, 0, 0, nullptr, nullptr, nullptr, A};
+    o.run = [=] { return symmetrize(s, n, A, lda); };
+    return rec_add(std::move(o));
+  }
   dim3 g(cdiv(n, 256), n);
   symmetrize_kernel<<<g, 256, 0, s>>>(n, A, lda);
   MSDP_HIP(hipGetLastError());
@@ -933,6 +1023,13 @@ __global__ void mirror_lower_kernel(int n, double* A, int64_t lda) {
 
 int mirror_lower(hipStream_t s, int n, double* A, int64_t lda) {
   if (n <= 1) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_MIRROR, n, n, 0, 0, lda, 0, 0, 0, 0, nullptr, nullptr, nullptr, A};
+    o.run = [=] { return mirror_lower(s, n, A, lda); };
+    return rec_add(std::move(o));
+  }
   dim3 g(cdiv(n, 256), n);
   mirror_lower_kernel<<<g, 256, 0, s>>>(n, A, lda);
   MSDP_HIP(hipGetLastError());
@@ -947,6 +1044,13 @@ __global__ void copy_kernel(int rows, const double* A, int64_t lda, double* B, i
 int copy_mat(hipStream_t s, int rows, int cols, const double* A, int64_t lda, double* B,
              int64_t ldb) {
   if (rows <= 0 || cols <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_COPY, rows, cols, 0, 0, lda, ldb, 0, 0, 0, A, nullptr, nullptr, B};
+    o.run = [=] { return copy_mat(s, rows, cols, A, lda, B, ldb); };
+    return rec_add(std::move(o));
+  }
   dim3 g(cdiv(rows, 256), cols);
   copy_kernel<<<g, 256, 0, s>>>(rows, A, lda, B, ldb);
   MSDP_HIP(hipGetLastError());
@@ -972,6 +1076,13 @@ __global__ void transpose_kernel(int rows, int cols, const double* A, int64_t ld
 int transpose(hipStream_t s, int rows, int cols, const double* A, int64_t lda, double* B,
               int64_t ldb) {
   if (rows <= 0 || cols <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_TRANSPOSE, rows, cols, 0, 0, lda, ldb, 0, 0, 0, A, nullptr, nullptr, B};
+    o.run = [=] { return transpose(s, rows, cols, A, lda, B, ldb); };
+    return rec_add(std::move(o));
+  }
   dim3 g(cdiv(rows, 32), cdiv(cols, 32));
   transpose_kernel<<<g, 256, 0, s>>>(rows, cols, A, lda, B, ldb);
   MSDP_HIP(hipGetLastError());
@@ -999,6 +1110,13 @@ static int ew_grid(int64_t count) {
 int axpby(hipStream_t s, int64_t count, double a, const double* X, double b, const double* Y,
           double* Z) {
   if (count <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_AXPBY, 0, 0, 0, count, 0, 0, a, b, 0, X, Y, nullptr, Z};
+    o.run = [=] { return axpby(s, count, a, X, b, Y, Z); };
+    return rec_add(std::move(o));
+  }
   axpby_kernel<<<ew_grid(count), 256, 0, s>>>(count, a, X, b, Y, Z);
   MSDP_HIP(hipGetLastError());
   return 0;
@@ -1007,6 +1125,13 @@ int axpby(hipStream_t s, int64_t count, double a, const double* X, double b, con
 int axpbypcz(hipStream_t s, int64_t count, double a, const double* X, double b, const double* Y,
              double c, const double* V, double* Z) {
   if (count <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_AXPBYPCZ, 0, 0, 0, count, 0, 0, a, b, c, X, Y, V, Z};
+    o.run = [=] { return axpbypcz(s, count, a, X, b, Y, c, V, Z); };
+    return rec_add(std::move(o));
+  }
   axpbypcz_kernel<<<ew_grid(count), 256, 0, s>>>(count, a, X, b, Y, c, V, Z);
   MSDP_HIP(hipGetLastError());
   return 0;
@@ -1033,6 +1158,13 @@ __global__ void fill_kernel(int64_t count, double v, double* X) {
 
 int fill(hipStream_t s, int64_t count, double v, double* X) {
   if (count <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_FILL, 0, 0, 0, count, 0, 0, v, 0, 0, nullptr, nullptr, nullptr, X};
+    o.run = [=] { return fill(s, count, v, X); };
+    return rec_add(std::move(o));
+  }
   fill_kernel<<<ew_grid(count), 256, 0, s>>>(count, v, X);
   MSDP_HIP(hipGetLastError());
   return 0;
@@ -1068,6 +1200,13 @@ __global__ void scale_rows_kernel(int rows, double* A, int64_t lda, const double
 
 int scale_cols(hipStream_t s, int rows, int cols, double* A, int64_t lda, const double* d, int p) {
   if (rows <= 0 || cols <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_SCALE_COLS, rows, cols, p, 0, lda, 0, 0, 0, 0, nullptr, d, nullptr, A};
+    o.run = [=] { return scale_cols(s, rows, cols, A, lda, d, p); };
+    return rec_add(std::move(o));
+  }
   dim3 g(cdiv(rows, 256), cols);
   scale_cols_kernel<<<g, 256, 0, s>>>(rows, A, lda, d, p);
   MSDP_HIP(hipGetLastError());
@@ -1076,8 +1215,208 @@ int scale_cols(hipStream_t s, int rows, int cols, double* A, int64_t lda, const 
 
 int scale_rows(hipStream_t s, int rows, int cols, double* A, int64_t lda, const double* d, int p) {
   if (rows <= 0 || cols <= 0) return 0;
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_EW;
+    o.e = EwItem{EW_SCALE_ROWS, rows, cols, p, 0, lda, 0, 0, 0, 0, nullptr, d, nullptr, A};
+    o.run = [=] { return scale_rows(s, rows, cols, A, lda, d, p); };
+    return rec_add(std::move(o));
+  }
   dim3 g(cdiv(rows, 256), cols);
   scale_rows_kernel<<<g, 256, 0, s>>>(rows, A, lda, d, p);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+
+// ==========================================================================
+// Batched launches of recorded per-block ops (see Recorder in common.h).
+// Each item's arithmetic is that of the single-op kernel it replaces.
+// ==========================================================================
+__device__ inline int vb_item(const int* __restrict__ start, int count) {
+  int lo = 0, hi = count - 1;
+  const int w = blockIdx.x;
+  while (lo < hi) {  // last item with start <= w
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void ew_vb_kernel(const EwItem* __restrict__ items,
+                                                    const int* __restrict__ start, int count) {
+  const int k = vb_item(start, count);
+  const EwItem it = items[k];
+  const int64_t e = (int64_t)(blockIdx.x - start[k]) * 256 + threadIdx.x;
+  switch (it.op) {
+    case EW_AXPBY:
+      if (e < it.count) it.Z[e] = it.a * it.X[e] + it.b * it.Y[e];
+      break;
+    case EW_AXPBYPCZ:
+      if (e < it.count) it.Z[e] = it.a * it.X[e] + it.b * it.Y[e] + it.c * it.V[e];
+      break;
+    case EW_FILL:
+      if (e < it.count) it.Z[e] = it.a;
+      break;
+    default: {
+      const int i = (int)(e % it.rows), j = (int)(e / it.rows);
+      if (j >= it.cols) break;
+      switch (it.op) {
+        case EW_COPY: it.Z[i + (int64_t)j * it.ldb] = it.X[i + (int64_t)j * it.lda]; break;
+        case EW_TRANSPOSE: it.Z[j + (int64_t)i * it.ldb] = it.X[i + (int64_t)j * it.lda]; break;
+        case EW_SYM:
+          if (i > j) {
+            double* a = it.Z + i + (int64_t)j * it.lda;
+            double* b = it.Z + j + (int64_t)i * it.lda;
+            const double v = (*a + *b) / 2;
+            *a = v;
+            *b = v;
+          }
+          break;
+        case EW_MIRROR:
+          if (i < j) it.Z[i + (int64_t)j * it.lda] = it.Z[j + (int64_t)i * it.lda];
+          break;
+        case EW_SCALE_COLS: {
+          const double sc = sqrt(it.Y[j]);
+          double* a = it.Z + i + (int64_t)j * it.lda;
+          if (it.p > 0) *a *= sc; else *a /= sc;
+          break;
+        }
+        case EW_SCALE_ROWS: {
+          const double sc = sqrt(it.Y[i]);
+          double* a = it.Z + i + (int64_t)j * it.lda;
+          if (it.p > 0) *a *= sc; else *a /= sc;
+          break;
+        }
+        case EW_RNT: {
+          const int n = it.rows;
+          it.Z[i + (int64_t)j * n] =
+              -(it.X[i + (int64_t)j * n] + it.X[j + (int64_t)i * n]) / (it.Y[i] + it.Y[j]);
+          break;
+        }
+        case EW_SCALE_BY_DIAG: {
+          const int n = it.rows;
+          it.Z[i + (int64_t)j * n] = it.X[i + (int64_t)j * n] * it.Y[j + (int64_t)j * n];
+          break;
+        }
+        default: break;
+      }
+    }
+  }
+}
+
+// One workgroup per reduction.  The dot product and the maximum keep the
+// two-pass order of dot / max_abs (the first pass's workgroups visited in
+// turn), so the sums are the same bits.
+__global__ __launch_bounds__(256) void red_vb_kernel(const RedItem* __restrict__ items) {
+  const RedItem it = items[blockIdx.x];
+  __shared__ double sh[4];
+  __shared__ double part[RED_BLOCKS];
+  const int tid = threadIdx.x;
+  if (it.op == RED_SQRTMIN) {
+    double v = INFINITY;
+    for (int64_t k = tid; k < it.count; k += 256) {
+      v = fmin(v, it.X[k]);
+      it.D[k] = sqrt(it.X[k]);
+    }
+    v = block_reduce<2>(v, sh);
+    if (tid == 0) *it.out = v;
+    return;
+  }
+  const int g = (int)max((int64_t)1, min((int64_t)RED_BLOCKS, (it.count + 1023) / 1024));
+  for (int b = 0; b < g; ++b) {
+    double v = 0.0;
+    for (int64_t i = (int64_t)b * 256 + tid; i < it.count; i += (int64_t)g * 256) {
+      if (it.op == RED_DOT) v += it.X[i] * it.Y[i];
+      else v = fmax(v, fabs(it.X[i]));
+    }
+    v = it.op == RED_DOT ? block_reduce<0>(v, sh) : block_reduce<1>(v, sh);
+    if (tid == 0) part[b] = v;
+  }
+  __syncthreads();
+  double v = 0.0;
+  for (int i = tid; i < g; i += 256) v = it.op == RED_DOT ? v + part[i] : fmax(v, part[i]);
+  v = it.op == RED_DOT ? block_reduce<0>(v, sh) : block_reduce<1>(v, sh);
+  if (tid == 0) *it.out = v;
+}
+
+static size_t rec_item_bytes(int kind) {
+  switch (kind) {
+    case REC_EW: return sizeof(EwItem);
+    case REC_RED: return sizeof(RedItem);
+    case REC_CHOL64: return sizeof(CholItem);
+    default: return sizeof(GemmItem);
+  }
+}
+
+static size_t rec_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t rec_bytes(int kind, size_t count) {
+  return rec_align(count * rec_item_bytes(kind)) + rec_align((count + 1) * sizeof(int));
+}
+
+size_t rec_pack(int kind, const std::vector<const RecOp*>& items, char* host, int* grid) {
+  const size_t count = items.size();
+  const size_t ib = rec_item_bytes(kind);
+  int* start = reinterpret_cast<int*>(host + rec_align(count * ib));
+  int total = 0;
+  for (size_t k = 0; k < count; ++k) {
+    const RecOp& o = *items[k];
+    int wgs = 1;
+    switch (kind) {
+      case REC_EW: {
+        std::memcpy(host + k * ib, &o.e, ib);
+        const int64_t work = (o.e.op == EW_AXPBY || o.e.op == EW_AXPBYPCZ || o.e.op == EW_FILL)
+                                 ? o.e.count
+                                 : (int64_t)o.e.rows * o.e.cols;
+        wgs = (int)std::max<int64_t>(1, (work + 255) / 256);
+        break;
+      }
+      case REC_RED: std::memcpy(host + k * ib, &o.r, ib); break;
+      case REC_CHOL64: std::memcpy(host + k * ib, &o.ch, ib); break;
+      default:
+        std::memcpy(host + k * ib, &o.g, ib);
+        wgs = cdiv(o.g.M, GT) * cdiv(o.g.N, GT);
+        break;
+    }
+    start[k] = total;
+    total += wgs;
+  }
+  start[count] = total;
+  *grid = total;
+  return rec_bytes(kind, count);
+}
+
+int rec_launch(hipStream_t s, int kind, int count, int grid, const char* dev) {
+  if (count <= 0) return 0;
+  const size_t ib = rec_item_bytes(kind);
+  const int* start = reinterpret_cast<const int*>(dev + rec_align(count * ib));
+  switch (kind) {
+    case REC_EW:
+      ew_vb_kernel<<<grid, 256, 0, s>>>(reinterpret_cast<const EwItem*>(dev), start, count);
+      break;
+    case REC_RED:
+      red_vb_kernel<<<count, 256, 0, s>>>(reinterpret_cast<const RedItem*>(dev));
+      break;
+    case REC_CHOL64:
+      chol_small_vb_kernel<<<count, 64, 0, s>>>(reinterpret_cast<const CholItem*>(dev));
+      break;
+    case REC_GEMM_NN:
+      gemm_vb_kernel<false, false><<<grid, 256, 0, s>>>(reinterpret_cast<const GemmItem*>(dev), start, count);
+      break;
+    case REC_GEMM_TN:
+      gemm_vb_kernel<true, false><<<grid, 256, 0, s>>>(reinterpret_cast<const GemmItem*>(dev), start, count);
+      break;
+    case REC_GEMM_NT:
+      gemm_vb_kernel<false, true><<<grid, 256, 0, s>>>(reinterpret_cast<const GemmItem*>(dev), start, count);
+      break;
+    case REC_GEMM_TT:
+      gemm_vb_kernel<true, true><<<grid, 256, 0, s>>>(reinterpret_cast<const GemmItem*>(dev), start, count);
+      break;
+    default:
+      set_error("rec_launch: not a batchable kind");
+      return -2;
+  }
   MSDP_HIP(hipGetLastError());
   return 0;
 }

@@ -1723,6 +1723,7 @@ size_t syevd_iwork_elems(int n) { return dc_iwork_elems(n); }
 
 int syevd(hipStream_t s, int n, const double* A, int64_t lda, double* w, double* V, int64_t ldv,
           bool vectors, double* work, int* iwork) {
+  MSDP_TRY(rec_forbid("syevd"));
   if (n <= 0) return 0;
   if (n <= EV_MAX && V) {
     syev_one_kernel<<<1, 512, 0, s>>>(EigBatch{n, A, lda, w, nullptr, V, ldv, work});
@@ -2110,6 +2111,7 @@ size_t eig_min_work_elems(int n) {
 }
 
 int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out, double* work, int*) {
+  MSDP_TRY(rec_forbid("eig_min"));
   if (n <= 0) return 0;
   if (n <= 64) {
     double* wv = work;

@@ -37,6 +37,7 @@ __global__ __launch_bounds__(256) void jprod_dense_kernel(int64_t n2, const int*
 }
 
 int jprod(hipStream_t st, const BlockOps& b, const double* X, double* out) {
+  MSDP_TRY(rec_forbid("jprod"));
   if (b.kind == BlockOps::EMPTY) return 0;
   if (b.kind == BlockOps::COO) {
     jprod_coo_kernel<<<cdiv(b.ncon, 256), 256, 0, st>>>(b.ncon, b.con_list, b.con_ptr, b.row,
@@ -96,6 +97,7 @@ int jtprod_split(int n, int ncon) {
 }
 
 int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out) {
+  MSDP_TRY(rec_forbid("jtprod"));
   const int64_t n2 = (int64_t)b.n * b.n;
   if (b.kind == BlockOps::EMPTY) return fill(st, n2, 0.0, out);
   if (b.kind == BlockOps::COO) {
@@ -177,6 +179,7 @@ int scatter_lower_add(hipStream_t st, int mi, const int* con, const double* Hb, 
 
 int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
                 double* work, size_t work_elems) {
+  MSDP_TRY(rec_forbid("schur_block"));
   const int n = b.n;
   if (b.kind == BlockOps::EMPTY || b.ncon == 0) return 0;
   if (b.kind == BlockOps::COO) {
@@ -307,6 +310,13 @@ __global__ void nt_rnt_kernel(int n, const double* M, const double* D, double* R
 }
 
 int nt_rnt(hipStream_t st, int n, const double* M, const double* D, double* R) {
+  if (rec_active()) {
+    RecOp op;
+    op.kind = REC_EW;
+    op.e = EwItem{EW_RNT, n, n, 0, 0, n, n, 0, 0, 0, M, D, nullptr, R};
+    op.run = [=] { return nt_rnt(st, n, M, D, R); };
+    return rec_add(std::move(op));
+  }
   dim3 g(cdiv(n, 256), n);
   nt_rnt_kernel<<<g, 256, 0, st>>>(n, M, D, R);
   MSDP_HIP(hipGetLastError());
@@ -327,6 +337,13 @@ __global__ __launch_bounds__(256) void sqrt_min_kernel(int n, const double* lam,
 }
 
 int sqrt_min(hipStream_t st, int n, const double* lam, double* D, double* minlam) {
+  if (rec_active()) {
+    RecOp op;
+    op.kind = REC_RED;
+    op.r = RedItem{RED_SQRTMIN, 0, n, lam, nullptr, minlam, D};
+    op.run = [=] { return sqrt_min(st, n, lam, D, minlam); };
+    return rec_add(std::move(op));
+  }
   sqrt_min_kernel<<<1, 256, 0, st>>>(n, lam, D, minlam);
   MSDP_HIP(hipGetLastError());
   return 0;
@@ -340,6 +357,13 @@ __global__ void scale_by_diag_kernel(int n, const double* W, const double* M, do
 }
 
 int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, double* out) {
+  if (rec_active()) {
+    RecOp op;
+    op.kind = REC_EW;
+    op.e = EwItem{EW_SCALE_BY_DIAG, n, n, 0, 0, n, n, 0, 0, 0, W, M, nullptr, out};
+    op.run = [=] { return scale_by_diag(st, n, W, M, out); };
+    return rec_add(std::move(op));
+  }
   dim3 g(cdiv(n, 256), n);
   scale_by_diag_kernel<<<g, 256, 0, st>>>(n, W, M, out);
   MSDP_HIP(hipGetLastError());

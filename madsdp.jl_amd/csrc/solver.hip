@@ -85,6 +85,18 @@ struct msdp_solver {
   std::vector<hipEvent_t> aux_done;
   hipEvent_t forked = nullptr;
   std::vector<double*> aux_trwork, aux_red, aux_swork;
+  // Recorded regions of the multi-block stages (Recorder in common.h): the
+  // per-block ops of a region become one batched launch per op position.
+  // Their arguments go through a ring of pinned host slots (an event per
+  // slot guards its reuse) into one device buffer, in stream order.
+  static constexpr int REC_RING = 4;
+  Recorder rec;
+  bool rec_on = false;
+  char* rec_host[REC_RING] = {};
+  hipEvent_t rec_ev[REC_RING] = {};
+  char* rec_dev = nullptr;
+  size_t rec_cap = 0;
+  int rec_slot = 0;
   int nb = 0;
   int64_t m = 0, ns = 0;
   std::vector<Blk> blk;
@@ -153,6 +165,7 @@ struct msdp_solver {
   ~msdp_solver() {
     free_device();
     destroy_aux();
+    free_rec();
     if (hsc_pin) (void)hipHostFree(hsc_pin);
     if (st && own_stream) (void)hipStreamDestroy(st);
   }
@@ -183,6 +196,19 @@ struct msdp_solver {
   }
   int release();
   int keep_solution();
+  // the recorded regions' argument buffers (after the stream has drained)
+  void free_rec() {
+    for (int k = 0; k < REC_RING; ++k) {
+      if (rec_ev[k]) (void)hipEventSynchronize(rec_ev[k]);
+      if (rec_host[k]) (void)hipHostFree(rec_host[k]);
+      if (rec_ev[k]) (void)hipEventDestroy(rec_ev[k]);
+      rec_host[k] = nullptr;
+      rec_ev[k] = nullptr;
+    }
+    if (rec_dev) (void)hipFree(rec_dev);
+    rec_dev = nullptr;
+    rec_cap = 0;
+  }
   void destroy_aux() {
     for (hipStream_t q : aux) (void)hipStreamDestroy(q);
     for (hipEvent_t ev : aux_done) (void)hipEventDestroy(ev);
@@ -210,12 +236,37 @@ struct msdp_solver {
     }
     return 0;
   }
+  // a recorded region in place of fork() ... join() (multi-block only)
+  int rec_begin() {
+    if (!multi) return 0;
+    rec.ops.assign(nb, {});
+    rec.cur = 0;
+    rec_active() = &rec;
+    rec_on = true;
+    return 0;
+  }
+  int rec_end() {
+    if (!rec_on) return 0;
+    rec_active() = nullptr;
+    rec_on = false;
+    return rec_flush();
+  }
+  int rec_flush();
+  // f now, or recorded as one op without a batched form (replayed per block
+  // in its OnBlock scope)
+  int opaque(std::function<int()> f) {
+    if (!rec_active()) return f();
+    RecOp o;
+    o.run = std::move(f);
+    return rec_add(std::move(o));
+  }
   // block i's work on its auxiliary stream (and scratch) for the scope
   struct OnBlock {
     msdp_solver* s;
     hipStream_t st0;
     double *tw0, *red0, *sw0;
     OnBlock(msdp_solver* s_, int i) : s(s_), st0(s_->st), tw0(s_->trwork), red0(s_->red), sw0(s_->swork) {
+      if (s->rec_on) s->rec.cur = i;
       if (s->multi) {
         const int k = i % NAUX;
         s->st = s->aux[k];
@@ -583,6 +634,7 @@ int msdp_solver::release() {
   if (finalized) rc = keep_solution();
   free_device();
   destroy_aux();
+  free_rec();
   released = finalized;
   if (!own_stream) st = nullptr;
   return rc;
@@ -618,18 +670,21 @@ int msdp_solver::keep_solution() {
 int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) {
   const int n = bk.n;
   if (n <= 64) return cholesky_small(st, n, M, n, L, n, inf);
-  MSDP_TRY(copy_mat(st, n, n, M, n, L, n));
-  MSDP_TRY(symmetrize(st, n, L, n));
-  MSDP_TRY(potrf(st, n, L, n, inf, trwork));
-  MSDP_TRY(chol_flush(st, n, L, n, red));
-  return 0;
+  return opaque([=]() -> int {
+    MSDP_TRY(copy_mat(st, n, n, M, n, L, n));
+    MSDP_TRY(symmetrize(st, n, L, n));
+    MSDP_TRY(potrf(st, n, L, n, inf, trwork));
+    MSDP_TRY(chol_flush(st, n, L, n, red));
+    return 0;
+  });
 }
 
 // prepare_block! (blocks.jl:126-130): Ls = chol(S), Si = S^-1 (potri)
 int msdp_solver::prepare_block(Blk& bk, int i) {
   const int n = bk.n;
   MSDP_TRY(cholesky_factor(bk, bk.S, bk.Ls, binfo(i, 0)));
-  MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));
+  Blk* bp = &bk;
+  MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Ls, n, bp->Lsinv, n, trwork); }));
   MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Lsinv, n, bk.Lsinv, n, 0.0, bk.Si, n));
   MSDP_TRY(mirror_lower(st, n, bk.Si, n));
   return 0;
@@ -656,6 +711,81 @@ int msdp_solver::set_dual_slacks(const double* yv) {
 
 // ==========================================================================
 // PrimalDual components
+// Executes the recorded region: op position by op position, every batchable
+// kind of op present at the position as one batched launch on the main
+// stream, the other ops per block on the auxiliary streams (fork ... join).
+// All batched arguments are packed into one pinned slot and copied once.
+int msdp_solver::rec_flush() {
+  size_t L = 0;
+  for (auto& v : rec.ops) L = std::max(L, v.size());
+  struct Group {
+    int kind;
+    std::vector<const RecOp*> items;
+    size_t off = 0;
+    int grid = 0;
+  };
+  std::vector<std::vector<Group>> groups(L);
+  std::vector<std::vector<int>> opq(L);  // blocks whose op at the position is opaque
+  size_t bytes = 0;
+  for (size_t k = 0; k < L; ++k) {
+    std::vector<const RecOp*> bykind[REC_NKINDS];
+    for (int i = 0; i < nb; ++i) {
+      if (k >= rec.ops[i].size()) continue;
+      const RecOp& o = rec.ops[i][k];
+      if (o.kind == REC_OPAQUE) opq[k].push_back(i);
+      else bykind[o.kind].push_back(&o);
+    }
+    for (int t = 1; t < REC_NKINDS; ++t) {
+      if (bykind[t].empty()) continue;
+      Group g;
+      g.kind = t;
+      g.items = std::move(bykind[t]);
+      g.off = bytes;
+      bytes += rec_bytes(t, g.items.size());
+      groups[k].push_back(std::move(g));
+    }
+  }
+  if (bytes > 0) {
+    if (bytes > rec_cap) {  // grow the ring and the device buffer
+      MSDP_HIP(hipStreamSynchronize(st));
+      for (int r = 0; r < REC_RING; ++r) {
+        if (rec_ev[r]) MSDP_HIP(hipEventSynchronize(rec_ev[r]));
+        if (rec_host[r]) MSDP_HIP(hipHostFree(rec_host[r]));
+        rec_host[r] = nullptr;
+      }
+      if (rec_dev) MSDP_HIP(hipFree(rec_dev));
+      rec_dev = nullptr;
+      rec_cap = std::max<size_t>(bytes * 2, 1 << 20);
+      for (int r = 0; r < REC_RING; ++r) {
+        MSDP_HIP(hipHostMalloc((void**)&rec_host[r], rec_cap, hipHostMallocDefault));
+        if (!rec_ev[r]) MSDP_HIP(hipEventCreateWithFlags(&rec_ev[r], hipEventDisableTiming));
+      }
+      MSDP_HIP(hipMalloc((void**)&rec_dev, rec_cap));
+    }
+    rec_slot = (rec_slot + 1) % REC_RING;
+    char* host = rec_host[rec_slot];
+    MSDP_HIP(hipEventSynchronize(rec_ev[rec_slot]));  // its previous copy has completed
+    for (auto& gk : groups)
+      for (auto& g : gk) rec_pack(g.kind, g.items, host + g.off, &g.grid);
+    MSDP_HIP(hipMemcpyAsync(rec_dev, host, bytes, hipMemcpyHostToDevice, st));
+    MSDP_HIP(hipEventRecord(rec_ev[rec_slot], st));
+  }
+  for (size_t k = 0; k < L; ++k) {
+    for (auto& g : groups[k])
+      MSDP_TRY(rec_launch(st, g.kind, (int)g.items.size(), g.grid, rec_dev + g.off));
+    if (!opq[k].empty()) {
+      MSDP_TRY(fork());
+      for (int i : opq[k]) {
+        OnBlock on(this, i);
+        MSDP_TRY(rec.ops[i][k].run());
+      }
+      MSDP_TRY(join());
+    }
+  }
+  for (auto& v : rec.ops) v.clear();
+  return 0;
+}
+
 // ==========================================================================
 // update_residuals! (step.jl:147-182)
 int msdp_solver::update_residuals() {
@@ -665,16 +795,17 @@ int msdp_solver::update_residuals() {
     if (ns > 0) MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, x, ns, 1.0, Rp, m));
     MSDP_TRY(axpby(st, m, 1.0, b, -1.0, Rp, Rp));
   }
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
     const int64_t n2 = (int64_t)bk.n * bk.n;
-    MSDP_TRY(jtprod(st, bk.ops, y, bk.tmp2));
+    Blk* bp = &bk;
+    MSDP_TRY(opaque([=] { return jtprod(st, bp->ops, y, bp->tmp2); }));
     MSDP_TRY(axpbypcz(st, n2, 1.0, bk.C, -1.0, bk.S, -1.0, bk.tmp2, bk.Rd));
     MSDP_TRY(max_abs(st, n2, bk.Rd, slot(bslot(i, 1)), red));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (ns > 0) {
     MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, y, m, 0.0, ws1, ns));
     MSDP_TRY(axpbypcz(st, ns, 1.0, dlin, -1.0, z, -1.0, ws1, Rdlin));
@@ -692,37 +823,40 @@ int msdp_solver::update_residuals() {
 int msdp_solver::prepare(bool& ok) {
   MSDP_TRY(fill(st, 1, 0.0, slot(2)));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
+    Blk* bp = &bk;
     const int n = bk.n;
     MSDP_TRY(prepare_block(bk, i));
     const bool need_lx = nt() || opt.step == MSDP_STEP_EIGEN;
     if (need_lx) {
       MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
-      MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));
+      MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
     }
     if (nt()) {  // prepare_direction!(::NTWorkspace) directions.jl:45-75
       MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n));
       MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Mm, n, bk.Mm, n, 0.0, bk.Nm, n));
       MSDP_TRY(symmetrize(st, n, bk.Nm, n));
       if (n > JACOBI_VB_MAX) {
-        MSDP_TRY(syevd(st, n, bk.Nm, n, bk.lam, bk.Vv, n, true, ework, eiwork));
-        MSDP_TRY(nt_scaling(bk, i));
+        MSDP_TRY(opaque([=]() -> int {
+          MSDP_TRY(syevd(st, n, bp->Nm, n, bp->lam, bp->Vv, n, true, ework, eiwork));
+          return nt_scaling(*bp, i);
+        }));
       }
     }
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (nt() && nsmall > 0) {
     // eigen-decompositions of all small blocks in one launch
     MSDP_TRY(syev_vbatched(st, eb_nt, nsmall));
-    MSDP_TRY(fork());
+    MSDP_TRY(rec_begin());
     for (int i = 0; i < nb; ++i) {
       OnBlock on(this, i);
       if (blk[i].n <= JACOBI_VB_MAX) MSDP_TRY(nt_scaling(blk[i], i));
     }
-    MSDP_TRY(join());
+    MSDP_TRY(rec_end());
   }
   if (ns > 0) MSDP_TRY(scalar_op(st, SOP_ZINV, (int)ns, 0, nullptr, z, nullptr, nullptr, nullptr, zinv, nullptr, false));
   std::vector<int> inf;
@@ -872,7 +1006,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
   // rhs
   if (m > 0) {
     MSDP_HIP(hipMemcpyAsync(dely, Rp, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-    MSDP_TRY(fork());
+    MSDP_TRY(rec_begin());
     for (int i = 0; i < nb; ++i) {
       OnBlock on(this, i);
       Blk& bk = blk[i];
@@ -884,7 +1018,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
       }
       if (corrected) MSDP_TRY(axpby(st, n2, 1.0, bk.buf, -1.0, bk.corr, bk.buf));
     }
-    MSDP_TRY(join());
+    MSDP_TRY(rec_end());
     // A(.) accumulates every block into dely: in block order on the main stream
     for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.buf, dely));
     if (ns > 0) {
@@ -894,13 +1028,14 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     MSDP_TRY(schur_solve(dely));
   }
   // directions
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
     const int n = bk.n;
     const int64_t n2 = (int64_t)n * n;
-    MSDP_TRY(jtprod(st, bk.ops, dely, bk.tmp2));
+    Blk* bp = &bk;
+    MSDP_TRY(opaque([=] { return jtprod(st, bp->ops, dely, bp->tmp2); }));
     MSDP_TRY(axpby(st, n2, 1.0, bk.Rd, -1.0, bk.tmp2, bk.delS));
     MSDP_TRY(symmetrize(st, n, bk.delS, n));
     if (bk.ops.kind == BlockOps::EMPTY) MSDP_TRY(fill(st, n2, 0.0, bk.buf2));
@@ -913,7 +1048,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     if (corrected) MSDP_TRY(axpby(st, n2, 1.0, bk.delX, 1.0, bk.corr, bk.delX));
     MSDP_TRY(symmetrize(st, n, bk.delX, n));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (ns > 0) {
     MSDP_TRY(gemm(st, true, false, (int)ns, 1, (int)m, 1.0, Clin, m, dely, m, 0.0, ws1, ns));
     if (corrected) MSDP_HIP(hipMemcpyAsync(delx, corrlin, sizeof(double) * ns, hipMemcpyDeviceToDevice, st));
@@ -924,7 +1059,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
 
 // correction! (step.jl:263-271; directions.jl:90-100 NT, 123-127 HKM)
 int msdp_solver::correction() {
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
@@ -942,7 +1077,7 @@ int msdp_solver::correction() {
     }
     MSDP_TRY(symmetrize(st, n, bk.corr, n));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (ns > 0) MSDP_TRY(scalar_op(st, SOP_CORR, (int)ns, 0, nullptr, nullptr, zinv, delx, delz, corrlin, nullptr, false));
   return 0;
 }
@@ -950,7 +1085,7 @@ int msdp_solver::correction() {
 // step_lengths (step.jl:186-198 EigenStep, 200-228 CholeskyBacktrack)
 int msdp_solver::step_lengths(double& a, double& bt) {
   if (opt.step == MSDP_STEP_EIGEN) {
-    MSDP_TRY(fork());
+    MSDP_TRY(rec_begin());
     for (int i = 0; i < nb; ++i) {
       OnBlock on(this, i);
       Blk& bk = blk[i];
@@ -962,11 +1097,14 @@ int msdp_solver::step_lengths(double& a, double& bt) {
         MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
         MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n));
         MSDP_TRY(symmetrize(st, n, T[k], n));
-        if (n > JACOBI_VB_MAX)
-          MSDP_TRY(eig_min(st, n, T[k], n, slot(bslot(i, 3 + k)), ework, eiwork));
+        if (n > JACOBI_VB_MAX) {
+          double* Tk = T[k];
+          double* out = slot(bslot(i, 3 + k));
+          MSDP_TRY(opaque([=] { return eig_min(st, n, Tk, n, out, ework, eiwork); }));
+        }
       }
     }
-    MSDP_TRY(join());
+    MSDP_TRY(rec_end());
     if (nsmall > 0) MSDP_TRY(eigvals_vbatched(st, eb_step, 2 * nsmall));
     if (ns > 0) {
       MSDP_TRY(scalar_min_step(st, (int)ns, x, delx, slot(3)));
@@ -1021,7 +1159,7 @@ int msdp_solver::step_lengths(double& a, double& bt) {
 
 // affine_complementarity (step.jl:231-243)
 int msdp_solver::affine_complementarity(double a, double bt, double& out) {
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
@@ -1031,7 +1169,7 @@ int msdp_solver::affine_complementarity(double a, double bt, double& out) {
     MSDP_TRY(dot(st, n2, bk.delX, bk.S, slot(bslot(i, 5)), red));
     MSDP_TRY(dot(st, n2, bk.delX, bk.delS, slot(bslot(i, 6)), red));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (ns > 0) MSDP_TRY(scalar_affine(st, (int)ns, x, delx, z, delz, a, bt, slot(5)));
   MSDP_TRY(read_slots());
   double s = 0.0;
@@ -1047,12 +1185,12 @@ int msdp_solver::affine_complementarity(double a, double bt, double& out) {
 }
 
 int msdp_solver::total_complementarity(double& out) {
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     MSDP_TRY(dot(st, (int64_t)blk[i].n * blk[i].n, blk[i].X, blk[i].S, slot(bslot(i, 3)), red));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (ns > 0) MSDP_TRY(dot(st, ns, x, z, slot(5), red));
   MSDP_TRY(read_slots());
   double s = 0.0;
@@ -1066,7 +1204,7 @@ int msdp_solver::total_complementarity(double& out) {
 int msdp_solver::take_step(double a, double bt) {
   alpha = a;
   beta = bt;
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
@@ -1076,7 +1214,7 @@ int msdp_solver::take_step(double a, double bt) {
     MSDP_TRY(axpby(st, n2, 1.0, bk.S, bt, bk.delS, bk.S));
     MSDP_TRY(symmetrize(st, bk.n, bk.S, bk.n));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (m > 0) MSDP_TRY(axpby(st, m, 1.0, y, bt, dely, y));
   if (ns > 0) {
     MSDP_TRY(axpby(st, ns, 1.0, x, a, delx, x));
@@ -1129,12 +1267,12 @@ int msdp_solver::iterate() {
 // update_termination! (primal_dual.jl:123-135, dual_centering.jl:293-303)
 int msdp_solver::update_termination() {
   if (dual_centering()) MSDP_TRY(update_residuals());
-  MSDP_TRY(fork());
+  MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     MSDP_TRY(dot(st, (int64_t)blk[i].n * blk[i].n, blk[i].C, blk[i].X, slot(bslot(i, 7)), red));
   }
-  MSDP_TRY(join());
+  MSDP_TRY(rec_end());
   if (ns > 0) {
     MSDP_TRY(dot(st, ns, dlin, x, slot(6), red));
     MSDP_TRY(max_abs(st, ns, Rdlin, slot(7), red));
