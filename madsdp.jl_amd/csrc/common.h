@@ -208,7 +208,7 @@ constexpr int JACOBI_VB_MAX = 128;
 // --------------------------------------------------------------------------
 enum RecKind {
   REC_OPAQUE = 0, REC_GEMM_NN, REC_GEMM_TN, REC_GEMM_NT, REC_GEMM_TT, REC_EW, REC_RED,
-  REC_CHOL64, REC_NKINDS
+  REC_CHOL64, REC_TRTRI64, REC_NKINDS
 };
 struct GemmItem {
   int M, N, K, lower;
@@ -220,7 +220,7 @@ struct GemmItem {
 };
 enum EwOp {
   EW_AXPBY = 0, EW_AXPBYPCZ, EW_FILL, EW_COPY, EW_SYM, EW_MIRROR, EW_TRANSPOSE, EW_SCALE_COLS,
-  EW_SCALE_ROWS, EW_RNT, EW_SCALE_BY_DIAG
+  EW_SCALE_ROWS, EW_RNT, EW_SCALE_BY_DIAG, EW_ZERO_UPPER
 };
 struct EwItem {
   int op, rows, cols, p;
@@ -247,12 +247,19 @@ struct CholItem {  // _cholesky_factor! of an n <= 64 block (cholesky_small)
   int* info;
   int64_t ldm, ldl;
 };
+struct TrtriItem {  // L^-1 of an n <= 64 lower factor (trtri_lower)
+  int n, pad;
+  const double* L;
+  double* Li;
+  int64_t ldl, ldi;
+};
 struct RecOp {
   int kind = REC_OPAQUE;
   GemmItem g{};
   EwItem e{};
   RedItem r{};
   CholItem ch{};
+  TrtriItem tr{};
   std::function<int()> run;
 };
 struct Recorder {

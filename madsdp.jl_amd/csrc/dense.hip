@@ -747,15 +747,12 @@ __device__ __attribute__((always_inline)) inline void trtri64_step(const double*
   if constexpr (K + 1 < 64) trtri64_step<K + 1>(T, nb, x);
 }
 
-__global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, int64_t ldl,
-                                                        int64_t sL, double* Li, int64_t ldi,
-                                                        int64_t sLi) {
+__device__ __attribute__((always_inline)) inline void trtri64_body(int n, const double* L,
+                                                                   int64_t ldl, double* Li,
+                                                                   int64_t ldi, int b0) {
   // Column c of T^-1 by forward substitution in lane c, right-looking: x_k
   // is final at step k (1 / T_kk for k = c, x_k / T_kk below), then
-  // x_i -= T_ik x_k for i > k.  blockIdx.y: the matrix of a batch.
-  L += (int64_t)blockIdx.y * sL;
-  Li += (int64_t)blockIdx.y * sLi;
-  const int b0 = blockIdx.x * 64;
+  // x_i -= T_ik x_k for i > k.
   const int nb = min(64, n - b0);
   __shared__ double T[64 * W64_LD];
   const int lane = threadIdx.x;
@@ -778,6 +775,20 @@ __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, 
 #pragma unroll 8
   for (int c = 0; c < 64; ++c)
     if (c < nb) Li[(b0 + lane) + (int64_t)(b0 + c) * ldi] = T[lane * W64_LD + c];
+}
+
+// the diagonal 64-blocks (blockIdx.x) of a batch of matrices (blockIdx.y)
+__global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* L, int64_t ldl,
+                                                        int64_t sL, double* Li, int64_t ldi,
+                                                        int64_t sLi) {
+  trtri64_body(n, L + (int64_t)blockIdx.y * sL, ldl, Li + (int64_t)blockIdx.y * sLi, ldi,
+               blockIdx.x * 64);
+}
+
+// recorded n <= 64 inverses of different blocks, one wavefront each
+__global__ __launch_bounds__(64) void trtri64_vb_kernel(const TrtriItem* __restrict__ items) {
+  const TrtriItem it = items[blockIdx.x];
+  trtri64_body(it.n, it.L, it.ldl, it.Li, it.ldi, 0);
 }
 
 __global__ void zero_upper_kernel(int n, double* A, int64_t lda, int64_t sA) {
@@ -810,6 +821,28 @@ size_t trtri_work_elems(int n) { return (size_t)(n / 2 + 64) * (size_t)(n / 2 + 
 int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_t ldl,
                         int64_t sL, double* Li, int64_t ldi, int64_t sLi, double* work,
                         int64_t sW) {
+  if (rec_active() && count == 1 && n > 0 && n <= 64) {
+    // no recursion, no workspace: the zeroed upper triangle and the one
+    // diagonal-block inverse as two recorded ops
+    RecOp z;
+    z.kind = REC_EW;
+    z.e = EwItem{EW_ZERO_UPPER, n, n, 0, 0, ldi, 0, 0, 0, 0, nullptr, nullptr, nullptr, Li};
+    z.run = [=]() -> int {
+      zero_upper_kernel<<<dim3(cdiv(n, 256), n, 1), 256, 0, s>>>(n, Li, ldi, 0);
+      MSDP_HIP(hipGetLastError());
+      return 0;
+    };
+    MSDP_TRY(rec_add(std::move(z)));
+    RecOp t;
+    t.kind = REC_TRTRI64;
+    t.tr = TrtriItem{n, 0, L, Li, ldl, ldi};
+    t.run = [=]() -> int {
+      trtri_diag_kernel<<<dim3(1, 1), 64, 0, s>>>(n, L, ldl, 0, Li, ldi, 0);
+      MSDP_HIP(hipGetLastError());
+      return 0;
+    };
+    return rec_add(std::move(t));
+  }
   MSDP_TRY(rec_forbid("trtri"));
   if (n <= 0 || count <= 0) return 0;
   dim3 g(cdiv(n, 256), n, count);
@@ -1276,6 +1309,9 @@ __global__ __launch_bounds__(256) void ew_vb_kernel(const EwItem* __restrict__ i
         case EW_MIRROR:
           if (i < j) it.Z[i + (int64_t)j * it.lda] = it.Z[j + (int64_t)i * it.lda];
           break;
+        case EW_ZERO_UPPER:
+          if (i < j) it.Z[i + (int64_t)j * it.lda] = 0.0;
+          break;
         case EW_SCALE_COLS: {
           const double sc = sqrt(it.Y[j]);
           double* a = it.Z + i + (int64_t)j * it.lda;
@@ -1345,6 +1381,7 @@ static size_t rec_item_bytes(int kind) {
     case REC_EW: return sizeof(EwItem);
     case REC_RED: return sizeof(RedItem);
     case REC_CHOL64: return sizeof(CholItem);
+    case REC_TRTRI64: return sizeof(TrtriItem);
     default: return sizeof(GemmItem);
   }
 }
@@ -1374,6 +1411,7 @@ size_t rec_pack(int kind, const std::vector<const RecOp*>& items, char* host, in
       }
       case REC_RED: std::memcpy(host + k * ib, &o.r, ib); break;
       case REC_CHOL64: std::memcpy(host + k * ib, &o.ch, ib); break;
+      case REC_TRTRI64: std::memcpy(host + k * ib, &o.tr, ib); break;
       default:
         std::memcpy(host + k * ib, &o.g, ib);
         wgs = cdiv(o.g.M, GT) * cdiv(o.g.N, GT);
@@ -1400,6 +1438,9 @@ int rec_launch(hipStream_t s, int kind, int count, int grid, const char* dev) {
       break;
     case REC_CHOL64:
       chol_small_vb_kernel<<<count, 64, 0, s>>>(reinterpret_cast<const CholItem*>(dev));
+      break;
+    case REC_TRTRI64:
+      trtri64_vb_kernel<<<count, 64, 0, s>>>(reinterpret_cast<const TrtriItem*>(dev));
       break;
     case REC_GEMM_NN:
       gemm_vb_kernel<false, false><<<grid, 256, 0, s>>>(reinterpret_cast<const GemmItem*>(dev), start, count);

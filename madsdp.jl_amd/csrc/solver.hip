@@ -684,7 +684,8 @@ int msdp_solver::prepare_block(Blk& bk, int i) {
   const int n = bk.n;
   MSDP_TRY(cholesky_factor(bk, bk.S, bk.Ls, binfo(i, 0)));
   Blk* bp = &bk;
-  MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Ls, n, bp->Lsinv, n, trwork); }));
+  if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));  // recorded
+  else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Ls, n, bp->Lsinv, n, trwork); }));
   MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Lsinv, n, bk.Lsinv, n, 0.0, bk.Si, n));
   MSDP_TRY(mirror_lower(st, n, bk.Si, n));
   return 0;
@@ -833,7 +834,8 @@ int msdp_solver::prepare(bool& ok) {
     const bool need_lx = nt() || opt.step == MSDP_STEP_EIGEN;
     if (need_lx) {
       MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
-      MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
+      if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));  // recorded
+      else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
     }
     if (nt()) {  // prepare_direction!(::NTWorkspace) directions.jl:45-75
       MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n));
