@@ -555,12 +555,13 @@ int msdp_solver::finalize() {
       MSDP_TRY(stream_sync(st));
     }
   }
-  // chunk buffers of the dense Schur assembly: up to 64 constraints or 256 MiB
+  // chunk buffers of the dense Schur assembly: as many constraints per chunk
+  // as fit in 64 MiB (fewer, larger batched products for small blocks)
   size_t want = 0;
   for (auto& bb : blk)
     if (bb.ops.kind == BlockOps::DENSE)
-      want = std::max(want, 2 * (size_t)bb.n * bb.n * (size_t)std::min(bb.ops.ncon, 64));
-  want = std::min(want, (size_t)(256u << 20) / 8);
+      want = std::max(want, 2 * (size_t)bb.n * bb.n * (size_t)bb.ops.ncon);
+  want = std::min(want, (size_t)(64u << 20) / 8);
   for (auto& bb : blk)
     if (bb.ops.kind == BlockOps::DENSE) want = std::max(want, 2 * (size_t)bb.n * bb.n);
   swork_elems = want;
@@ -613,12 +614,12 @@ int64_t msdp_solver::footprint() const {
       bytes += 2 * mi * n2 * d + mi * 4 + (mi == m ? 0 : mi * mi * d);
       bytes += (int64_t)jtprod_split(b.n, (int)mi) * n2 * d;
       if (opt.precision == 1) bytes += 2 * mi * n2 * 4 + 2 * n2 * 4;
-      swork_b = std::max(swork_b, 2 * n2 * std::min<int64_t>(mi, 64) * d);
+      swork_b = std::max(swork_b, std::max(2 * n2 * d, std::min<int64_t>(2 * n2 * mi * d, 64ll << 20)));
     } else if (b.has_coo) {
       bytes += (int64_t)b.h_con.size() * (4 * 3 + 8 + 8 + 4 + 4) + 64;
     }
   }
-  bytes += std::min<int64_t>(swork_b, 256ll << 20);
+  bytes += swork_b;
   bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
   bytes += (mm / 512) * (512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdiw

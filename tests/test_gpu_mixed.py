@@ -124,3 +124,30 @@ def test_mixed_dense_m1100(gpu):
     print(f"fp64 {a.iter} it, mixed {b.iter} it, lowp {b.info().lowp_factorizations}, "
           f"refine {b.info().refine_steps}, fallbacks {b.info().fp64_fallbacks}", flush=True)
     _check(a, b)
+
+
+def test_mixed_error_vs_oracle(gpu):
+    """The mixed path's end point against the FP64 oracle's solve of the same
+    instance: objective, primal and dual residuals, and X, y, S.  The errors
+    are printed (and stated in the design notes); the bounds hold at the 1e-7
+    relative gap both solves stop at."""
+    from oracle import madsdp_oracle as O
+    model = instances.dense_sdp(30, 200, 30, 4)
+    ref = O.Solver(model).solve()
+    b = M.MadSDPSolver(model, precision="mixed").solve()
+    assert ref.status == O.Status.SOLVE_SUCCEEDED and b.status == M.Status.SOLVE_SUCCEEDED
+    assert b.info().lowp_factorizations > 0
+    tr = b.trace()
+    r = ref.trace[-1]
+    e_obj = abs(b.primal_objective() - r.pobj) / max(1.0, abs(r.pobj))
+    rel = lambda a, c: np.linalg.norm(a - c) / max(1.0, np.linalg.norm(c))
+    e_X = rel(b.X(0), ref.blocks[0].X)
+    e_S = rel(b.S(0), ref.blocks[0].S)
+    e_y = rel(b.y, ref.y)
+    print(f"mixed vs oracle: obj {e_obj:.2e}, inf_pr {tr[-1, 2]:.2e} (oracle {r.inf_pr:.2e}), "
+          f"inf_du {tr[-1, 3]:.2e} (oracle {r.inf_du:.2e}), X {e_X:.2e}, y {e_y:.2e}, "
+          f"S {e_S:.2e}; iterations {b.iter} vs {ref.iter}", flush=True)
+    assert e_obj <= 1e-6
+    assert tr[-1, 2] <= 1e-7 and tr[-1, 3] <= 1e-7
+    assert max(e_X, e_y, e_S) <= 1e-3
+    assert abs(b.iter - ref.iter) <= 2
