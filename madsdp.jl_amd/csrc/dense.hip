@@ -9,6 +9,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -797,6 +798,94 @@ __global__ void zero_upper_kernel(int n, double* A, int64_t lda, int64_t sA) {
   if (i < n && i < j) A[i + (int64_t)j * lda] = 0.0;
 }
 
+// All nodes of one depth of the recursion (count <= TRL_MAX, each smaller
+// than 512 on both sides) in one launch: stage 1 forms every node's
+// work = L21 L11^-1 (the node's own slice of the work buffer), stage 2 its
+// X21 = -L22^-1 work.  The nodes are independent; each tile is the 64-tile
+// kernel's, so the arithmetic is that of one product per node.
+constexpr int TRL_MAX = 64;
+struct TrtriLevel {
+  int count;
+  int lo[TRL_MAX], mid[TRL_MAX], hi[TRL_MAX], start[TRL_MAX + 1];
+  int64_t woff[TRL_MAX];
+};
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void trtri_level_kernel(TrtriLevel lv, const double* L,
+                                                          int64_t ldl, double* Li, int64_t ldi,
+                                                          double* work) {
+  int lo = 0, hi = lv.count - 1;
+  const int w = blockIdx.x;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (lv.start[m] <= w) lo = m; else hi = m - 1;
+  }
+  const int k = lo;
+  const int a = lv.lo[k], b = lv.mid[k], c = lv.hi[k];
+  const int r = c - b, cc = b - a;
+  const int t = w - lv.start[k];
+  const int ntm = (r + GT - 1) / GT;
+  double* wk = work + lv.woff[k];
+  if (STAGE == 1)
+    gemm64_tile<false, false>(r, cc, cc, 1.0, L + b + (int64_t)a * ldl, ldl,
+                              Li + a + (int64_t)a * ldi, ldi, 0.0, wk, r, 0, t % ntm, t / ntm);
+  else
+    gemm64_tile<false, false>(r, cc, r, -1.0, Li + b + (int64_t)b * ldi, ldi, wk, r, 0.0,
+                              Li + b + (int64_t)a * ldi, ldi, 0, t % ntm, t / ntm);
+}
+
+static void trtri_tree(int lo, int hi, int depth, std::vector<std::vector<std::array<int, 3>>>& lv) {
+  const int nblk = cdiv(hi - lo, 64);
+  if (nblk <= 1) return;
+  const int mid = lo + 64 * ((nblk + 1) / 2);
+  trtri_tree(lo, mid, depth + 1, lv);
+  trtri_tree(mid, hi, depth + 1, lv);
+  if ((int)lv.size() <= depth) lv.resize(depth + 1);
+  lv[depth].push_back({lo, mid, hi});
+}
+
+// The recursion of trtri_rec for one matrix, depth by depth from the leaves.
+static int trtri_levels(hipStream_t s, int n, const double* L, int64_t ldl, double* Li,
+                        int64_t ldi, double* work) {
+  std::vector<std::vector<std::array<int, 3>>> lv;
+  trtri_tree(0, n, 0, lv);
+  for (int dpt = (int)lv.size() - 1; dpt >= 0; --dpt) {
+    const auto& nodes = lv[dpt];
+    bool small = (int)nodes.size() <= TRL_MAX;
+    for (const auto& nd : nodes) small = small && nd[2] - nd[1] < 512 && nd[1] - nd[0] < 512;
+    if (!small) {
+      for (const auto& nd : nodes) {
+        const int lo = nd[0], mid = nd[1], hi = nd[2];
+        const int r = hi - mid, c = mid - lo;
+        MSDP_TRY(gemm(s, false, false, r, c, c, 1.0, L + mid + (int64_t)lo * ldl, ldl,
+                      Li + lo + (int64_t)lo * ldi, ldi, 0.0, work, r));
+        MSDP_TRY(gemm(s, false, false, r, c, r, -1.0, Li + mid + (int64_t)mid * ldi, ldi, work, r,
+                      0.0, Li + mid + (int64_t)lo * ldi, ldi));
+      }
+      continue;
+    }
+    TrtriLevel p{};
+    p.count = (int)nodes.size();
+    int tiles = 0;
+    int64_t woff = 0;
+    for (int k = 0; k < p.count; ++k) {
+      p.lo[k] = nodes[k][0];
+      p.mid[k] = nodes[k][1];
+      p.hi[k] = nodes[k][2];
+      const int r = p.hi[k] - p.mid[k], c = p.mid[k] - p.lo[k];
+      p.start[k] = tiles;
+      p.woff[k] = woff;
+      tiles += cdiv(r, GT) * cdiv(c, GT);
+      woff += (int64_t)r * c;
+    }
+    p.start[p.count] = tiles;
+    trtri_level_kernel<1><<<tiles, 256, 0, s>>>(p, L, ldl, Li, ldi, work);
+    trtri_level_kernel<2><<<tiles, 256, 0, s>>>(p, L, ldl, Li, ldi, work);
+    MSDP_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
 // One level of the recursion for every matrix of the batch at once (the
 // batch strides sL, sLi, sW apply to L, Li and the work buffer).
 static int trtri_rec(hipStream_t s, int lo, int hi, const double* L, int64_t ldl, int64_t sL,
@@ -850,6 +939,7 @@ int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_
   MSDP_HIP(hipGetLastError());
   trtri_diag_kernel<<<dim3(cdiv(n, 64), count), 64, 0, s>>>(n, L, ldl, sL, Li, ldi, sLi);
   MSDP_HIP(hipGetLastError());
+  if (count == 1) return trtri_levels(s, n, L, ldl, Li, ldi, work);
   return trtri_rec(s, 0, n, L, ldl, sL, Li, ldi, sLi, work, sW, count);
 }
 

@@ -1759,7 +1759,7 @@ int syevd(hipStream_t s, int n, const double* A, int64_t lda, double* w, double*
 // ==========================================================================
 constexpr int LZ_MIN_N = 400;     // below this, tridiagonalisation + Sturm
 constexpr double LZ_RTOL = 1e-10;
-constexpr int LZ_CHECK = 8;
+constexpr int LZ_CHECK = 16;
 
 __device__ inline double lz_hash(int64_t i) {
   uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;

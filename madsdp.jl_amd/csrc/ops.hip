@@ -201,6 +201,7 @@ int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, doubl
   // per-chunk buffers for T_j and P_j
   const int64_t per = 2 * n2;
   int cb = (int)std::max<int64_t>(1, std::min<int64_t>(mi, (int64_t)(work_elems / per)));
+  if (cb > 64 && cb < mi) cb = cb / 64 * 64;  // whole 64-constraint batches (measured faster)
   double* T = work;
   double* P = work + (int64_t)cb * n2;
   for (int j0 = 0; j0 < mi; j0 += cb) {
