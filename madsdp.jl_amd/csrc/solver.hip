@@ -1807,8 +1807,21 @@ static void keep_pool(int device) {
   done[device] = true;
 }
 
+// A recorded region that an error return left open ends with the API call:
+// the calling thread records nothing afterwards.
+struct RecScope {
+  msdp_solver* s;
+  ~RecScope() {
+    if (s && s->rec_on) {
+      rec_active() = nullptr;
+      s->rec_on = false;
+      for (auto& v : s->rec.ops) v.clear();
+    }
+  }
+};
 #define GUARD0(s) \
-  if (!(s)) return as_err("null solver");
+  if (!(s)) return as_err("null solver"); \
+  RecScope rec_scope_{const_cast<msdp_solver*>(s)};
 #define GUARD(s) \
   GUARD0(s)      \
   if ((s)->released) return as_err("solver released: its device state was freed after the solve");
