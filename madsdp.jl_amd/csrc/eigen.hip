@@ -1256,21 +1256,28 @@ __global__ __launch_bounds__(1024) void dc_deflate_kernel(const DcNode* nodes, c
   double* gds = ds;
   double* gzs = zs;
   if (use_lds) { ds = lds_ds; zs = lds_zs; }
-  if (tid == 0) {
-  double dmax = 0.0, zmax = 0.0;
-  for (int r = 0; r < K; ++r) { dmax = fmax(dmax, fabs(ds[r])); zmax = fmax(zmax, fabs(zs[r])); }
-  const double tol = 8.0 * DEPS * fmax(dmax, zmax);
+  // the deflation tolerance: max |d| and max |z| over the whole merge
+  __shared__ double red[16];
+  __shared__ int s_kp, s_nd;
+  double dmx = 0.0, zmx = 0.0;
+  for (int r = tid; r < K; r += nt) {
+    dmx = fmax(dmx, fabs(ds[r]));
+    zmx = fmax(zmx, fabs(zs[r]));
+  }
+  const double dmax = block_reduce<1>(dmx, red);
+  const double zmax = block_reduce<1>(zmx, red);
   int* fpos = v.fpos + lo;
   int* defl = v.defl + lo;       // list of deflated sorted positions
+  int* ndl = v.orig + lo;        // list of non-deflated sorted positions
+  double* dn = v.dn + lo;
+  double* zn = v.zn + lo;
+  if (tid == 0) {
+  const double tol = 8.0 * DEPS * fmax(dmax, zmax);
   int* rota = v.rot_a + lo;
   int* rotb = v.rot_b + lo;
   double* rotc = v.rot_c + lo;
   double* rots = v.rot_s + lo;
-  double* dn = v.dn + lo;
-  double* zn = v.zn + lo;
   int kp = 0, nd_ = 0, nr = 0;
-  // non-deflated positions are collected into fpos[0..kp) temporarily
-  int* ndl = v.orig + lo;  // temp list of non-deflated sorted positions
   if (rho * zmax <= tol) {
     for (int r = 0; r < K; ++r) defl[nd_++] = r;
   } else {
@@ -1299,16 +1306,23 @@ __global__ __launch_bounds__(1024) void dc_deflate_kernel(const DcNode* nodes, c
     }
     if (pj >= 0) ndl[kp++] = pj;
   }
-  for (int q = 0; q < kp; ++q) {
-    const int r = ndl[q];
-    fpos[r] = q;
-    dn[q] = ds[r];
-    zn[q] = zs[r];
-  }
-  for (int q = 0; q < nd_; ++q) fpos[defl[q]] = kp + q;
   v.kp[blockIdx.x] = kp;
   v.nrot[blockIdx.x] = nr;
   v.rho[blockIdx.x] = rho;
+  s_kp = kp;
+  s_nd = nd_;
+  }
+  __syncthreads();
+  // positions of the non-deflated values first, then the deflated ones
+  {
+    const int kp = s_kp, nd_ = s_nd;
+    for (int q = tid; q < kp; q += nt) {
+      const int r = ndl[q];
+      fpos[r] = q;
+      dn[q] = ds[r];
+      zn[q] = zs[r];
+    }
+    for (int q = tid; q < nd_; q += nt) fpos[defl[q]] = kp + q;
   }
   __syncthreads();
   if (use_lds)
