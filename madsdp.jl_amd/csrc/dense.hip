@@ -949,34 +949,47 @@ int trtri_lower(hipStream_t s, int n, const double* L, int64_t ldl, double* Li, 
 }
 
 // ==========================================================================
-// reductions (deterministic: fixed grid, two passes)
+// reductions (deterministic: fixed grid, partials added in a fixed order)
 // ==========================================================================
 constexpr int RED_BLOCKS = 512;
 
+// One launch: every workgroup reduces its grid-strided share into a partial,
+// and the last one to finish (an integer counter in the scratch after the
+// partials, reset by that workgroup) adds the partials in index order.  The
+// grid is fixed by the count, so the order, and the result, is too.
 template <int OP>
-__global__ __launch_bounds__(256) void reduce_pass1(int64_t count, const double* X,
-                                                    const double* Y, double* partial) {
+__global__ __launch_bounds__(256) void reduce_one_kernel(int64_t count, const double* X,
+                                                         const double* Y, double* partial,
+                                                         int* counter, double* out) {
   __shared__ double sh[4];
-  double v = OP == 0 ? 0.0 : 0.0;
+  __shared__ int last;
+  double v = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * 256) {
     if (OP == 0) v += X[i] * Y[i];
     else v = fmax(v, fabs(X[i]));
   }
   v = block_reduce<OP == 0 ? 0 : 1>(v, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = v;
+    __threadfence();
+    last = atomicAdd(counter, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double* pv = partial;
+  double w = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += 256) w = OP == 0 ? w + pv[i] : fmax(w, pv[i]);
+  w = block_reduce<OP == 0 ? 0 : 1>(w, sh);
+  if (threadIdx.x == 0) {
+    *out = w;
+    *counter = 0;
+  }
 }
 
-template <int OP>
-__global__ __launch_bounds__(256) void reduce_pass2(int nparts, const double* partial,
-                                                    double* out) {
-  __shared__ double sh[4];
-  double v = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += 256) v = OP == 0 ? v + partial[i] : fmax(v, partial[i]);
-  v = block_reduce<OP == 0 ? 0 : 1>(v, sh);
-  if (threadIdx.x == 0) *out = v;
-}
-
+// the partials (RED_BLOCKS) and, after them, the counter of reduce_one_kernel
+// (zero when the scratch is handed out)
 size_t reduce_scratch_elems(int64_t) { return RED_BLOCKS; }
 
 int dot(hipStream_t s, int64_t count, const double* X, const double* Y, double* out,
@@ -991,8 +1004,8 @@ int dot(hipStream_t s, int64_t count, const double* X, const double* Y, double* 
     return rec_add(std::move(o));
   }
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (count + 1023) / 1024));
-  reduce_pass1<0><<<g, 256, 0, s>>>(count, X, Y, scratch);
-  reduce_pass2<0><<<1, 256, 0, s>>>(g, scratch, out);
+  reduce_one_kernel<0><<<g, 256, 0, s>>>(count, X, Y, scratch,
+                                         reinterpret_cast<int*>(scratch + RED_BLOCKS), out);
   MSDP_HIP(hipGetLastError());
   return 0;
 }
@@ -1008,8 +1021,8 @@ int max_abs(hipStream_t s, int64_t count, const double* X, double* out, double* 
     return rec_add(std::move(o));
   }
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (count + 1023) / 1024));
-  reduce_pass1<1><<<g, 256, 0, s>>>(count, X, nullptr, scratch);
-  reduce_pass2<1><<<1, 256, 0, s>>>(g, scratch, out);
+  reduce_one_kernel<1><<<g, 256, 0, s>>>(count, X, nullptr, scratch,
+                                         reinterpret_cast<int*>(scratch + RED_BLOCKS), out);
   MSDP_HIP(hipGetLastError());
   return 0;
 }
@@ -1432,8 +1445,8 @@ __global__ __launch_bounds__(256) void ew_vb_kernel(const EwItem* __restrict__ i
 }
 
 // One workgroup per reduction.  The dot product and the maximum keep the
-// two-pass order of dot / max_abs (the first pass's workgroups visited in
-// turn), so the sums are the same bits.
+// order of dot / max_abs (a partial per workgroup of their fixed grid, the
+// partials added in index order), so the sums are the same bits.
 __global__ __launch_bounds__(256) void red_vb_kernel(const RedItem* __restrict__ items) {
   const RedItem it = items[blockIdx.x];
   __shared__ double sh[4];

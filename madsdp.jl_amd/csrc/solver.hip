@@ -375,6 +375,7 @@ int msdp_solver::setup_aux() {
     double *a, *b, *c = nullptr;
     MSDP_TRY(alloc(&a, tw));
     MSDP_TRY(alloc(&b, reduce_scratch_elems(0) + 64));
+    MSDP_TRY(fill(st, 64, 0.0, b + reduce_scratch_elems(0)));  // the reduction counter
     if (swork_elems) MSDP_TRY(alloc(&c, swork_elems));
     aux_trwork.push_back(a);
     aux_red.push_back(b);
@@ -524,6 +525,7 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&sc, nslots));
   if (!hsc_pin) MSDP_HIP(hipHostMalloc((void**)&hsc_pin, sizeof(double) * nslots, hipHostMallocDefault));
   MSDP_TRY(alloc(&red, reduce_scratch_elems(0) + 64));
+  MSDP_TRY(fill(st, 64, 0.0, red + reduce_scratch_elems(0)));  // the reduction counter
   MSDP_TRY(alloc(&ework, syevd_work_elems(nmax)));
   MSDP_TRY(alloc(&eiwork, syevd_iwork_elems(nmax)));
   MSDP_TRY(alloc(&trwork, std::max(std::max(trtri_work_elems(nmax), potrf_work_elems(nmax)),
