@@ -123,7 +123,10 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
 // Lower Cholesky in place (reads the lower triangle).  *info_dev receives 0 or
 // the 1-based index of the first non-positive pivot; kernels after a failure
 // return immediately.
-int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info_dev, double* work);
+// With `inverses`, work (potrf_work_elems) receives the inverses of the
+// factor's diagonal 64-blocks (block b at work + 4096 b, 64 x 64, ld 64).
+int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info_dev, double* work,
+          bool inverses = false);
 size_t potrf_work_elems(int n);
 // Batched Cholesky of small matrices (n <= 64), one workgroup each.
 int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t stride,

@@ -375,6 +375,8 @@ struct BlasHandle {
 thread_local BlasHandle t_blas;
 }  // namespace
 
+constexpr int SPLITK_WGS = 1024, SPLITK_MIN_K = 256, SPLITK_MIN_CHUNK = 64;
+
 static bool vendor_gemm_shape(int M, int N, int K, int batch, bool lower, const int* guard) {
   return batch == 1 && !lower && !guard && M >= 512 && N >= 512 && K >= 32;
 }
@@ -444,9 +446,11 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     return 0;
   }
   const int64_t tiles64 = (int64_t)cdiv(M, GT) * cdiv(N, GT) * batch;
-  if (!guard && batch == 1 && tiles64 < 128 && K >= 1024) {
-    // split K so that about 256 workgroups run
-    int parts = (int)std::min<int64_t>(std::max<int64_t>(2, 256 / tiles64), K / 256);
+  if (!guard && batch == 1 && tiles64 < 128 && K >= SPLITK_MIN_K) {
+    // split K so that about SPLITK_WGS workgroups run, chunks of at least
+    // SPLITK_MIN_CHUNK
+    int parts = (int)std::min<int64_t>(std::max<int64_t>(2, SPLITK_WGS / tiles64),
+                                       K / SPLITK_MIN_CHUNK);
     const int kc = ((K + parts - 1) / parts + GK - 1) / GK * GK;
     parts = (K + kc - 1) / kc;
     void* P = nullptr;
@@ -653,62 +657,82 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     dinv[lane + 64 * c] = (lane < n && c < n && c <= lane) ? Lc[lane * W64_LD + c] : 0.0;
 }
 
-// A21 <- A21 * L11^-T in place, with L11^-1 from chol_diag_kernel; each
-// workgroup owns 64 rows.
-__global__ __launch_bounds__(256) void chol_panel_kernel(int rows, double* A21, int64_t lda,
-                                                         const double* dinv, const int* info) {
+__global__ void diag_inv_kernel(int n, const double* L, int64_t ldl, double* Di, const int* info);
+
+// A21 <- A21 L11^-T in place by substitution (the rows are independent):
+// one wavefront per 64 rows, lane = row, the row in registers; for k = 0..63
+// x_k *= 1 / L_kk, then x_j -= x_k L_jk for j > k (BLAS dtrsm, right side,
+// which also scales by the reciprocal).  Column k of L11 is read from LDS in
+// chunks of 16 whose loads are issued together.  The step guard (always true
+// for the 64-wide panels) ends a basic block per step: without it the
+// scheduler hoists the LDS reads of later steps and the row spills.
+template <int K>
+__device__ __attribute__((always_inline)) inline void chol_panel_step(const double* Lt,
+                                                                      const double* rd, int nb,
+                                                                      double (&x)[64]) {
+  if (K >= nb) return;
+  x[K] *= rd[K];
+#pragma unroll
+  for (int g = ((K + 1) / 16) * 16; g < 64; g += 16) {
+    double lc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (g + q > K) lc[q] = Lt[K * W64_LD + g + q];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (g + q > K) x[g + q] -= x[K] * lc[q];
+  }
+  if constexpr (K + 1 < 64) chol_panel_step<K + 1>(Lt, rd, nb, x);
+}
+
+__global__ __launch_bounds__(64) void chol_panel_kernel(int rows, double* A21, int64_t lda,
+                                                        const double* L11, const int* info, int jb) {
   if (*info != 0) return;
-  __shared__ double T[64][65];
-  __shared__ double U[64][65];
-  const int r0 = blockIdx.x * 64;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e & 63, k = e >> 6;
-    T[r][k] = r0 + r < rows ? A21[(r0 + r) + (int64_t)k * lda] : 0.0;
-    U[r][k] = dinv[r + 64 * k];
-  }
-  __syncthreads();
-  const int rr = tid & 15, cc = tid >> 4;
-  double acc[4][4] = {};
-#pragma unroll 8
-  for (int k = 0; k < 64; ++k) {
-    double tv[4], uv[4];
+  __shared__ double Lt[64 * W64_LD];  // L11[j][k] at Lt[k * W64_LD + j]
+  __shared__ double rd[64];
+  const int lane = threadIdx.x;
+  const int r = blockIdx.x * 64 + lane;
+  double x[64];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { tv[q] = T[rr + 16 * q][k]; uv[q] = U[cc + 16 * q][k]; }
+  for (int k = 0; k < 64; ++k) x[k] = lane >= k ? L11[lane + (int64_t)k * lda] : 0.0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+  for (int k = 0; k < 64; ++k) Lt[k * W64_LD + lane] = x[k];
+  // the panel's loads are in flight while the reciprocals are formed
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] += tv[a] * uv[b];
-  }
+  for (int k = 0; k < 64; ++k) x[k] = r < rows ? A21[r + (int64_t)k * lda] : 0.0;
+  wave_lds_sync();
+  rd[lane] = 1.0 / Lt[lane * W64_LD + lane];
+  wave_lds_sync();
+  chol_panel_step<0>(Lt, rd, jb, x);
+  if (r >= rows) return;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int r = r0 + rr + 16 * a;
-    if (r >= rows) continue;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) A21[r + (int64_t)(cc + 16 * b) * lda] = acc[a][b];
-  }
+  for (int k = 0; k < 64; ++k) A21[r + (int64_t)k * lda] = x[k];
 }
 
 size_t potrf_work_elems(int n) { return (size_t)cdiv(std::max(n, 1), 64) * 4096; }
 
-int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work) {
+int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work, bool inverses) {
   MSDP_TRY(rec_forbid("potrf"));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int jb = std::min(64, n - j0);
     double* A11 = A + j0 + (int64_t)j0 * lda;
-    double* dinv = work + (size_t)(j0 / 64) * 4096;
-    chol_diag_kernel<<<1, 64, 0, s>>>(jb, A11, lda, 0, info, j0, 0, dinv);
+    chol_diag_kernel<<<1, 64, 0, s>>>(jb, A11, lda, 0, info, j0, 0, nullptr);
     MSDP_HIP(hipGetLastError());
     const int rest = n - j0 - jb;
-    if (rest > 0) {
+    if (rest > 0) {  // jb == 64 here
       double* A21 = A11 + jb;
-      chol_panel_kernel<<<cdiv(rest, 64), 256, 0, s>>>(rest, A21, lda, dinv, info);
+      chol_panel_kernel<<<cdiv(rest, 64), 64, 0, s>>>(rest, A21, lda, A11, info, jb);
       MSDP_HIP(hipGetLastError());
       double* A22 = A21 + (int64_t)jb * lda;
       MSDP_TRY(gemm_guarded(s, false, true, rest, rest, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
                             1, 0, 0, 0, true, info));
     }
+  }
+  if (inverses) {  // all diagonal-block inverses in one launch
+    diag_inv_kernel<<<cdiv(n, 64), 64, 0, s>>>(n, A, lda, work, info);
+    MSDP_HIP(hipGetLastError());
   }
   return 0;
 }
@@ -748,6 +772,7 @@ __device__ __attribute__((always_inline)) inline void trtri64_step(const double*
   if constexpr (K + 1 < 64) trtri64_step<K + 1>(T, nb, x);
 }
 
+template <bool DENSE64 = false>
 __device__ __attribute__((always_inline)) inline void trtri64_body(int n, const double* L,
                                                                    int64_t ldl, double* Li,
                                                                    int64_t ldi, int b0) {
@@ -772,10 +797,26 @@ __device__ __attribute__((always_inline)) inline void trtri64_body(int n, const 
 #pragma unroll
   for (int i = 0; i < 64; ++i) T[i * W64_LD + lane] = x[i];
   wave_lds_sync();
+  if constexpr (DENSE64) {  // the whole 64 x 64 tile, zeros outside the nb x nb block
+#pragma unroll 8
+    for (int c = 0; c < 64; ++c)
+      Li[lane + (int64_t)c * ldi] = (lane < nb && c < nb) ? T[lane * W64_LD + c] : 0.0;
+    return;
+  }
   if (lane >= nb) return;
 #pragma unroll 8
   for (int c = 0; c < 64; ++c)
     if (c < nb) Li[(b0 + lane) + (int64_t)(b0 + c) * ldi] = T[lane * W64_LD + c];
+}
+
+// inverses of the diagonal 64-blocks of a Cholesky factor, block b into
+// Di + 4096 b (64 x 64, ld 64, zeros outside the lower triangle of the block)
+__global__ __launch_bounds__(64) void diag_inv_kernel(int n, const double* L, int64_t ldl,
+                                                      double* Di, const int* info) {
+  if (*info != 0) return;
+  const int b0 = blockIdx.x * 64;
+  trtri64_body<true>(n - b0, L + b0 + (int64_t)b0 * ldl, ldl, Di + (size_t)blockIdx.x * 4096, 64,
+                     0);
 }
 
 // the diagonal 64-blocks (blockIdx.x) of a batch of matrices (blockIdx.y)

@@ -1678,13 +1678,20 @@ __device__ __attribute__((always_inline)) inline void larft_step(int nb, const d
   if constexpr (I + 1 < 64) larft_step<I + 1>(nb, Gs, taur, tr);
 }
 
-__global__ __launch_bounds__(64) void larft_kernel(int nb, const double* G, const double* tau,
-                                                   double* T) {
+// One workgroup per panel of 64 reflectors (the last may be narrower): G and
+// T of panel p at G + 4096 p, T + 4096 p, leading dimension 64.
+__global__ __launch_bounds__(64) void larft_kernel(int nref, const double* Gall,
+                                                   const double* tauall, double* Tall) {
   __shared__ double Gs[64 * 65];  // column i of G at Gs[i * 65 + k]
   const int r = threadIdx.x;
+  const int k0 = blockIdx.x * 64;
+  const int nb = min(64, nref - k0);
+  const double* G = Gall + (int64_t)blockIdx.x * 4096;
+  const double* tau = tauall + k0;
+  double* T = Tall + (int64_t)blockIdx.x * 4096;
   double tr[64];
 #pragma unroll
-  for (int i = 0; i < 64; ++i) tr[i] = (r < nb && i < nb) ? G[r + (int64_t)i * nb] : 0.0;
+  for (int i = 0; i < 64; ++i) tr[i] = (r < nb && i < nb) ? G[r + (int64_t)i * 64] : 0.0;
   const double taur = r < nb ? tau[r] : 0.0;
 #pragma unroll
   for (int i = 0; i < 64; ++i) Gs[i * 65 + r] = tr[i];
@@ -1695,7 +1702,17 @@ __global__ __launch_bounds__(64) void larft_kernel(int nb, const double* G, cons
   if (r >= nb) return;
 #pragma unroll
   for (int i = 0; i < 64; ++i)
-    if (i < nb) T[r + (int64_t)i * nb] = tr[i];
+    if (i < nb) T[r + (int64_t)i * 64] = tr[i];
+}
+
+// Panels of 64 reflectors, applied last to first.  The Gram matrices
+// G_p = V_p' V_p of all panels are one batched product and their T factors
+// one launch, before the sequential updates of Z.  Column k of V is zero in
+// rows <= k, so every full panel's Gram is taken over rows 1..n-1 (the extra
+// rows add exact zeros).
+static size_t ormtr_work_elems(int n) {
+  const size_t npanel = (size_t)cdiv(std::max(n - 2, 1), 64);
+  return 2 * 4096 * npanel + 2 * 64 * (size_t)n;
 }
 
 static int ormtr(hipStream_t s, int n, const double* V, const double* tau, double* Z, int64_t ldz,
@@ -1703,22 +1720,32 @@ static int ormtr(hipStream_t s, int n, const double* V, const double* tau, doubl
   const int nref = n - 2;
   if (nref <= 0) return 0;
   const int nb = 64;
-  double* G = work;                       // nb x nb
-  double* T = G + nb * nb;                // nb x nb
-  double* W1 = T + nb * nb;               // nb x n
-  double* W2 = W1 + (size_t)nb * n;       // nb x n
   const int npanel = cdiv(nref, nb);
+  double* G = work;                          // npanel x (64 x 64)
+  double* T = G + (size_t)4096 * npanel;     // npanel x (64 x 64)
+  double* W1 = T + (size_t)4096 * npanel;    // nb x n
+  double* W2 = W1 + (size_t)nb * n;          // nb x n
+  const int nfull = nref / nb;
+  if (nfull > 0)
+    MSDP_TRY(gemm(s, true, false, nb, nb, n - 1, 1.0, V + 1, n, V + 1, n, 0.0, G, nb, nfull,
+                  (int64_t)nb * n, (int64_t)nb * n, 4096));
+  if (nfull < npanel) {
+    const int k0 = nfull * nb, kb = nref - k0, r0 = k0 + 1;
+    const double* Vp = V + r0 + (int64_t)k0 * n;
+    MSDP_TRY(gemm(s, true, false, kb, kb, n - r0, 1.0, Vp, n, Vp, n, 0.0,
+                  G + (size_t)4096 * nfull, nb));
+  }
+  larft_kernel<<<npanel, 64, 0, s>>>(nref, G, tau, T);
+  MSDP_HIP(hipGetLastError());
   for (int pi = npanel - 1; pi >= 0; --pi) {
     const int k0 = pi * nb;
     const int kb = std::min(nb, nref - k0);
     const int r0 = k0 + 1;                // first row touched
     const int rows = n - r0;
     const double* Vp = V + r0 + (int64_t)k0 * n;  // rows x kb, ld n
-    MSDP_TRY(gemm(s, true, false, kb, kb, rows, 1.0, Vp, n, Vp, n, 0.0, G, kb));
-    larft_kernel<<<1, 64, 0, s>>>(kb, G, tau + k0, T);
-    MSDP_HIP(hipGetLastError());
+    const double* Tp = T + (size_t)4096 * pi;
     MSDP_TRY(gemm(s, true, false, kb, n, rows, 1.0, Vp, n, Z + r0, ldz, 0.0, W1, kb));
-    MSDP_TRY(gemm(s, false, false, kb, n, kb, 1.0, T, kb, W1, kb, 0.0, W2, kb));
+    MSDP_TRY(gemm(s, false, false, kb, n, kb, 1.0, Tp, nb, W1, kb, 0.0, W2, kb));
     MSDP_TRY(gemm(s, false, false, rows, n, kb, -1.0, Vp, n, W2, kb, 1.0, Z + r0, ldz));
   }
   return 0;
@@ -1729,8 +1756,8 @@ static int ormtr(hipStream_t s, int n, const double* V, const double* tau, doubl
 // ==========================================================================
 size_t eig_min_work_elems(int n);
 size_t syevd_work_elems(int n) {
-  return std::max(trd_work_elems(n) + dc_work_elems(n) + (size_t)n * n + 2 * 64 * 64 +
-                      2 * 64 * (size_t)n + 64,
+  return std::max(trd_work_elems(n) + (size_t)n * n +
+                      std::max(dc_work_elems(n), ormtr_work_elems(n)) + 64,
                   eig_min_work_elems(n));
 }
 size_t syevd_iwork_elems(int n) { return dc_iwork_elems(n); }

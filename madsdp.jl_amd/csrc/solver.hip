@@ -963,7 +963,7 @@ int msdp_solver::factorize_schur(bool& ok) {
   for (int t = 0; t < opt.max_schur_reg; ++t) {
     MSDP_TRY(copy_mat(st, (int)m, (int)m, Hs, m, H, m));
     if (reg > 0) MSDP_TRY(add_diag(st, (int)m, H, m, nullptr, reg));
-    MSDP_TRY(potrf(st, (int)m, H, m, info, Hdinv));
+    MSDP_TRY(potrf(st, (int)m, H, m, info, Hdinv, true));
     int hinfo = 0;
     MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
     MSDP_TRY(stream_sync(st));
