@@ -233,6 +233,12 @@ size_t msdp_k_syevd_iwork(int32_t n);
 /* smallest eigenvalue of a symmetric matrix into *out_dev (LRO.lanczos_eigmin!) */
 int msdp_k_eig_min(void* stream, int32_t n, const double* A, int64_t lda, double* out_dev,
                    double* work);
+/* the primal and dual step eigenvalues of EigenStep (src/step.jl:180-184: lambda_min
+ * of L_X^-1 dX L_X^-T and of L_S^-1 dS L_S^-T) in the same launches: lambda_min(A0)
+ * into *out0_dev, lambda_min(A1) into *out1_dev, each bit-identical to msdp_k_eig_min;
+ * work holds msdp_k_syevd_work(n) doubles */
+int msdp_k_eig_min2(void* stream, int32_t n, const double* A0, const double* A1, int64_t lda,
+                    double* out0_dev, double* out1_dev, double* work);
 /* C(FP64) = alpha op(A) op(B) + beta C with FP32 A, B on FP32 MFMA (the
  * mixed-precision Schur assembly product); no counterpart in the reference */
 int msdp_k_gemm32(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,

@@ -184,6 +184,11 @@ size_t syevd_iwork_elems(int n);
 // Smallest eigenvalue of the symmetric matrix A into *out_dev.
 int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out_dev,
             double* work, int* iwork);
+// The smallest eigenvalues of two symmetric matrices of the same side in the
+// same launches; work holds 2 eig_min_work_elems(n) (syevd_work_elems(n)).
+int eig_min2(hipStream_t s, int n, const double* A0, const double* A1, int64_t lda,
+             double* out0, double* out1, double* work);
+size_t eig_min_work_elems(int n);
 // One eigenproblem of a variable-size batch (n <= 128): eigenvalues into w
 // (ascending), optionally w[0] also into *wmin, vectors into V (ld ldv) with
 // Vtmp (n x n) as scratch.

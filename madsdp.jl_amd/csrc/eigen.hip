@@ -631,9 +631,16 @@ __device__ inline void trd1_pass(int n, int c0, double* A, const double* v, cons
   __syncthreads();
 }
 
+// Workgroup 1 (when launched) reduces Ain1 into the work arrays wstride
+// elements further on: two independent matrices in one launch.
 __global__ __launch_bounds__(TRD1_NT) void trd1_kernel(int n, const double* Ain, int64_t lda,
                                                        double* A, double* V, double* tau,
-                                                       double* d, double* e) {
+                                                       double* d, double* e, const double* Ain1,
+                                                       int64_t wstride) {
+  if (blockIdx.x) {
+    Ain = Ain1;
+    A += wstride; V += wstride; tau += wstride; d += wstride; e += wstride;
+  }
   __shared__ double vs[TRD1_MAX], ws[TRD1_MAX], vn[TRD1_MAX], ps[TRD1_MAX];
   __shared__ double part[TRD1_W][TRD1_MAX];
   __shared__ double red[16];
@@ -692,7 +699,7 @@ __global__ __launch_bounds__(TRD1_NT) void trd1_kernel(int n, const double* Ain,
 
 static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWork& t) {
   if (n <= TRD1_MAX) {
-    trd1_kernel<<<1, TRD1_NT, 0, s>>>(n, A, lda, t.A, t.V, t.tau, t.d, t.e);
+    trd1_kernel<<<1, TRD1_NT, 0, s>>>(n, A, lda, t.A, t.V, t.tau, t.d, t.e, A, 0);
     MSDP_HIP(hipGetLastError());
     return 0;
   }
@@ -768,8 +775,12 @@ __device__ inline int sturm_count(int n, const double* d, const double* e, doubl
   return cnt;
 }
 
+// workgroup 1 (when launched): the tridiagonal ws elements further on, into out1
 __global__ __launch_bounds__(256) void tridiag_min_kernel(int n, const double* d, const double* e,
-                                                          double* out) {
+                                                          double* out, int64_t ws, double* out1) {
+  if (blockIdx.x) {
+    d += ws; e += ws; out = out1;
+  }
   __shared__ double red[4];
   __shared__ double lo_s, hi_s;
   __shared__ int first;
@@ -1758,7 +1769,7 @@ size_t eig_min_work_elems(int n);
 size_t syevd_work_elems(int n) {
   return std::max(trd_work_elems(n) + (size_t)n * n +
                       std::max(dc_work_elems(n), ormtr_work_elems(n)) + 64,
-                  eig_min_work_elems(n));
+                  2 * eig_min_work_elems(n));  // room for eig_min2
 }
 size_t syevd_iwork_elems(int n) { return dc_iwork_elems(n); }
 
@@ -1810,8 +1821,13 @@ __device__ inline double lz_hash(int64_t i) {
   return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-__global__ __launch_bounds__(1024) void lz_init_kernel(int n, double* q) {
+// The multi-workgroup kernels below run one Lanczos process per instance:
+// instance y = blockIdx.y (blockIdx.x for lz_init) + inst0, whose work
+// arrays lie y * ws elements after those passed (and whose matrix is A1
+// for y = 1), so that two step-length matrices share every launch.
+__global__ __launch_bounds__(1024) void lz_init_kernel(int n, double* q, int64_t ws, int inst0) {
   __shared__ double red[16];
+  q += (blockIdx.x + inst0) * ws;
   double ss = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const double v = lz_hash(i);
@@ -1824,9 +1840,14 @@ __global__ __launch_bounds__(1024) void lz_init_kernel(int n, double* q) {
 }
 
 // w[c] = sum_i A[i, c] q[i]  (A symmetric), one workgroup per column
-__global__ __launch_bounds__(256) void lz_gemv_kernel(int n, const double* A, int64_t lda,
-                                                      const double* q, double* w) {
+__global__ __launch_bounds__(256) void lz_gemv_kernel(int n, const double* A, const double* A1,
+                                                      int64_t lda, const double* q, double* w,
+                                                      int64_t ws, int inst0) {
   __shared__ double red[4];
+  const int y = blockIdx.y + inst0;
+  if (y) A = A1;
+  q += y * ws;
+  w += y * ws;
   const int c = blockIdx.x;
   const double* col = A + (int64_t)c * lda;
   double v = 0.0;
@@ -1837,8 +1858,10 @@ __global__ __launch_bounds__(256) void lz_gemv_kernel(int n, const double* A, in
 
 // h[c] = Q[:, c]' w, one workgroup per column
 __global__ __launch_bounds__(256) void lz_dots_kernel(int n, const double* Q, const double* w,
-                                                      double* h) {
+                                                      double* h, int64_t ws, int inst0) {
   __shared__ double red[4];
+  const int64_t off = (blockIdx.y + inst0) * ws;
+  Q += off; w += off; h += off;
   const int c = blockIdx.x;
   const double* col = Q + (int64_t)c * n;
   double v = 0.0;
@@ -1850,8 +1873,11 @@ __global__ __launch_bounds__(256) void lz_dots_kernel(int n, const double* Q, co
 // w -= Q[:, 0:ncols] h: 64 rows per workgroup, 4 lanes per row split the
 // columns, partial sums combined in LDS.
 __global__ __launch_bounds__(256) void lz_update_kernel(int n, int ncols, const double* Q,
-                                                        const double* h, double* w) {
+                                                        const double* h, double* w, int64_t ws,
+                                                        int inst0) {
   __shared__ double part[4][64];
+  const int64_t off = (blockIdx.y + inst0) * ws;
+  Q += off; h += off; w += off;
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + r;
   double v = 0.0;
@@ -1865,8 +1891,11 @@ __global__ __launch_bounds__(256) void lz_update_kernel(int n, int ncols, const 
 // alpha_j = h1[j] + h2[j], beta_j = ||w||, q_{j+1} = w / beta_j
 __global__ __launch_bounds__(1024) void lz_finish_kernel(int n, int j, const double* w,
                                                          const double* h1, const double* h2,
-                                                         double* alpha, double* beta, double* qn) {
+                                                         double* alpha, double* beta, double* qn,
+                                                         int64_t ws, int inst0) {
   __shared__ double red[16];
+  const int64_t off = (blockIdx.y + inst0) * ws;
+  w += off; h1 += off; h2 += off; alpha += off; beta += off; qn += off;
   double ss = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) ss += w[i] * w[i];
   ss = block_reduce<0>(ss, red);
@@ -1991,13 +2020,15 @@ __device__ void lz_ritz_min(int k, const double* alpha, const double* beta, doub
 
 __global__ __launch_bounds__(256) void lz_check_kernel(int k, const double* alpha_g,
                                                        const double* beta_g, double* scratch_g,
-                                                       double* out) {
+                                                       double* out, int64_t ws, int inst0) {
   __shared__ double red[4];
   // the tridiagonal and the inverse-iteration arrays live in LDS (k <= 1024)
   __shared__ double alpha[1024], beta[1024];
   __shared__ double scratch[7 * 1024];
   __shared__ LzShared sh;
   (void)scratch_g;
+  const int64_t off = (blockIdx.y + inst0) * ws;
+  alpha_g += off; beta_g += off; out += off;
   for (int i = threadIdx.x; i < k; i += blockDim.x) {
     alpha[i] = alpha_g[i];
     beta[i] = beta_g[i];
@@ -2016,8 +2047,13 @@ __global__ __launch_bounds__(256) void lz_check_kernel(int k, const double* alph
 constexpr int LZ1_MAX = 640;
 constexpr int LZ1_NT = 1024;
 
+// workgroup 1 (when launched): A1, its work ws elements further on, into out1
 __global__ __launch_bounds__(LZ1_NT) void lz1_kernel(int n, const double* A, int64_t lda,
-                                                    double* Q, double* out) {
+                                                    double* Q, double* out, const double* A1,
+                                                    int64_t ws, double* out1) {
+  if (blockIdx.x) {
+    A = A1; Q += ws; out = out1;
+  }
   __shared__ double w[LZ1_MAX], q[LZ1_MAX], h[LZ1_MAX + 1];
   __shared__ double alpha[LZ1_MAX], beta[LZ1_MAX];
   __shared__ double scratch[7 * LZ1_MAX];
@@ -2107,8 +2143,11 @@ __global__ __launch_bounds__(LZ1_NT) void lz1_kernel(int n, const double* A, int
   if (tid == 0) out[0] = theta;
 }
 
-static int lanczos_min(hipStream_t s, int n, const double* A, int64_t lda, double* out,
-                       double* work) {
+// count (1 or 2) Lanczos processes in the same launches: matrix A0 (and A1)
+// into out0 (and out1), instance y's work at work + y * ws.  A process that
+// has converged drops out of the launches.
+static int lanczos_min(hipStream_t s, int n, const double* A0, const double* A1, int64_t lda,
+                       double* out0, double* out1, double* work, int64_t ws, int count) {
   const int kmax = std::min(n, 1024);
   double* Q = work;                                  // n x (kmax + 1)
   double* w = Q + (size_t)n * (kmax + 1);            // n
@@ -2118,31 +2157,47 @@ static int lanczos_min(hipStream_t s, int n, const double* A, int64_t lda, doubl
   double* be = al + kmax;                            // kmax
   double* chk = be + kmax;                           // 4
   double* scr = chk + 4;                             // 7 kmax
-  lz_init_kernel<<<1, 1024, 0, s>>>(n, Q);
+  bool active[2] = {true, count > 1};
+  double* outs[2] = {out0, out1};
+  double anorm_est[2] = {0.0, 0.0};
+  lz_init_kernel<<<count, 1024, 0, s>>>(n, Q, ws, 0);
   MSDP_HIP(hipGetLastError());
-  double hv[3] = {0, 0, 0};
-  double anorm_est = 0.0;
   for (int j = 0; j < kmax; ++j) {
+    const int inst0 = active[0] ? 0 : 1;
+    const int ny = (active[0] && active[1]) ? 2 : 1;
     const double* qj = Q + (size_t)j * n;
-    lz_gemv_kernel<<<n, 256, 0, s>>>(n, A, lda, qj, w);
-    lz_dots_kernel<<<j + 1, 256, 0, s>>>(n, Q, w, h1);
-    lz_update_kernel<<<cdiv(n, 64), 256, 0, s>>>(n, j + 1, Q, h1, w);
-    lz_dots_kernel<<<j + 1, 256, 0, s>>>(n, Q, w, h2);
-    lz_update_kernel<<<cdiv(n, 64), 256, 0, s>>>(n, j + 1, Q, h2, w);
-    lz_finish_kernel<<<1, 1024, 0, s>>>(n, j, w, h1, h2, al, be, Q + (size_t)(j + 1) * n);
+    lz_gemv_kernel<<<dim3(n, ny), 256, 0, s>>>(n, A0, A1, lda, qj, w, ws, inst0);
+    lz_dots_kernel<<<dim3(j + 1, ny), 256, 0, s>>>(n, Q, w, h1, ws, inst0);
+    lz_update_kernel<<<dim3(cdiv(n, 64), ny), 256, 0, s>>>(n, j + 1, Q, h1, w, ws, inst0);
+    lz_dots_kernel<<<dim3(j + 1, ny), 256, 0, s>>>(n, Q, w, h2, ws, inst0);
+    lz_update_kernel<<<dim3(cdiv(n, 64), ny), 256, 0, s>>>(n, j + 1, Q, h2, w, ws, inst0);
+    lz_finish_kernel<<<dim3(1, ny), 1024, 0, s>>>(n, j, w, h1, h2, al, be,
+                                                  Q + (size_t)(j + 1) * n, ws, inst0);
     MSDP_HIP(hipGetLastError());
     const int k = j + 1;
     if (k % LZ_CHECK == 0 || k == kmax) {
-      lz_check_kernel<<<1, 256, 0, s>>>(k, al, be, scr, chk);
+      lz_check_kernel<<<dim3(1, ny), 256, 0, s>>>(k, al, be, scr, chk, ws, inst0);
       MSDP_HIP(hipGetLastError());
-      MSDP_HIP(hipMemcpyAsync(hv, chk, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+      double hv[2][3] = {{0, 0, 0}, {0, 0, 0}};
+      for (int y = 0; y < 2; ++y)
+        if (active[y])
+          MSDP_HIP(hipMemcpyAsync(hv[y], chk + y * ws, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                  s));
       MSDP_TRY(stream_sync(s));
-      anorm_est = std::max(anorm_est, std::fabs(hv[0]));
-      const bool invariant = hv[2] <= 1e-14 * std::max(anorm_est, 1e-300);
-      if (hv[1] <= LZ_RTOL * std::max(std::fabs(hv[0]), 1e-300) || invariant || k == n) break;
+      for (int y = 0; y < 2; ++y) {
+        if (!active[y]) continue;
+        anorm_est[y] = std::max(anorm_est[y], std::fabs(hv[y][0]));
+        const bool invariant = hv[y][2] <= 1e-14 * std::max(anorm_est[y], 1e-300);
+        if (hv[y][1] <= LZ_RTOL * std::max(std::fabs(hv[y][0]), 1e-300) || invariant || k == n ||
+            k == kmax) {
+          MSDP_HIP(hipMemcpyAsync(outs[y], chk + y * ws, sizeof(double), hipMemcpyDeviceToDevice,
+                                  s));
+          active[y] = false;
+        }
+      }
+      if (!active[0] && !active[1]) break;
     }
   }
-  MSDP_HIP(hipMemcpyAsync(out, chk, sizeof(double), hipMemcpyDeviceToDevice, s));
   return 0;
 }
 
@@ -2151,26 +2206,47 @@ size_t eig_min_work_elems(int n) {
   return std::max(trd_work_elems(n), (size_t)n * (kmax + 2) + 12 * kmax + 64);
 }
 
-int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out, double* work, int*) {
-  MSDP_TRY(rec_forbid("eig_min"));
+// count (1 or 2) matrices of side n: instance y's work at work + y * ws,
+// ws = eig_min_work_elems(n); the two share every launch.
+static int eig_min_count(hipStream_t s, int n, const double* A0, const double* A1, int64_t lda,
+                         double* out0, double* out1, double* work, int count) {
   if (n <= 0) return 0;
+  const int64_t ws = (int64_t)eig_min_work_elems(n);
+  if (count == 1) {
+    A1 = A0;
+    out1 = out0;
+  }
   if (n <= 64) {
-    double* wv = work;
-    MSDP_TRY(jacobi_batched(s, n, A, lda, 0, wv, 0, nullptr, 0, 0, 1, false));
-    MSDP_HIP(hipMemcpyAsync(out, wv, sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int y = 0; y < count; ++y) {
+      double* wv = work + y * ws;
+      MSDP_TRY(jacobi_batched(s, n, y ? A1 : A0, lda, 0, wv, 0, nullptr, 0, 0, 1, false));
+      MSDP_HIP(hipMemcpyAsync(y ? out1 : out0, wv, sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     return 0;
   }
   if (n >= LZ_MIN_N && n <= LZ1_MAX) {
-    lz1_kernel<<<1, LZ1_NT, 0, s>>>(n, A, lda, work, out);
+    lz1_kernel<<<count, LZ1_NT, 0, s>>>(n, A0, lda, work, out0, A1, ws, out1);
     MSDP_HIP(hipGetLastError());
     return 0;
   }
-  if (n >= LZ_MIN_N) return lanczos_min(s, n, A, lda, out, work);
+  if (n >= LZ_MIN_N) return lanczos_min(s, n, A0, A1, lda, out0, out1, work, ws, count);
+  // n < LZ_MIN_N <= TRD1_MAX: the one-workgroup tridiagonalisation, then Sturm
   TrdWork t = trd_carve(n, work);
-  MSDP_TRY(sytrd(s, n, A, lda, t));
-  tridiag_min_kernel<<<1, 256, 0, s>>>(n, t.d, t.e, out);
+  trd1_kernel<<<count, TRD1_NT, 0, s>>>(n, A0, lda, t.A, t.V, t.tau, t.d, t.e, A1, ws);
+  tridiag_min_kernel<<<count, 256, 0, s>>>(n, t.d, t.e, out0, ws, out1);
   MSDP_HIP(hipGetLastError());
   return 0;
+}
+
+int eig_min(hipStream_t s, int n, const double* A, int64_t lda, double* out, double* work, int*) {
+  MSDP_TRY(rec_forbid("eig_min"));
+  return eig_min_count(s, n, A, nullptr, lda, out, nullptr, work, 1);
+}
+
+int eig_min2(hipStream_t s, int n, const double* A0, const double* A1, int64_t lda, double* out0,
+             double* out1, double* work) {
+  MSDP_TRY(rec_forbid("eig_min2"));
+  return eig_min_count(s, n, A0, A1, lda, out0, out1, work, 2);
 }
 
 }  // namespace msdp

@@ -1102,11 +1102,11 @@ int msdp_solver::step_lengths(double& a, double& bt) {
         MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
         MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n));
         MSDP_TRY(symmetrize(st, n, T[k], n));
-        if (n > JACOBI_VB_MAX) {
-          double* Tk = T[k];
-          double* out = slot(bslot(i, 3 + k));
-          MSDP_TRY(opaque([=] { return eig_min(st, n, Tk, n, out, ework, eiwork); }));
-        }
+      }
+      if (n > JACOBI_VB_MAX) {  // the primal and dual eigenvalues in the same launches
+        double *T0 = T[0], *T1 = T[1];
+        double *o0 = slot(bslot(i, 3)), *o1 = slot(bslot(i, 4));
+        MSDP_TRY(opaque([=] { return eig_min2(st, n, T0, T1, n, o0, o1, ework); }));
       }
     }
     MSDP_TRY(rec_end());
@@ -2207,6 +2207,10 @@ int msdp_k_syevd(void* stream, int32_t n, const double* A, int64_t lda, double* 
 }
 int msdp_k_eig_min(void* stream, int32_t n, const double* A, int64_t lda, double* out, double* work) {
   return eig_min((hipStream_t)stream, n, A, lda, out, work, nullptr);
+}
+int msdp_k_eig_min2(void* stream, int32_t n, const double* A0, const double* A1, int64_t lda,
+                    double* out0, double* out1, double* work) {
+  return eig_min2((hipStream_t)stream, n, A0, A1, lda, out0, out1, work);
 }
 int msdp_k_gemm32(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K,
                   float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,

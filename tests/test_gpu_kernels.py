@@ -212,6 +212,33 @@ def test_eig_min(env, n, kind):
     assert abs(out.item() - ref) <= 1e-12 * max(1, np.abs(A).max() * np.sqrt(n))
 
 
+@pytest.mark.parametrize("n", [100, 250, 500, 700, 1100])
+def test_eig_min2(env, n):
+    """The primal/dual pair in shared launches: each result is the single
+    call's, bit for bit, and the smallest eigenvalue to 1e-12 (the two
+    Lanczos processes of n > 640 converge at different steps)."""
+    torch, lib, st = env
+    rng = np.random.default_rng(n + 11)
+    A0 = rng.standard_normal((n, n))
+    A0 = (A0 + A0.T) / 2
+    G = rng.standard_normal((n, 10))
+    A1 = G @ G.T / n - 0.5 * np.eye(n) + np.diag(rng.uniform(0, 2, n))
+    A1 = (A1 + A1.T) / 2
+    d0, d1 = dev(torch, A0), dev(torch, A1)
+    outs = torch.zeros(4, dtype=torch.float64, device="cuda")
+    work = torch.zeros(int(lib.msdp_k_syevd_work(n)), dtype=torch.float64, device="cuda")
+    base = outs.data_ptr()
+    assert lib.msdp_k_eig_min2(st, n, P(d0), P(d1), n, base, base + 8, P(work)) == 0
+    assert lib.msdp_k_eig_min(st, n, P(d0), n, base + 16, P(work)) == 0
+    assert lib.msdp_k_eig_min(st, n, P(d1), n, base + 24, P(work)) == 0
+    torch.cuda.synchronize()
+    o = outs.cpu().numpy()
+    assert o[0] == o[2] and o[1] == o[3], o
+    for v, A in ((o[0], A0), (o[1], A1)):
+        ref = np.linalg.eigvalsh(A)[0]
+        assert abs(v - ref) <= 1e-12 * max(1, np.abs(A).max() * np.sqrt(n))
+
+
 def test_schur_onehot(env):
     """Max-cut Schur complement H = W o W (lower), src/components.jl:102-104."""
     torch, lib, st = env
