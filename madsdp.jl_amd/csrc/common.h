@@ -127,6 +127,9 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
 // factor's diagonal 64-blocks (block b at work + 4096 b, 64 x 64, ld 64).
 int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info_dev, double* work,
           bool inverses = false);
+// Two independent factorisations of the same side in the same launches:
+// info_dev[0] and info_dev[1] receive the two statuses.
+int potrf2(hipStream_t s, int n, double* A0, double* A1, int64_t lda, int* info_dev);
 size_t potrf_work_elems(int n);
 // Batched Cholesky of small matrices (n <= 64), one workgroup each.
 int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t stride,

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(
     int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
     int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
-  if (guard && *guard != 0) return;
+  if (guard && guard[blockIdx.z] != 0) return;  // one guard per batch entry
   const int64_t bz = blockIdx.z;
   gemm64_tile<TA, TB>(M, N, K, alpha, A + bz * sA, lda, B + bz * sB, ldb, beta, C + bz * sC, ldc,
                       lower, blockIdx.x, blockIdx.y);
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
     int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
     int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
-  if (guard && *guard != 0) return;
+  if (guard && guard[blockIdx.z] != 0) return;  // one guard per batch entry
   int tm = blockIdx.x, tn = blockIdx.y;
   if (lower == 2) {  // square lower triangle on a 1-D, XCD-grouped grid
     if (!lower_tile_xcd((M + BT - 1) / BT, tm, tn)) return;
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
                                                        int64_t stride, int* info, int info_base,
                                                        int per_batch_info, double* dinv) {
   int* inf = info + (per_batch_info ? blockIdx.x : 0);
-  if (!per_batch_info && *inf != 0) return;
+  if (*inf != 0) return;  // an earlier block of this matrix failed
   A += (int64_t)blockIdx.x * stride;
   __shared__ double Lc[64 * W64_LD];
   const int lane = threadIdx.x;
@@ -687,7 +687,11 @@ __device__ __attribute__((always_inline)) inline void chol_panel_step(const doub
 }
 
 __global__ __launch_bounds__(64) void chol_panel_kernel(int rows, double* A21, int64_t lda,
-                                                        const double* L11, const int* info, int jb) {
+                                                        const double* L11, const int* info, int jb,
+                                                        int64_t sA) {
+  A21 += blockIdx.y * sA;
+  L11 += blockIdx.y * sA;
+  info += blockIdx.y;
   if (*info != 0) return;
   __shared__ double Lt[64 * W64_LD];  // L11[j][k] at Lt[k * W64_LD + j]
   __shared__ double rd[64];
@@ -712,29 +716,47 @@ __global__ __launch_bounds__(64) void chol_panel_kernel(int rows, double* A21, i
 
 size_t potrf_work_elems(int n) { return (size_t)cdiv(std::max(n, 1), 64) * 4096; }
 
-int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work, bool inverses) {
-  MSDP_TRY(rec_forbid("potrf"));
-  MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+// count (1 or 2) matrices: the second at A + sA, its info at info[1]
+static int potrf_count(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
+                       int* info) {
+  MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * count, s));
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int jb = std::min(64, n - j0);
     double* A11 = A + j0 + (int64_t)j0 * lda;
-    chol_diag_kernel<<<1, 64, 0, s>>>(jb, A11, lda, 0, info, j0, 0, nullptr);
+    chol_diag_kernel<<<count, 64, 0, s>>>(jb, A11, lda, sA, info, j0, 1, nullptr);
     MSDP_HIP(hipGetLastError());
     const int rest = n - j0 - jb;
     if (rest > 0) {  // jb == 64 here
       double* A21 = A11 + jb;
-      chol_panel_kernel<<<cdiv(rest, 64), 64, 0, s>>>(rest, A21, lda, A11, info, jb);
+      chol_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(rest, A21, lda, A11, info, jb,
+                                                                   sA);
       MSDP_HIP(hipGetLastError());
       double* A22 = A21 + (int64_t)jb * lda;
       MSDP_TRY(gemm_guarded(s, false, true, rest, rest, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
-                            1, 0, 0, 0, true, info));
+                            count, sA, sA, sA, true, info));
     }
   }
+  return 0;
+}
+
+int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work, bool inverses) {
+  MSDP_TRY(rec_forbid("potrf"));
+  MSDP_TRY(potrf_count(s, n, A, lda, 0, 1, info));
   if (inverses) {  // all diagonal-block inverses in one launch
     diag_inv_kernel<<<cdiv(n, 64), 64, 0, s>>>(n, A, lda, work, info);
     MSDP_HIP(hipGetLastError());
   }
   return 0;
+}
+
+int potrf2(hipStream_t s, int n, double* A0, double* A1, int64_t lda, int* info) {
+  MSDP_TRY(rec_forbid("potrf2"));
+  const intptr_t d = reinterpret_cast<intptr_t>(A1) - reinterpret_cast<intptr_t>(A0);
+  if (d % (intptr_t)sizeof(double) != 0) {
+    set_error("potrf2: the two matrices are not a whole number of doubles apart");
+    return -2;
+  }
+  return potrf_count(s, n, A0, lda, (int64_t)(d / (intptr_t)sizeof(double)), 2, info);
 }
 
 int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t stride, int batch,

@@ -313,7 +313,9 @@ struct msdp_solver {
   int finalize();
   // block kernels
   int cholesky_factor(Blk& b, const double* M, double* L, int* inf);
-  int prepare_block(Blk& b, int i);
+  int cholesky_factor2(Blk& b, const double* M0, double* L0, const double* M1, double* L1,
+                       int* inf);
+  int prepare_block(Blk& b, int i, bool factored = false);
   int set_dual_slacks(const double* yv);
   int slack(Blk& b, const double* yv, double* out);
   // PrimalDual
@@ -682,10 +684,28 @@ int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) 
   });
 }
 
-// prepare_block! (blocks.jl:126-130): Ls = chol(S), Si = S^-1 (potri)
-int msdp_solver::prepare_block(Blk& bk, int i) {
+// two _cholesky_factor! of one block (n > 64) in the same launches; the
+// statuses into inf[0] and inf[1]
+int msdp_solver::cholesky_factor2(Blk& bk, const double* M0, double* L0, const double* M1,
+                                  double* L1, int* inf) {
   const int n = bk.n;
-  MSDP_TRY(cholesky_factor(bk, bk.S, bk.Ls, binfo(i, 0)));
+  return opaque([=]() -> int {
+    MSDP_TRY(copy_mat(st, n, n, M0, n, L0, n));
+    MSDP_TRY(symmetrize(st, n, L0, n));
+    MSDP_TRY(copy_mat(st, n, n, M1, n, L1, n));
+    MSDP_TRY(symmetrize(st, n, L1, n));
+    MSDP_TRY(potrf2(st, n, L0, L1, n, inf));
+    MSDP_TRY(chol_flush(st, n, L0, n, red));
+    MSDP_TRY(chol_flush(st, n, L1, n, red));
+    return 0;
+  });
+}
+
+// prepare_block! (blocks.jl:126-130): Ls = chol(S), Si = S^-1 (potri);
+// `factored`: Ls is already there
+int msdp_solver::prepare_block(Blk& bk, int i, bool factored) {
+  const int n = bk.n;
+  if (!factored) MSDP_TRY(cholesky_factor(bk, bk.S, bk.Ls, binfo(i, 0)));
   Blk* bp = &bk;
   if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));  // recorded
   else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Ls, n, bp->Lsinv, n, trwork); }));
@@ -833,10 +853,13 @@ int msdp_solver::prepare(bool& ok) {
     Blk& bk = blk[i];
     Blk* bp = &bk;
     const int n = bk.n;
-    MSDP_TRY(prepare_block(bk, i));
     const bool need_lx = nt() || opt.step == MSDP_STEP_EIGEN;
+    // chol(S) and chol(X) of a large block share their launches
+    const bool pair = need_lx && n > 64;
+    if (pair) MSDP_TRY(cholesky_factor2(bk, bk.S, bk.Ls, bk.X, bk.Lx, binfo(i, 0)));
+    MSDP_TRY(prepare_block(bk, i, pair));
     if (need_lx) {
-      MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
+      if (!pair) MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
       if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));  // recorded
       else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
     }
