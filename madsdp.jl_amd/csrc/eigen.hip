@@ -336,9 +336,9 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
                                                        double* tau, double* colk, double* part,
                                                        double* d, double* e) {
   __shared__ double rowW[TRD_NB], rowV[TRD_NB], hvs[TRD_NB], hws[TRD_NB];
-  __shared__ double red[4];
+  __shared__ double red[4], red2[4];
   __shared__ double s_c2, s_wk, s_vk, s_tp;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int prev = j - 1;
   const int kp = k0 + prev;             // column of the previous reflector (j > 0)
   const int ncor = j > 0 ? prev : 0;    // panel columns whose terms pc_i still lacks
@@ -389,7 +389,13 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
       vpk = V[k + (int64_t)kp * n];
     }
   }
-  __syncthreads();
+  // three barriers in all: (A) the panel scalars and the partial sums of
+  // p . v are in LDS, (B) the column's scalars are, (C) the partial squares are
+  if (j > 0) {
+    sp = wave_sum(sp);
+    if (lane == 0) red[wv] = sp;
+  }
+  __syncthreads();  // (A)
   double pcv = pi;
   if (live) {
 #pragma unroll
@@ -400,8 +406,9 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
       }
   }
   if (j > 0) {
-    sp = block_reduce<0>(sp, red);
     if (tid < 64) {  // wave 0: the scalars of column k
+      sp = red[0];
+      for (int q = 1; q < 4; ++q) sp += red[q];
       const int t = tid;
       double hh = t < ncor ? hvs[t] * hws[t] : 0.0;
       double ck = (k >= 0 && t < ncor) ? rowV[t] * hws[t] + rowW[t] * hvs[t] : 0.0;
@@ -417,7 +424,7 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
         }
       }
     }
-    __syncthreads();
+    __syncthreads();  // (B)
   }
   double ss = 0.0;
   if (live) {
@@ -441,8 +448,14 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
     }
     return;
   }
-  ss = block_reduce<0>(ss, red);
-  if (tid == 0) part[blockIdx.x] = ss;
+  ss = wave_sum(ss);
+  if (lane == 0) red2[wv] = ss;
+  __syncthreads();  // (C)
+  if (tid == 0) {
+    ss = red2[0];
+    for (int q = 1; q < 4; ++q) ss += red2[q];
+    part[blockIdx.x] = ss;
+  }
 }
 
 // One workgroup per output: p[c] = A[base:, c]' v for the trailing columns
@@ -460,8 +473,7 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
                                                                 double* e, double* p,
                                                                 double* hv, double* ppart,
                                                                 double* vl, double* vr) {
-  __shared__ double red[4];
-  __shared__ double s_sc;
+  __shared__ double red[4], red2[4];
   const int tid = threadIdx.x;
   const int base = k + 1;
   const int L = n - base;
@@ -478,7 +490,8 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
     colp = W + (int64_t)(c - L - j) * n;
     out = hv + TRD_NB + (c - L - j);
   }
-  // first batch of rows, issued before the reflector's reduction waits
+  // every load is issued before the first wait: the column's rows, column k,
+  // the partial squares and the scalars thread 0 needs
   double av[TRD_PR], cv[TRD_PR];
 #pragma unroll
   for (int s = 0; s < TRD_PR; ++s) {
@@ -486,48 +499,56 @@ __global__ __launch_bounds__(256) void trd_panel_product_kernel(int n, int k, in
     av[s] = i < n ? colp[i] : 0.0;
     cv[s] = i < n ? colk[i] : 0.0;
   }
-  // the reflector of column k (the same arithmetic in every workgroup)
   double ss = 0.0;
   for (int q = tid; q < nparts; q += 256) ss += part[q];
-  ss = block_reduce<0>(ss, red);
+  double alpha = 0.0, ckc = 0.0;
   if (tid == 0) {
-    const double alpha = colk[base];
-    double beta, t, sc;
-    if (ss == 0.0) {
-      beta = alpha; t = 0.0; sc = 0.0;
-    } else {
-      beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-      t = (beta - alpha) / beta;
-      sc = 1.0 / (alpha - beta);
-    }
-    s_sc = sc;
-    if (c == 0) {
-      tau[k] = t;
-      e[k] = beta;
-    }
+    alpha = colk[base];
+    if (c < L) ckc = colk[base + c];
   }
-  __syncthreads();
-  const double sc = s_sc;
+  // p_c = A[base, c] + sc * sum_{i > base} A[i, c] col_k[i]: the unscaled sum
+  // needs no reflector, so one reduction (of the pair) serves both
   double acc = 0.0;
 #pragma unroll
   for (int s = 0; s < TRD_PR; ++s) {
     const int i = base + tid + 256 * s;
-    acc += av[s] * (i == base ? 1.0 : cv[s] * sc);
+    if (i > base) acc += av[s] * cv[s];
   }
-  for (int i = base + tid + 256 * TRD_PR; i < n; i += 256) acc += colp[i] * (colk[i] * sc);
-  acc = block_reduce<0>(acc, red);
-  if (tid == 0) {
-    *out = acc;
-    double pv = 0.0;
-    if (c < L) {
-      const double vc = c == 0 ? 1.0 : colk[base + c] * sc;
-      V[(base + c) + (int64_t)k * n] = vc;
-      vl[base + c] = vc;  // the copies left and right of W (rows > k)
-      vr[base + c] = vc;
-      pv = acc * vc;
-    }
-    ppart[c] = pv;
+  for (int i = base + tid + 256 * TRD_PR; i < n; i += 256) acc += colp[i] * colk[i];
+  const int lane = tid & 63, w = tid >> 6;
+  ss = wave_sum(ss);
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    red[w] = ss;
+    red2[w] = acc;
   }
+  __syncthreads();
+  if (tid != 0) return;
+  ss = (red[0] + red[1]) + (red[2] + red[3]);
+  acc = (red2[0] + red2[1]) + (red2[2] + red2[3]);
+  double beta, t, sc;
+  if (ss == 0.0) {
+    beta = alpha; t = 0.0; sc = 0.0;
+  } else {
+    beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+    t = (beta - alpha) / beta;
+    sc = 1.0 / (alpha - beta);
+  }
+  if (c == 0) {
+    tau[k] = t;
+    e[k] = beta;
+  }
+  const double pc = av[0] + sc * acc;  // av[0] = A[base, c] in thread 0
+  *out = pc;
+  double pv = 0.0;
+  if (c < L) {
+    const double vc = c == 0 ? 1.0 : ckc * sc;
+    V[(base + c) + (int64_t)k * n] = vc;
+    vl[base + c] = vc;  // the copies left and right of W (rows > k)
+    vr[base + c] = vc;
+    pv = pc * vc;
+  }
+  ppart[c] = pv;
 }
 
 // --------------------------------------------------------------------------
