@@ -353,17 +353,22 @@ __global__ __launch_bounds__(256) void trd_rows_kernel(int n, int k, int j, int 
     }
   }
   double sp = 0.0;
-  if (j > 0) {  // four independent sums keep four loads in flight
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int q = tid;
-    for (; q + 768 < npp; q += 1024) {
-      s0 += ppart[q];
-      s1 += ppart[q + 256];
-      s2 += ppart[q + 512];
-      s3 += ppart[q + 768];
+  if (j > 0) {  // 16 independent sums: up to 4096 partials in one round trip
+    double su[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) su[u] = 0.0;
+    for (int q0 = tid; q0 < npp; q0 += 256 * 16) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int q = q0 + 256 * u;
+        su[u] += q < npp ? ppart[q] : 0.0;
+      }
     }
-    for (; q < npp; q += 256) s0 += ppart[q];
-    sp = (s0 + s1) + (s2 + s3);
+#pragma unroll
+    for (int u = 0; u < 16; u += 2) su[u] += su[u + 1];
+#pragma unroll
+    for (int u = 0; u < 16; u += 4) su[u] += su[u + 2];
+    sp = (su[0] + su[4]) + (su[8] + su[12]);
   }
   const int i = start + blockIdx.x * 256 + tid;
   const bool live = i < n;
