@@ -42,10 +42,30 @@ void set_blocking_sync(bool on);
 // --------------------------------------------------------------------------
 // device helpers
 // --------------------------------------------------------------------------
+__device__ inline double readlane_f64(double v, int l);
+// One DPP move of a double (two 32-bit halves); lanes outside row_mask, or
+// whose source lane is out of range, receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+// Sum over the wavefront, returned in every lane: an inclusive scan within
+// each row of 16 lanes (row_shr 1, 2, 4, 8), the row totals carried by
+// row_bcast 15 and 31, then lane 63's total read back.  DPP moves take a few
+// cycles where a shuffle through ds_bpermute takes an LDS round trip; the
+// order of the additions is fixed, so the result is deterministic.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f64<0x111, 0xf>(v);
+  v += dpp_f64<0x112, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  return readlane_f64(v, 63);
 }
 __device__ inline double wave_max(double v) {
 #pragma unroll
