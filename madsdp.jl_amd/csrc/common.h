@@ -67,15 +67,40 @@ __device__ inline double wave_sum(double v) {
   v += dpp_f64<0x143, 0xc>(v);
   return readlane_f64(v, 63);
 }
+// The same DPP move with `id` (the reduction's identity) where the source
+// lane is out of range or the row is masked off.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_f64_or(double v, double id) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long o = (unsigned long long)__double_as_longlong(id);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, CTRL, ROW_MASK,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32),
+                                             CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+// Maximum / minimum over the wavefront (exact in any order), by the scan of
+// wave_sum, returned in every lane.
 __device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  const double id = -INFINITY;
+  v = fmax(v, dpp_f64_or<0x111, 0xf>(v, id));
+  v = fmax(v, dpp_f64_or<0x112, 0xf>(v, id));
+  v = fmax(v, dpp_f64_or<0x114, 0xf>(v, id));
+  v = fmax(v, dpp_f64_or<0x118, 0xf>(v, id));
+  v = fmax(v, dpp_f64_or<0x142, 0xa>(v, id));
+  v = fmax(v, dpp_f64_or<0x143, 0xc>(v, id));
+  return readlane_f64(v, 63);
 }
 __device__ inline double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
+  const double id = INFINITY;
+  v = fmin(v, dpp_f64_or<0x111, 0xf>(v, id));
+  v = fmin(v, dpp_f64_or<0x112, 0xf>(v, id));
+  v = fmin(v, dpp_f64_or<0x114, 0xf>(v, id));
+  v = fmin(v, dpp_f64_or<0x118, 0xf>(v, id));
+  v = fmin(v, dpp_f64_or<0x142, 0xa>(v, id));
+  v = fmin(v, dpp_f64_or<0x143, 0xc>(v, id));
+  return readlane_f64(v, 63);
 }
 // Orders a wavefront's LDS stores before its later LDS loads (no workgroup
 // barrier: the wavefront's LDS accesses complete in program order).

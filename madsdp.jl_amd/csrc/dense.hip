@@ -1189,14 +1189,52 @@ int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L
 // ==========================================================================
 // elementwise
 // ==========================================================================
-__global__ void symmetrize_kernel(int n, double* A, int64_t lda) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= n || i <= j) return;  // each pair (i > j) handled once
-  double* a = A + i + (int64_t)j * lda;
-  double* b = A + j + (int64_t)i * lda;
-  const double v = (*a + *b) / 2;
-  *a = v;
-  *b = v;
+// (A + A')/2 (mode 0) or the lower triangle copied over the upper (mode 1),
+// in place, by 64 x 64 tile pairs: workgroup b takes the tiles (I, J) and
+// (J, I), I >= J, b = I (I + 1) / 2 + J.  Both tiles are read and written
+// along their columns and the transpose goes through LDS; each pair (x > y)
+// gets (A[x,y] + A[y,x]) / 2, the arithmetic of the batched form.
+__global__ __launch_bounds__(256) void sym_tiles_kernel(int n, double* A, int64_t lda, int mode) {
+  __shared__ double Ut[64][65];  // Ut[yy][xx] = A[y0 + yy, x0 + xx]
+  const int b = blockIdx.x;
+  int I = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= b) ++I;
+  while (I * (I + 1) / 2 > b) --I;
+  const int J = b - I * (I + 1) / 2;
+  const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int x0 = 64 * I, y0 = 64 * J;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {  // the upper tile
+    const int xx = 16 * g + q, row = y0 + r, col = x0 + xx;
+    Ut[r][xx] = (row < n && col < n) ? A[row + (int64_t)col * lda] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {  // the lower tile: x = x0 + r, y = y0 + yy
+    const int yy = 16 * g + q, x = x0 + r, y = y0 + yy;
+    if (x < n && y < n && x > y) {
+      double* a = A + x + (int64_t)y * lda;
+      double v = *a;
+      if (mode == 0) {
+        v = (v + Ut[yy][r]) / 2;
+        *a = v;
+      }
+      Ut[yy][r] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {  // the upper tile back
+    const int xx = 16 * g + q, row = y0 + r, col = x0 + xx;
+    if (row < n && col < n && col > row) A[row + (int64_t)col * lda] = Ut[r][xx];
+  }
+}
+
+static int sym_tiles(hipStream_t s, int n, double* A, int64_t lda, int mode) {
+  const int nt = cdiv(n, 64);
+  sym_tiles_kernel<<<nt * (nt + 1) / 2, 256, 0, s>>>(n, A, lda, mode);
+  MSDP_HIP(hipGetLastError());
+  return 0;
 }
 
 int symmetrize(hipStream_t s, int n, double* A, int64_t lda) {
@@ -1208,16 +1246,7 @@ int symmetrize(hipStream_t s, int n, double* A, int64_t lda) {
     o.run = [=] { return symmetrize(s, n, A, lda); };
     return rec_add(std::move(o));
   }
-  dim3 g(cdiv(n, 256), n);
-  symmetrize_kernel<<<g, 256, 0, s>>>(n, A, lda);
-  MSDP_HIP(hipGetLastError());
-  return 0;
-}
-
-__global__ void mirror_lower_kernel(int n, double* A, int64_t lda) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= n || i >= j) return;
-  A[i + (int64_t)j * lda] = A[j + (int64_t)i * lda];
+  return sym_tiles(s, n, A, lda, 0);
 }
 
 int mirror_lower(hipStream_t s, int n, double* A, int64_t lda) {
@@ -1229,10 +1258,7 @@ int mirror_lower(hipStream_t s, int n, double* A, int64_t lda) {
     o.run = [=] { return mirror_lower(s, n, A, lda); };
     return rec_add(std::move(o));
   }
-  dim3 g(cdiv(n, 256), n);
-  mirror_lower_kernel<<<g, 256, 0, s>>>(n, A, lda);
-  MSDP_HIP(hipGetLastError());
-  return 0;
+  return sym_tiles(s, n, A, lda, 1);
 }
 
 __global__ void copy_kernel(int rows, const double* A, int64_t lda, double* B, int64_t ldb) {
