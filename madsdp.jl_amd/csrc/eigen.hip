@@ -1323,8 +1323,17 @@ __global__ __launch_bounds__(1024) void dc_deflate_kernel(const DcNode* nodes, c
       if (rho * fabs(zs[j]) <= tol) { defl[nd_++] = j; continue; }
       if (pj < 0) { pj = j; continue; }
       double s = zs[pj], c = zs[j];
-      const double tau = hypot(c, s);
       const double t = ds[j] - ds[pj];
+      // |t c s| <= tol for the normalised (c, s) reads |t| |c| |s| <= tol
+      // (c^2 + s^2): a pair well outside it (a factor 2 beyond any rounding)
+      // is kept without forming hypot and the quotients; the exact test
+      // below decides every other pair, so the decisions are unchanged
+      if (fabs(t) * fabs(c) * fabs(s) > 2.0 * tol * (c * c + s * s)) {
+        ndl[kp++] = pj;
+        pj = j;
+        continue;
+      }
+      const double tau = hypot(c, s);
       c /= tau;
       s = -s / tau;
       if (fabs(t * c * s) <= tol) {
