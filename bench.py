@@ -193,7 +193,7 @@ def main():
                          "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+        print("bench.py: the benchmark runs on one GPU; multi-GPU runs are not supported "
               "(independent solves shard across GPUs with no exchange).", file=sys.stderr)
         sys.exit(2)
 

@@ -70,7 +70,7 @@ typedef struct msdp_options {
    * 1 = mixed: FP32 MFMA assembly of the dense-constraint blocks while
    * mu >= mu_switch, FP64 Cholesky, and every Schur solve refined against
    * the FP64 operator (at most refine_max_iter steps, FP64 assembly if the
-   * refinement stalls).  Not in the reference: an extension of this build. */
+   * refinement stalls).  Not in the reference: an extension of this library. */
   int32_t precision;               /* 0 */
   double mu_switch;                /* 1e-5 */
   int32_t refine_max_iter;         /* 10 */
