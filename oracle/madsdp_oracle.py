@@ -27,7 +27,15 @@ How each part is pinned (what would catch a wrong sign, index or term):
   identity scale sqrt(15000) (`test/identity_scale.jl:32-42`), termination
   semantics (`test/termination.jl`);
 * brute force: H_jk = tr(A_j W A_k W) by definition, W S W = X, lambda_min
-  by dense eig, Schur regularisation ladder.
+  by dense eig, Schur regularisation ladder;
+* independent mathematics (tests/test_oracle_pins.py): the NT correction from
+  the Lyapunov equation in the W^1/2 scaling (scipy solve_sylvester), the
+  commuting closed form -dX dS / S of both corrections, whole Mehrotra
+  iterations through the unreduced KKT matrix with step lengths by bisection
+  on eigvalsh(X + a dX), EigenStep against the brute-force largest step,
+  DualCentering's alpha by brentq on a log-det form of xi and its predictor
+  step with H_jk = tr(A_j S^-1 A_k S^-1); planted mistakes (correction halved
+  or zeroed, alpha = tau, DC alpha x 0.7, xi x 2) fail them.
 Parity unpinned: LRO.lanczos_eigmin!'s exact output (the restatement uses the
 exact lambda_min, which Lanczos approximates).
 """
