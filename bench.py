@@ -1,16 +1,23 @@
 """Benchmark of the MadSDP interior-point hot path on one MI355X.
 
-A step is one complete solve (initialize!, then iterate! until
-update_termination! stops at the 1e-7 relative gap) of the plain FP64 GPU
-path on the max-cut SDP over a seeded G(3000, 0.01) graph with +-1 weights:
-one 3000 x 3000 PSD block, m = 3000 one-hot constraints, NT direction,
-EigenStep(0.95), Mehrotra(3), DenseCholesky.  The problem data are uploaded
-once and stay resident in HBM; the timed region holds only solves.
+Default workload: the serving stream.  Independent SDP instances arrive as a
+Poisson process (the seed-4 stream: half max-cut with n ~ U{20..600}, half
+dense SDPs with n ~ U{16..200}, m ~ U{50..3000}), each solved to the 1e-7
+relative gap (NT, EigenStep(0.95), Mehrotra(3), DenseCholesky).  A step is a
+window of 16 consecutive requests; `--warmup` windows are served untimed,
+then `--steps` windows as one continuous stream by the mixed-precision
+continuous-batching loop (value = its solves/s), and the same arrivals by the
+plain FP64 path (one solve at a time) for the speed-up, plus the batching
+loop in FP64.  p50/p99 are request latencies (completion - arrival).
+
+Other workloads (`--workload maxcut3000|maxcut100|dense250|blocks96`) time
+one instance solved repeatedly, its data resident in HBM: a step is one
+complete solve.
 
 Output: one JSON line with solves/s (whole job), p50/p99 latency, the live
-roofline of the dominant kernel (HIP events on the library's stream over the
-timed region) and the CPU baseline (the FP64 oracle, a bounded sample of the
-same solve timed on this host).
+roofline of the dominant kernel (HIP events stamped on its dispatches over the
+timed region) and the CPU baseline (the FP64 oracle on a bounded sample of the
+same work, timed on this host).
 """
 from __future__ import annotations
 
@@ -55,83 +62,161 @@ def build_model(workload):
     raise SystemExit(f"unknown workload {workload}")
 
 
+WINDOW = 16  # requests of the serving stream per bench step
+
+
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max((d.get("num_threads", 0) for d in threadpool_info()
+                    if d.get("user_api") == "blas"), default=None)
+    except Exception:
+        return None
+
+
+def stream_cpu_baseline(models, budget_s):
+    """The FP64 oracle on the stream's own instances, in arrival order, as many
+    whole solves as fit in about `budget_s` seconds with all BLAS threads, then
+    the same instances on one thread (the reference benchmark's convention,
+    benchmark/maxcut.jl:16) for at most half that budget."""
+    from oracle import madsdp_oracle as O
+    threads = _blas_threads() or os.cpu_count()
+    done, t0 = 0, time.perf_counter()
+    while done < len(models) and (done == 0 or time.perf_counter() - t0 < budget_s):
+        O.Solver(models[done]).solve()
+        done += 1
+    t_all = time.perf_counter() - t0
+    one = None
+    try:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1, user_api="blas"):
+            k, t1 = 0, time.perf_counter()
+            while k < done and (k == 0 or time.perf_counter() - t1 < budget_s / 2):
+                O.Solver(models[k]).solve()
+                k += 1
+            one = {"value": k / (time.perf_counter() - t1), "cores": 1, "solves": k}
+    except Exception:
+        pass
+    return {"value": done / t_all, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (numpy/scipy OpenBLAS, {threads} threads) solving the first "
+                      f"{done} instances of the timed stream back to back in {t_all:.1f} s",
+            "single_thread": one}
+
+
 def run_stream(args):
-    """Serving stream (c4): Poisson arrivals of independent instances, half
-    max-cut with n ~ U{20..600}, half dense SDPs with n ~ U{16..200},
-    m ~ U{50..3000}, n_s = n.  The same arrivals are served by the plain FP64
-    path (one solve at a time) and by the continuous-batching loop (`--slots`
-    concurrent solves) in `--batched-precision` and in the other precision.
-    Returns the JSON record."""
+    """Serving stream: the seed-4 Poisson stream of independent instances,
+    half max-cut with n ~ U{20..600}, half dense SDPs with n ~ U{16..200},
+    m ~ U{50..3000}, n_s = n.  A step is a window of WINDOW consecutive
+    requests: the first `warmup` windows warm the library up untimed, the
+    next `steps` windows are served as one continuous stream (arrival times
+    kept) by the mixed-precision continuous-batching loop (`slots` concurrent
+    solves), then over the same arrivals by the plain FP64 path (one solve at
+    a time) and by the batching loop in FP64.  Returns the JSON record."""
     import numpy as np
     from madsdp_amd import MadSDPSolver, instances, serve
+    import madsdp_amd.lib as L
 
-    stream = instances.serving_stream(count=args.count, seed=4, rate=args.rate)
-    arrivals = [a - stream[0][0] for a, _, _ in stream]
-
+    nwarm, ntimed = args.warmup * WINDOW, args.steps * WINDOW
+    stream = instances.serving_stream(count=nwarm + ntimed, seed=4, rate=args.rate)
     t = time.perf_counter()
     models = [instances.build_stream_instance(kind, params) for _, kind, params in stream]
     print(f"generated {len(models)} instances in {time.perf_counter() - t:.1f} s", file=sys.stderr,
           flush=True)
+    warm_models, timed_models = models[:nwarm], models[nwarm:]
+    t_first = stream[nwarm][0] if ntimed else 0.0
+    arrivals = [a - t_first for a, _, _ in stream[nwarm:]]
 
-    def one(slots, precision):
+    def one(ms, arr, slots, precision, profile=False):
         t = time.perf_counter()
-        solvers = [MadSDPSolver(model, device=0, precision=precision) for model in models]
+        solvers = [MadSDPSolver(model, device=0, precision=precision) for model in ms]
         prep = time.perf_counter() - t
-        res, summary = serve(solvers, arrivals, slots=slots, mem_budget=args.mem_budget << 30)
+        lib = L.load()
+        if profile:
+            L.check(lib.msdp_profile_begin(3), "msdp_profile_begin")
+        res, summary = serve(solvers, arr, slots=slots, mem_budget=args.mem_budget << 30)
+        prof = None
+        if profile:
+            import ctypes as C
+            ms_, n_, w_ = C.c_double(), C.c_int64(), C.c_double()
+            L.check(lib.msdp_profile_end(C.byref(ms_), C.byref(n_), C.byref(w_)), "msdp_profile_end")
+            prof = (ms_.value, n_.value, w_.value)
         st = [int(s.status) for s in solvers]
         its = [int(s.iter) for s in solvers]
         objs = [s.primal_objective() for s in solvers]
+        info = [s.info() for s in solvers]
         summary.update(prepare_s=prep, succeeded=int(sum(v == 1 for v in st)),
                        iterations_mean=float(np.mean(its)),
-                       not_succeeded=[(k, stream[k][1], stream[k][2].get("n"), stream[k][2].get("m"),
+                       lowp_solves=int(sum(i.lowp_factorizations > 0 for i in info)),
+                       fp64_fallbacks=int(sum(i.fp64_fallbacks for i in info)),
+                       not_succeeded=[(k, stream[nwarm + k][1] if len(ms) == ntimed else "",
                                        st[k], its[k]) for k in range(len(st)) if st[k] != 1])
         print(f"{precision} slots={slots}: {summary}", file=sys.stderr, flush=True)
-        return summary, its, objs
+        return summary, st, its, objs, prof
 
-    if args.stream_paths == "plain":
-        plain, it_p, obj_p = one(1, "fp64")
-        batched, it_b, obj_b = plain, it_p, obj_p
-        other = None
-    elif args.stream_paths == "batched":
-        batched, it_b, obj_b = one(args.slots, args.batched_precision)
-        plain, it_p, obj_p = batched, it_b, obj_b
-        other = None
-    else:
-        plain, it_p, obj_p = one(1, "fp64")
-        batched, it_b, obj_b = one(args.slots, args.batched_precision)
-        alt = "mixed" if args.batched_precision == "fp64" else "fp64"
-        other = one(args.slots, alt)[0]
+    # warm-up (untimed): kernels, stream-ordered memory pools, host threads
+    if nwarm:
+        warm_arr = [a - stream[0][0] for a, _, _ in stream[:nwarm]]
+        one(warm_models, warm_arr, args.slots, args.batched_precision)
+        one(warm_models[:2], [0.0, 0.0], 1, "fp64")
+
+    batched, st_b, it_b, obj_b, prof = one(timed_models, arrivals, args.slots,
+                                           args.batched_precision, profile=True)
+    plain, st_p, it_p, obj_p, _ = one(timed_models, arrivals, 1, "fp64")
+    other = None
+    if args.compare_precision:
+        alt = "fp64" if args.batched_precision == "mixed" else "mixed"
+        other = one(timed_models, arrivals, args.slots, alt)[0]
         other["precision"] = alt
     dit = [b - a for a, b in zip(it_p, it_b)]
     dobj = max(abs(a - b) / max(1.0, abs(a)) for a, b in zip(obj_p, obj_b))
-    return {
+    roof = None
+    if prof and prof[1] > 0 and prof[0] > 0:
+        ms_, n_, w_ = prof
+        achieved = w_ / (ms_ * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAKS["mfma"], "unit": "TFLOP/s",
+                "frac": achieved / PEAKS["mfma"], "traffic": None,
+                "kernel": "trd1_kernel (one-workgroup Householder tridiagonalisation, n <= 640: "
+                          "the NT eigenproblem and the step-length eigenvalues of the stream's "
+                          "blocks); work = (4/3) n^3 flops per matrix, FP64 vector peak",
+                "launches": n_, "avg_launch_us": 1e3 * ms_ / n_,
+                "algorithmic_per_launch": w_ / n_,
+                "kernel_share_of_timed": ms_ * 1e-3 / batched["span_s"]}
+    rec = {
         "metric": METRIC,
         "value": batched["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.count,
-        "warmup": 0,
-        "ms_per_step": 1e3 / batched["solves_per_s"],
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * batched["span_s"] / max(args.steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32" if args.batched_precision == "mixed" else "f64",
         "data": "synthetic (seeded numpy PCG64 instances, Poisson arrivals)",
-        "config": {"workload": f"serving stream: first {args.count} of the 4096-instance seed-4 stream "
-                               f"(half max-cut n~U{{20..600}}, half dense n~U{{16..200}}, m~U{{50..3000}}), "
-                               f"Poisson rate {args.rate}/s, tol 1e-7",
-                   "slots": args.slots, "rate": args.rate, "count": args.count},
+        "config": {"workload": f"serving stream: requests {nwarm}..{nwarm + ntimed - 1} of the "
+                               f"seed-4 stream (prefix of the 4096-instance stream; {WINDOW} requests "
+                               f"per step, the first {nwarm} served untimed as warm-up): half max-cut "
+                               f"n~U{{20..600}}, half dense n~U{{16..200}} m~U{{50..3000}} n_s=n; "
+                               f"Poisson rate {args.rate}/s; tol 1e-7; {args.batched_precision} "
+                               f"continuous batching with {args.slots} slots vs plain FP64 one at a "
+                               f"time over the same arrivals",
+                   "requests": ntimed, "window": WINDOW, "slots": args.slots, "rate": args.rate},
         "p50_latency_ms": 1e3 * batched["p50_latency_s"],
         "p99_latency_ms": 1e3 * batched["p99_latency_s"],
         "plain_fp64": plain,
         "batched": dict(batched, precision=args.batched_precision),
         "batched_other_precision": other,
         "speedup_vs_plain": batched["solves_per_s"] / plain["solves_per_s"],
-        "max_extra_iterations": int(max(dit)),
+        "same_status": st_b == st_p,
+        "max_extra_iterations": int(max(dit)) if dit else 0,
         "max_rel_objective_diff": dobj,
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = stream_cpu_baseline(timed_models, args.cpu_seconds)
+    return rec
 
 
 def cpu_baseline(model, iters_gpu, budget_s):
@@ -171,9 +256,11 @@ def cpu_baseline(model, iters_gpu, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="maxcut3000")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="stream",
+                    help="stream (default: the serving stream), maxcut3000, maxcut100, "
+                         "dense250, blocks96 or maxcut<N> (one instance solved repeatedly)")
     ap.add_argument("--profile", default="trd", choices=["gemm", "trd", "gemm32", "none"],
                     help="kernel timed live for the roofline: trd = trd_panel_product_kernel "
                          "(the dominant kernel of the max-cut solve), gemm = the FP64 GEMM class, "
@@ -181,13 +268,13 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--count", type=int, default=128, help="stream: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="stream: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=16, help="stream: concurrent solves")
     ap.add_argument("--mem-budget", type=int, default=128, help="stream: device GiB for admission")
-    ap.add_argument("--stream-paths", default="both", choices=["both", "plain", "batched"])
-    ap.add_argument("--batched-precision", default="fp64", choices=["mixed", "fp64"],
+    ap.add_argument("--batched-precision", default="mixed", choices=["mixed", "fp64"],
                     help="stream: precision of the batched path")
+    ap.add_argument("--compare-precision", type=int, default=1,
+                    help="stream: also serve the timed requests batched in the other precision")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run an elementwise kernel over MIB MiB three times (the "
                          "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -207,8 +294,7 @@ def main():
         rec = run_stream(args)
         print(json.dumps(rec), flush=True)
         # every request must end with the same status on both paths
-        same = rec["plain_fp64"]["not_succeeded"] == rec["batched"]["not_succeeded"]
-        return 0 if same else 1
+        return 0 if rec["same_status"] else 1
 
     if args.workload == "blocks96":
         # the per-block stages run on 8 auxiliary streams: give them their own

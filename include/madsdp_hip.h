@@ -74,6 +74,11 @@ typedef struct msdp_options {
   int32_t precision;               /* 0 */
   double mu_switch;                /* 1e-5 */
   int32_t refine_max_iter;         /* 10 */
+  /* mixed precision only when one Schur assembly of the dense-constraint
+   * blocks costs at least this many GFLOP (4 m_i n^3 + m_i^2 n(n+1)/2 summed
+   * over the blocks): below it the refinement costs more than the FP32
+   * assembly saves and the solve stays FP64 */
+  double lowp_min_gflop;           /* 50 */
 } msdp_options;
 
 /* iteration data of MadSDPSolver (src/solver.jl:37-48) */
@@ -147,11 +152,17 @@ int msdp_solver_set(msdp_solver* s, int32_t which, int32_t block, const double* 
 /* wait for the solver's stream */
 int msdp_solver_synchronize(msdp_solver* s);
 /* live timing of one kernel class on the solver's stream (0 = FP64 MFMA GEMM,
- * 1 = tridiagonalisation update, 2 = FP32 MFMA GEMM): HIP events around every launch, summed
+ * 1 = tridiagonalisation update, 2 = FP32 MFMA GEMM, 3 = one-workgroup
+ * tridiagonalisation): HIP events around every launch, summed
  * duration, launch count and algorithmic work (flops or bytes).  No
  * counterpart in the reference; it feeds the roofline report. */
 int msdp_solver_profile_begin(msdp_solver* s, int32_t kernel_class);
 int msdp_solver_profile_end(msdp_solver* s, double* total_ms, int64_t* launches, double* work);
+/* the same over every stream of the process (the serving loop's concurrent
+ * solves); class 3 = the one-workgroup tridiagonalisation (n <= 640).  No
+ * counterpart in the reference. */
+int msdp_profile_begin(int32_t kernel_class);
+int msdp_profile_end(double* total_ms, int64_t* launches, double* work);
 /* Run the solver on a caller-owned HIP stream (before msdp_solver_initialize;
  * the solver never destroys it).  Without it the solver creates its own
  * non-blocking stream.  No counterpart in the reference: Julia runs one

@@ -376,7 +376,7 @@ namespace msdp {
 // around the launch, or stamped on the dispatch by hipExtLaunchKernel) and
 // its algorithmic work (flops or bytes) is accumulated.
 // --------------------------------------------------------------------------
-enum ProfClass { PROF_GEMM = 0, PROF_TRD = 1, PROF_GEMM32 = 2, PROF_NONE = -1 };
+enum ProfClass { PROF_GEMM = 0, PROF_TRD = 1, PROF_GEMM32 = 2, PROF_TRD1 = 3, PROF_NONE = -1 };
 bool prof_active(int cls, hipStream_t s);
 void prof_before(hipStream_t s);
 void prof_after(hipStream_t s, double work);
@@ -384,6 +384,7 @@ void prof_after(hipStream_t s, double work);
 // runtime stamps on the dispatch itself (no marker packets around it), with
 // the launch's algorithmic work.
 void prof_dispatch_events(hipEvent_t* start, hipEvent_t* stop, double work);
+// s = nullptr: every stream of the process
 int prof_begin(int cls, hipStream_t s);
 int prof_end(double* total_ms, int64_t* launches, double* work);
 }  // namespace msdp

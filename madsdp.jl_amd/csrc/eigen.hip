@@ -723,12 +723,25 @@ __global__ __launch_bounds__(TRD1_NT) void trd1_kernel(int n, const double* Ain,
   }
 }
 
-static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWork& t) {
-  if (n <= TRD1_MAX) {
-    trd1_kernel<<<1, TRD1_NT, 0, s>>>(n, A, lda, t.A, t.V, t.tau, t.d, t.e, A, 0);
-    MSDP_HIP(hipGetLastError());
-    return 0;
+// `count` (1 or 2) one-workgroup reductions in one launch; under live
+// profiling the launch is timed on its dispatch, algorithmic work the
+// Householder tridiagonalisation's (4/3) n^3 flops per matrix
+static int launch_trd1(hipStream_t s, int count, int n, const double* A0, int64_t lda,
+                       const TrdWork& t, const double* A1, int64_t wstride) {
+  if (prof_active(PROF_TRD1, s)) {
+    hipEvent_t e0, e1;
+    prof_dispatch_events(&e0, &e1, count * (4.0 / 3.0) * (double)n * n * n);
+    hipExtLaunchKernelGGL(trd1_kernel, dim3(count), dim3(TRD1_NT), 0, s, e0, e1, 0, n, A0, lda,
+                          t.A, t.V, t.tau, t.d, t.e, A1, wstride);
+  } else {
+    trd1_kernel<<<count, TRD1_NT, 0, s>>>(n, A0, lda, t.A, t.V, t.tau, t.d, t.e, A1, wstride);
   }
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWork& t) {
+  if (n <= TRD1_MAX) return launch_trd1(s, 1, n, A, lda, t, A, 0);
   MSDP_TRY(copy_mat(s, n, n, A, lda, t.A, n));
   MSDP_TRY(fill(s, (int64_t)n * n, 0.0, t.V));
   // column step: rows (w_{k-1}, column k) then the reflector and products
@@ -2267,7 +2280,7 @@ static int eig_min_count(hipStream_t s, int n, const double* A0, const double* A
   if (n >= LZ_MIN_N) return lanczos_min(s, n, A0, A1, lda, out0, out1, work, ws, count);
   // n < LZ_MIN_N <= TRD1_MAX: the one-workgroup tridiagonalisation, then Sturm
   TrdWork t = trd_carve(n, work);
-  trd1_kernel<<<count, TRD1_NT, 0, s>>>(n, A0, lda, t.A, t.V, t.tau, t.d, t.e, A1, ws);
+  MSDP_TRY(launch_trd1(s, count, n, A0, lda, t, A1, ws));
   tridiag_min_kernel<<<count, 256, 0, s>>>(n, t.d, t.e, out0, ws, out1);
   MSDP_HIP(hipGetLastError());
   return 0;

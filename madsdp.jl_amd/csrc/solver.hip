@@ -327,6 +327,7 @@ struct msdp_solver {
   int apply_H(const double* v, double* out);
   int schur_solve(double* v);
   int factorize_schur(bool& ok);
+  double dense_assembly_gflop() const;
   int schur_scaling(Blk& bk, SchurScaling& sc);
   int apply_E(Blk& b, const double* M, bool diag, double* out);
   int solve_directions(double mu_, bool corrected);
@@ -934,13 +935,29 @@ int msdp_solver::schur_scaling(Blk& bk, SchurScaling& sc) {
   return 0;
 }
 
+// Flops (in GFLOP) of one Schur assembly of the dense-constraint blocks:
+// P_j = L A_j R for the m_i touching matrices (4 m_i n^3) and the product
+// H += B'B of their svec (vec for HKM) panels (m_i^2 d).  Low-precision
+// assembly pays only where this work outweighs the refinement it brings.
+double msdp_solver::dense_assembly_gflop() const {
+  double f = 0.0;
+  for (const auto& bk : blk) {
+    if (bk.ops.kind != BlockOps::DENSE) continue;
+    const double n = bk.n, mi = bk.ops.ncon;
+    const double d = nt() ? n * (n + 1) / 2 : n * n;
+    f += 4.0 * mi * n * n * n + mi * mi * d;
+  }
+  return f * 1e-9;
+}
+
 // factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
 int msdp_solver::factorize_schur(bool& ok) {
   ok = false;
   if (opt.schur == MSDP_SCHUR_QR) return qr_factorize(ok);
   if (m == 0) { ok = true; return 0; }
   // mixed precision: FP32 assembly of the dense blocks while mu >= mu_switch
-  const bool lowp = opt.precision == 1 && !dual_centering() && mu >= opt.mu_switch && !force_fp64;
+  const bool lowp = opt.precision == 1 && !dual_centering() && mu >= opt.mu_switch && !force_fp64 &&
+                    dense_assembly_gflop() >= opt.lowp_min_gflop;
   bool any_lowp = false;
   if (lowp) MSDP_TRY(ensure_mixed());
   MSDP_TRY(fill(st, m * m, 0.0, Hs));
@@ -1888,6 +1905,7 @@ void msdp_default_options(msdp_options* o) {
   o->precision = 0;
   o->mu_switch = 1e-5;
   o->refine_max_iter = 10;
+  o->lowp_min_gflop = 50.0;
 }
 
 int msdp_solver_create(msdp_solver** out, int device, int32_t nblocks, const int64_t* dims,
@@ -2071,6 +2089,14 @@ int msdp_solver_profile_begin(msdp_solver* s, int32_t cls) {
 }
 int msdp_solver_profile_end(msdp_solver* s, double* ms, int64_t* launches, double* work) {
   GUARD0(s);
+  return prof_end(ms, launches, work);
+}
+int msdp_profile_begin(int32_t cls) {
+  if (cls < PROF_GEMM || cls > PROF_TRD1) { set_error("no such kernel class"); return -2; }
+  return prof_begin(cls, nullptr);
+}
+int msdp_profile_end(double* ms, int64_t* launches, double* work) {
+  if (!ms || !launches || !work) { set_error("null output"); return -2; }
   return prof_end(ms, launches, work);
 }
 

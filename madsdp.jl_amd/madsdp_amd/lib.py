@@ -29,6 +29,7 @@ class Options(C.Structure):
         ("dc_beta", C.c_double), ("dc_delta", C.c_double), ("dc_A", C.c_double),
         ("dc_centering_max_iter", C.c_int32),
         ("precision", C.c_int32), ("mu_switch", C.c_double), ("refine_max_iter", C.c_int32),
+        ("lowp_min_gflop", C.c_double),
     ]
 
 
@@ -90,6 +91,8 @@ SIGNATURES = {
     "msdp_solver_synchronize": (C.c_int, [P]),
     "msdp_solver_profile_begin": (C.c_int, [P, C.c_int32]),
     "msdp_solver_profile_end": (C.c_int, [P, c_double_p, c_int64_p, c_double_p]),
+    "msdp_profile_begin": (C.c_int, [C.c_int32]),
+    "msdp_profile_end": (C.c_int, [c_double_p, c_int64_p, c_double_p]),
     "msdp_solver_set_stream": (C.c_int, [P, P]),
     "msdp_solver_footprint": (C.c_int, [P, c_int64_p]),
     "msdp_solver_release": (C.c_int, [P]),

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 from make_golden import FIELDS, trace_array, unpack  # noqa: E402
 
-FILES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+FILES = sorted(f for f in glob.glob(os.path.join(HERE, "golden", "*.npz"))
+               if not os.path.basename(f).startswith("config_"))
 
 
 def load(path):
@@ -36,3 +37,29 @@ def test_oracle_reproduces_golden(path):
     np.testing.assert_allclose(trace_array(s), z["trace"], rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(s.y, z["y"], rtol=1e-8, atol=1e-10)
     assert FIELDS[0] == "pobj"
+
+
+CONFIG = sorted(glob.glob(os.path.join(HERE, "golden", "config_*.npz")))
+
+
+@pytest.mark.parametrize("path", [p for p in CONFIG if "dense_250" not in p],
+                         ids=lambda p: os.path.basename(p)[7:-4])
+def test_config_traces_match_regenerated_instances(path):
+    """The workload traces were made from the instances the generators still
+    produce (same digest), and each ends converged."""
+    from make_config_traces import config_models, digest, stream_models
+    z = np.load(path)
+    name = os.path.basename(path)[7:-4]
+    if name == "stream_prefix":
+        items = stream_models(int(z["count"]))
+        keys = [f"{k}_" for k in range(len(items))]
+        models = [m for _, _, m in items]
+    else:
+        keys = [""]
+        models = [config_models()[name][0]()]
+    for key, model in zip(keys, models):
+        assert digest(model) == str(z[key + "digest"])
+        tr = z[key + "trace"]
+        assert int(z[key + "status"]) == int(O.Status.SOLVE_SUCCEEDED)
+        assert tr.shape[0] == int(z[key + "iter"]) + 1
+        assert max(tr[-1, 2], tr[-1, 3], tr[-1, 4]) <= 1e-7

@@ -51,7 +51,7 @@ def test_gemm32(env, ta, tb, Mm, N, K):
 
 def _pair(model, **kw):
     a = M.MadSDPSolver(model, **kw).solve()
-    b = M.MadSDPSolver(model, precision="mixed", **kw).solve()
+    b = M.MadSDPSolver(model, precision="mixed", lowp_min_gflop=0.0, **kw).solve()
     return a, b
 
 
@@ -96,7 +96,7 @@ def test_mixed_switch_never(gpu):
     its iterates are the FP64 path's, bit for bit."""
     model = instances.dense_sdp(20, 60, 10, 2)
     a = M.MadSDPSolver(model).solve()
-    b = M.MadSDPSolver(model, precision="mixed", mu_switch=1e300).solve()
+    b = M.MadSDPSolver(model, precision="mixed", lowp_min_gflop=0.0, mu_switch=1e300).solve()
     assert b.info().lowp_factorizations == 0
     assert a.iter == b.iter
     np.testing.assert_array_equal(a.trace(), b.trace())
@@ -107,7 +107,7 @@ def test_mixed_maxcut_is_fp64(gpu):
     path is the FP64 path."""
     model = instances.maxcut(60, 0.2, 4)
     a = M.MadSDPSolver(model).solve()
-    b = M.MadSDPSolver(model, precision="mixed").solve()
+    b = M.MadSDPSolver(model, precision="mixed", lowp_min_gflop=0.0).solve()
     assert b.info().lowp_factorizations == 0
     np.testing.assert_array_equal(a.trace(), b.trace())
 
@@ -134,7 +134,7 @@ def test_mixed_error_vs_oracle(gpu):
     from oracle import madsdp_oracle as O
     model = instances.dense_sdp(30, 200, 30, 4)
     ref = O.Solver(model).solve()
-    b = M.MadSDPSolver(model, precision="mixed").solve()
+    b = M.MadSDPSolver(model, precision="mixed", lowp_min_gflop=0.0).solve()
     assert ref.status == O.Status.SOLVE_SUCCEEDED and b.status == M.Status.SOLVE_SUCCEEDED
     assert b.info().lowp_factorizations > 0
     tr = b.trace()

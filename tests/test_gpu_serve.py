@@ -19,12 +19,12 @@ def mix():
         (instances.maxcut(40, 0.2, 1), {}),
         (instances.dense_sdp(20, 60, 10, 2), {}),
         (instances.maxcut(150, 0.05, 3, "pm1"), {}),
-        (instances.dense_sdp(30, 200, 30, 4), dict(precision="mixed")),
+        (instances.dense_sdp(30, 200, 30, 4), dict(precision="mixed", lowp_min_gflop=0.0)),
         (instances.maxcut(90, 0.1, 5), dict(direction=M.HKM())),
         (instances.block_sdp(instances.block_sizes(5, 4, 40, 6), 70, 6, 7), {}),
         (instances.dense_sdp(16, 50, 16, 8), dict(step=M.CholeskyBacktrack(), centering=M.AdaptiveMu())),
         (instances.maxcut(300, 0.03, 9), {}),
-        (instances.dense_sdp(60, 400, 0, 10), dict(precision="mixed")),
+        (instances.dense_sdp(60, 400, 0, 10), dict(precision="mixed", lowp_min_gflop=0.0)),
         (instances.maxcut(25, 0.3, 11), dict(algorithm=M.DualCentering())),
     ]
 
