@@ -77,7 +77,8 @@ typedef struct msdp_options {
   /* mixed precision only when one Schur assembly of the dense-constraint
    * blocks costs at least this many GFLOP (4 m_i n^3 + m_i^2 n(n+1)/2 summed
    * over the blocks): below it the refinement costs more than the FP32
-   * assembly saves and the solve stays FP64 */
+   * assembly saves and the solve stays FP64; it also stays FP64 when m
+   * exceeds the sum of the blocks' n(n+1)/2 and n_s (a singular H) */
   double lowp_min_gflop;           /* 50 */
 } msdp_options;
 

@@ -6,13 +6,12 @@
 // trmm/trsm/gemm/syrk (src/directions.jl:50-98, src/step.jl:160-162).
 #include "common.h"
 
-#include <rocblas/rocblas.h>
-
 #include <algorithm>
 #include <array>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 namespace msdp {
 
@@ -181,28 +180,18 @@ __global__ __launch_bounds__(256) void gemm_vb_kernel(const GemmItem* __restrict
 constexpr int BT = 128;
 constexpr int BK = 16;
 
+// The K slabs [s0, s1) of one 128 x 128 tile (m0, n0) of op(A) op(B), added
+// to acc.  The next 16-deep slab is fetched into registers while the current
+// one is multiplied out of LDS (double-buffered, one barrier per slab); the
+// last barrier leaves the LDS free for the caller's next tile.
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
-    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
-    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
-    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
-  if (guard && guard[blockIdx.z] != 0) return;  // one guard per batch entry
-  int tm = blockIdx.x, tn = blockIdx.y;
-  if (lower == 2) {  // square lower triangle on a 1-D, XCD-grouped grid
-    if (!lower_tile_xcd((M + BT - 1) / BT, tm, tn)) return;
-  } else if (lower && tn > tm) {
-    return;
-  }
-  const int64_t bz = blockIdx.z;
-  A += bz * sA;
-  B += bz * sB;
-  C += bz * sC;
-  __shared__ double As[2][BK][BT + 2];
-  __shared__ double Bs[2][BK][BT + 2];
+__device__ __attribute__((always_inline)) inline void gemm128_slabs(
+    int M, int N, int K, const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
+    int64_t ldb, int m0, int n0, int s0, int s1, double (*As)[BK][BT + 2],
+    double (*Bs)[BK][BT + 2], v4d (&acc)[4][4]) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = tm * BT, n0 = tn * BT;
   double ra[8], rb[8];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -230,18 +219,12 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
       Bs[buf][kk][nn] = rb[r];
     }
   };
-  v4d acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
-  const int nk = (K + BK - 1) / BK;
-  gload(0);
+  gload(s0 * BK);
   sstore(0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+  for (int kt = s0; kt < s1; ++kt) {
+    const int buf = (kt - s0) & 1;
+    if (kt + 1 < s1) gload((kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double av[4], bv[4];
@@ -256,9 +239,19 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
         for (int b = 0; b < 4; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    if (kt + 1 < nk) sstore(buf ^ 1);
+    if (kt + 1 < s1) sstore(buf ^ 1);
     __syncthreads();
   }
+}
+
+// C tile (m0, n0) = alpha acc + beta C (lower: the entries on or below the
+// diagonal only)
+__device__ __attribute__((always_inline)) inline void gemm128_store(
+    int M, int N, double alpha, double beta, double* __restrict__ C, int64_t ldc, int m0, int n0,
+    int lower, const v4d (&acc)[4][4]) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
 #pragma unroll
@@ -275,6 +268,137 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
       }
     }
   }
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
+    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
+    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
+  if (guard && guard[blockIdx.z] != 0) return;  // one guard per batch entry
+  int tm = blockIdx.x, tn = blockIdx.y;
+  if (lower == 2) {  // square lower triangle on a 1-D, XCD-grouped grid
+    if (!lower_tile_xcd((M + BT - 1) / BT, tm, tn)) return;
+  } else if (lower && tn > tm) {
+    return;
+  }
+  const int64_t bz = blockIdx.z;
+  A += bz * sA;
+  B += bz * sB;
+  C += bz * sC;
+  __shared__ double As[2][BK][BT + 2];
+  __shared__ double Bs[2][BK][BT + 2];
+  v4d acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+  gemm128_slabs<TA, TB>(M, N, K, A, lda, B, ldb, tm * BT, tn * BT, 0, (K + BK - 1) / BK, As, Bs, acc);
+  gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, lower, acc);
+}
+
+// --------------------------------------------------------------------------
+// Stream-K GEMM for large plain products.  The T = ntm x ntn tiles of 128 x
+// 128 are cut into U = T x I slab units (I = K slabs of 16 per tile) and the
+// G workgroups (one per CU) take equal contiguous runs of units, so a tile
+// grid that is not a multiple of the CU count leaves no partial last wave.
+// A workgroup whose run covers a whole tile stores it into C; a run that
+// covers part of a tile stores its accumulators into slot 2r (the run's first
+// tile) or 2r + 1 (its last) of the partials buffer P, in the accumulator
+// layout (register-major, thread-minor: coalesced), and the fixup kernel
+// adds the parts of each split tile in K order (deterministic) into C.  Run
+// r goes to workgroup r's XCD-contiguous position: consecutive workgroups sit
+// on different XCDs, so each XCD's L2 serves a contiguous range of tiles.
+// Tiles are visited in groups of SK_GM tile rows, column by column.
+// --------------------------------------------------------------------------
+constexpr int SK_GM = 8;
+
+struct SkGrid {
+  int ntm, ntn, I, G;
+  int64_t U;
+  __host__ __device__ int64_t first(int r) const { return (int64_t)r * U / G; }
+  __host__ __device__ int owner(int64_t u) const { return (int)(((u + 1) * G - 1) / U); }
+  __host__ __device__ void tile(int t, int& tm, int& tn) const {
+    const int per = SK_GM * ntn;
+    const int g = t / per, in = t % per;
+    const int rows = min(SK_GM, ntm - g * SK_GM);
+    tn = in / rows;
+    tm = g * SK_GM + in % rows;
+  }
+};
+
+__device__ inline int sk_run(const SkGrid& sg) {
+  const int w = blockIdx.x;
+  const int per = sg.G / 8, rem = sg.G % 8, x = w & 7, q = w >> 3;
+  // XCD x holds runs [x * per + min(x, rem), ...) when G is not a multiple of 8
+  return x * per + min(x, rem) + q;
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm_sk_f64_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda,
+    const double* __restrict__ B, int64_t ldb, double beta, double* __restrict__ C, int64_t ldc,
+    SkGrid sg, double* __restrict__ P) {
+  const int r = sk_run(sg);
+  if (r >= sg.G) return;
+  __shared__ double As[2][BK][BT + 2];
+  __shared__ double Bs[2][BK][BT + 2];
+  int64_t u = sg.first(r);
+  const int64_t u1 = sg.first(r + 1);
+  const int tfirst = (int)(u / sg.I);
+  while (u < u1) {
+    const int t = (int)(u / sg.I);
+    const int s0 = (int)(u - (int64_t)t * sg.I);
+    const int s1 = (int)min<int64_t>(sg.I, s0 + (u1 - u));
+    int tm, tn;
+    sg.tile(t, tm, tn);
+    v4d acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+    gemm128_slabs<TA, TB>(M, N, K, A, lda, B, ldb, tm * BT, tn * BT, s0, s1, As, Bs, acc);
+    if (s0 == 0 && s1 == sg.I) {
+      gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, 0, acc);
+    } else {
+      double* slot = P + (int64_t)(2 * r + (t == tfirst ? 0 : 1)) * (BT * BT);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) slot[((a * 4 + b) * 4 + q) * 256 + threadIdx.x] = acc[a][b][q];
+    }
+    u += s1 - s0;
+  }
+}
+
+__global__ __launch_bounds__(256) void gemm_sk_fixup_kernel(int M, int N, double alpha,
+                                                            double beta, double* __restrict__ C,
+                                                            int64_t ldc, SkGrid sg,
+                                                            const double* __restrict__ P) {
+  const int t = blockIdx.x;
+  const int64_t ua = (int64_t)t * sg.I, ub = ua + sg.I - 1;
+  const int ra = sg.owner(ua), rb = sg.owner(ub);
+  if (ra == rb) return;  // the whole tile was one run's: already in C
+  v4d acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+  for (int r = ra; r <= rb; ++r) {
+    const int tfirst = (int)(sg.first(r) / sg.I);
+    const double* slot = P + (int64_t)(2 * r + (t == tfirst ? 0 : 1)) * (BT * BT);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][b][q] += slot[((a * 4 + b) * 4 + q) * 256 + threadIdx.x];
+  }
+  int tm, tn;
+  sg.tile(t, tm, tn);
+  gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, 0, acc);
 }
 
 // Split-K GEMM for skinny outputs with a long K (few output tiles): each
@@ -359,65 +483,66 @@ __global__ void splitk_reduce_kernel(int M, int N, int parts, const double* P, d
   *c = beta == 0.0 ? alpha * v : alpha * v + beta * (*c);
 }
 
-// Large plain products (no batch, no triangle, no guard) go to the vendor
-// DGEMM: at these shapes it is 1.15-1.5x faster than gemm128_f64_kernel (the
-// 3000-wide tile grid leaves a partial second wave of workgroups).  Atomic
-// accumulation is disabled, so results are deterministic.  One handle per
-// host thread, pointed at the caller's stream for each call.
-namespace {
-struct BlasHandle {
-  rocblas_handle h = nullptr;
-  int device = -1;
-  ~BlasHandle() {
-    if (h) (void)rocblas_destroy_handle(h);
-  }
-};
-thread_local BlasHandle t_blas;
-}  // namespace
-
 constexpr int SPLITK_WGS = 1024, SPLITK_MIN_K = 256, SPLITK_MIN_CHUNK = 64;
 
-static bool vendor_gemm_shape(int M, int N, int K, int batch, bool lower, const int* guard) {
-  return batch == 1 && !lower && !guard && M >= 512 && N >= 512 && K >= 32;
-}
-
-
-
-static int blas_handle(hipStream_t s);
-
-static int vendor_gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
-                       const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                       double* C, int64_t ldc) {
-  MSDP_TRY(blas_handle(s));
-  const rocblas_status st = rocblas_dgemm(
-      t_blas.h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-      tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A, (int)lda, B,
-      (int)ldb, &beta, C, (int)ldc);
-  if (st != rocblas_status_success) {
-    set_error("rocblas_dgemm failed");
-    return -1;
-  }
-  return 0;
-}
-
-static int blas_handle(hipStream_t s) {
+// Compute units of the current device (the stream-K grid: one workgroup per
+// CU, so the partition -- and with it the summation order -- is fixed per
+// device).
+static int device_cus() {
+  static std::mutex mu;
+  static std::vector<int> cus;
   int dev = 0;
-  MSDP_HIP(hipGetDevice(&dev));
-  if (!t_blas.h || t_blas.device != dev) {
-    if (t_blas.h) (void)rocblas_destroy_handle(t_blas.h);
-    t_blas.h = nullptr;
-    if (rocblas_create_handle(&t_blas.h) != rocblas_status_success) {
-      set_error("rocblas_create_handle failed");
-      return -1;
-    }
-    (void)rocblas_set_atomics_mode(t_blas.h, rocblas_atomics_not_allowed);
-    (void)rocblas_set_pointer_mode(t_blas.h, rocblas_pointer_mode_host);
-    t_blas.device = dev;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)cus.size() <= dev) cus.resize(dev + 1, 0);
+  if (cus[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    cus[dev] = v;
   }
-  if (rocblas_set_stream(t_blas.h, s) != rocblas_status_success) {
-    set_error("rocblas_set_stream failed");
-    return -1;
-  }
+  return cus[dev];
+}
+
+// Plain products (no batch, no triangle, no guard) whose 128-tile grid would
+// leave a partial last wave go through the stream-K kernel: at least
+// SK_MIN_UNITS slabs per workgroup, and more tiles than CUs.
+constexpr int SK_MIN_UNITS = 8;
+
+static bool stream_k_shape(int M, int N, int K, int batch, bool lower, const int* guard, int G,
+                           SkGrid& sg) {
+  if (batch != 1 || lower || guard || M < BT || N < BT) return false;
+  sg.ntm = cdiv(M, BT);
+  sg.ntn = cdiv(N, BT);
+  sg.I = cdiv(K, BK);
+  sg.G = G;
+  sg.U = (int64_t)sg.ntm * sg.ntn * sg.I;
+  const int64_t T = (int64_t)sg.ntm * sg.ntn;
+  return T > G && T % G != 0 && sg.U >= (int64_t)SK_MIN_UNITS * G;
+}
+
+#ifndef MSDP_SK_OCC
+#define MSDP_SK_OCC 2
+#endif
+// stream-K workgroups per CU (two of the 256-thread, 67 KB-LDS workgroups fit)
+constexpr int SK_OCC = MSDP_SK_OCC;
+
+size_t gemm_sk_partials_bytes() { return sizeof(double) * 2 * (size_t)SK_OCC * device_cus() * BT * BT; }
+
+static int gemm_stream_k(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                         double* C, int64_t ldc, const SkGrid& sg) {
+  void* P = nullptr;
+  MSDP_HIP(hipMallocAsync(&P, sizeof(double) * 2 * (size_t)sg.G * BT * BT, s));
+  double* Pd = static_cast<double*>(P);
+  const dim3 g(sg.G);
+  if (!ta && !tb) gemm_sk_f64_kernel<false, false><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
+  else if (ta && !tb) gemm_sk_f64_kernel<true, false><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
+  else if (!ta && tb) gemm_sk_f64_kernel<false, true><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
+  else gemm_sk_f64_kernel<true, true><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
+  gemm_sk_fixup_kernel<<<sg.ntm * sg.ntn, 256, 0, s>>>(M, N, alpha, beta, C, ldc, sg, Pd);
+  MSDP_HIP(hipGetLastError());
+  MSDP_HIP(hipFreeAsync(P, s));
   return 0;
 }
 
@@ -439,9 +564,10 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     return 0;
   }
   const bool prof = prof_active(PROF_GEMM, s);
-  if (vendor_gemm_shape(M, N, K, batch, lower, guard)) {
+  SkGrid sg;
+  if (stream_k_shape(M, N, K, batch, lower, guard, SK_OCC * device_cus(), sg)) {
     if (prof) prof_before(s);
-    MSDP_TRY(vendor_gemm(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc));
+    MSDP_TRY(gemm_stream_k(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg));
     if (prof) prof_after(s, 2.0 * M * N * (double)K);
     return 0;
   }

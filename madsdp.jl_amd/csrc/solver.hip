@@ -328,6 +328,7 @@ struct msdp_solver {
   int schur_solve(double* v);
   int factorize_schur(bool& ok);
   double dense_assembly_gflop() const;
+  bool lowp_eligible() const;
   int schur_scaling(Blk& bk, SchurScaling& sc);
   int apply_E(Blk& b, const double* M, bool diag, double* out);
   int solve_directions(double mu_, bool corrected);
@@ -950,6 +951,17 @@ double msdp_solver::dense_assembly_gflop() const {
   return f * 1e-9;
 }
 
+// Low-precision assembly needs a Schur complement that can be positive
+// definite: with more constraints than the sum of svec dimensions and
+// scalars, H is singular, the reference's ladder of absolute shifts
+// (1e-12 ... 1e-5) keeps it factorable in FP64 but not past an FP32
+// rounding error of H.
+bool msdp_solver::lowp_eligible() const {
+  double dim = (double)ns;
+  for (const auto& bk : blk) dim += 0.5 * (double)bk.n * (bk.n + 1);
+  return (double)m <= dim && dense_assembly_gflop() >= opt.lowp_min_gflop;
+}
+
 // factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
 int msdp_solver::factorize_schur(bool& ok) {
   ok = false;
@@ -957,7 +969,7 @@ int msdp_solver::factorize_schur(bool& ok) {
   if (m == 0) { ok = true; return 0; }
   // mixed precision: FP32 assembly of the dense blocks while mu >= mu_switch
   const bool lowp = opt.precision == 1 && !dual_centering() && mu >= opt.mu_switch && !force_fp64 &&
-                    dense_assembly_gflop() >= opt.lowp_min_gflop;
+                    lowp_eligible();
   bool any_lowp = false;
   if (lowp) MSDP_TRY(ensure_mixed());
   MSDP_TRY(fill(st, m * m, 0.0, Hs));

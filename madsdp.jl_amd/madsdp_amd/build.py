@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(one, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
-           f"-L{ROCM}/lib", "-lrocblas", f"-Wl,-rpath,{ROCM}/lib"]
+           f"-L{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

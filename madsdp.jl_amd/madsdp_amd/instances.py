@@ -2,8 +2,10 @@
 
 The graphs and data are drawn with numpy's PCG64 `default_rng(seed)`; they are
 this project's own instances (the reference benchmark's `sprand` stream,
-`benchmark/maxcut.jl:19-26`, is Julia-specific), generated the same way on
-every machine.
+`benchmark/maxcut.jl:19-26`, is Julia-specific), bit-identical on every
+machine: the derived data (C, b, d) are accumulated in a fixed order with
+correctly rounded elementwise operations, never by a BLAS call whose
+summation order depends on the host CPU.
 
 * ``maxcut(n, p, seed, weights)`` -- min <-L/4, X>  s.t. diag(X) = 1, X PSD,
   with L the Laplacian of an Erdos-Renyi graph (strict upper triangle drawn,
@@ -16,6 +18,8 @@ every machine.
   each constraint touches 1-3 random blocks densely, data built as above.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
@@ -56,14 +60,29 @@ def _sym_normal(rng, k, n):
     return A
 
 
+def _combination(coef, mats):
+    """sum_j coef_j mats_j accumulated in index order, one correctly rounded
+    multiply and add per entry and term: the same bits on every machine (a
+    BLAS product's summation order depends on the host CPU)."""
+    out = np.zeros(mats.shape[1:])
+    for c, M in zip(coef, mats):
+        out += c * M
+    return out
+
+
+def _rhs(mats, Clin):
+    """b_j = tr(A_j) + sum(a_j), each exactly rounded (math.fsum)."""
+    return np.array([math.fsum(np.concatenate((np.diagonal(A), a))) for A, a in zip(mats, Clin)])
+
+
 def dense_sdp(n: int, m: int, n_s: int, seed: int) -> Model:
     rng = np.random.default_rng(seed)
     mats = _sym_normal(rng, m, n)
     Clin = rng.random((m, n_s))
-    b = np.trace(mats, axis1=1, axis2=2) + Clin.sum(axis=1)
+    b = _rhs(mats, Clin)
     y0 = rng.standard_normal(m)
-    C = np.tensordot(y0, mats, axes=1) + np.eye(n)
-    d = Clin.T @ y0 + 1.0
+    C = _combination(y0, mats) + np.eye(n)
+    d = _combination(y0, Clin) + 1.0
     blk = BlockConstraints("dense", n, con_idx=np.arange(m, dtype=np.int64), mats=mats)
     return Model([C], [blk], b, d, Clin)
 
@@ -80,15 +99,18 @@ def block_sdp(sizes, m: int, n_s: int, seed: int) -> Model:
     blocks, Cs = [], []
     b = np.zeros(m)
     y0 = rng.standard_normal(m)
+    parts = [[] for _ in range(m)]  # the terms of each b_j, summed exactly below
     for i, n in enumerate(sizes):
         idx = np.array(sorted(touch[i]), dtype=np.int64)
         mats = _sym_normal(rng, idx.size, n)
-        b[idx] += np.trace(mats, axis1=1, axis2=2)
-        Cs.append(np.tensordot(y0[idx], mats, axes=1) + np.eye(n))
+        for j, A in zip(idx, mats):
+            parts[j].append(np.diagonal(A))
+        Cs.append(_combination(y0[idx], mats) + np.eye(n))
         blocks.append(BlockConstraints("dense", n, con_idx=idx, mats=mats))
     Clin = rng.random((m, n_s))
-    b += Clin.sum(axis=1)
-    d = Clin.T @ y0 + 1.0
+    for j in range(m):
+        b[j] = math.fsum(np.concatenate(parts[j] + [Clin[j]]))
+    d = _combination(y0, Clin) + 1.0
     return Model(Cs, blocks, b, d, Clin)
 
 

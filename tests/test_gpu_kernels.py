@@ -38,8 +38,10 @@ def P(t):
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 3, 129),
                                    (300, 260, 40), (64, 200, 3000), (3, 1, 5000), (600, 520, 64),
-                                   (1030, 700, 301)])
+                                   (1030, 700, 301), (2200, 2100, 200), (3000, 3000, 3000)])
 def test_gemm(env, ta, tb, M, N, K):
+    """The two largest shapes have more 128-tiles than CUs: the stream-K
+    kernel and its fixup (the max-cut n = 3000 products)."""
     torch, lib, st = env
     rng = np.random.default_rng(M * 7 + N + K)
     A = rng.standard_normal((K, M) if ta else (M, K))
