@@ -219,6 +219,19 @@ int msdp_server_destroy(msdp_server* v);
 int msdp_k_gemm(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                 double beta, double* C, int64_t ldc);
+/* msdp_k_gemm that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial results of the stream-K and split-K paths go to one library-owned
+ * device array of msdp_k_gemm_noalloc_scratch_bytes() (128 MiB) per device
+ * instead of the stream-ordered pool; a product whose partials exceed it
+ * returns -2 with the sizes in msdp_last_error().  Results are bit-identical
+ * to msdp_k_gemm's.  Every call on a device shares that device's array, so
+ * calls that overlap on one device (other streams, other threads) race. */
+int msdp_k_gemm_noalloc(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N,
+                        int32_t K, double alpha, const double* A, int64_t lda, const double* B,
+                        int64_t ldb, double beta, double* C, int64_t ldc);
+size_t msdp_k_gemm_noalloc_scratch_bytes(void);
 /* lower triangle of C = alpha op(A)' op(A) + beta C, upper triangle untouched
  * (BLAS dsyrk; the Schur assembly H = B'B behind LRO.schur_complement!,
  * src/solver.jl:304); trans = 1: C = A' A with A K x N; trans = 0: C = A A' with A N x K */

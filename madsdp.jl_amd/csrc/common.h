@@ -165,6 +165,12 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
          const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
          double* C, int64_t ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0,
          int64_t sC = 0, bool lower = false);
+// the same product with its partials in the library's per-device scratch
+// array (no allocation; calls that overlap on one device race)
+int gemm_noalloc(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc);
+size_t gemm_noalloc_scratch_bytes();
 // Lower Cholesky in place (reads the lower triangle).  *info_dev receives 0 or
 // the 1-based index of the first non-positive pivot; kernels after a failure
 // return immediately.

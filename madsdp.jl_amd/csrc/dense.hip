@@ -529,12 +529,31 @@ constexpr int SK_OCC = MSDP_SK_OCC;
 
 size_t gemm_sk_partials_bytes() { return sizeof(double) * 2 * (size_t)SK_OCC * device_cus() * BT * BT; }
 
+// Partials of the stream-K and split-K paths: the caller's workspace `ws` of
+// `ws_elems` doubles, or (ws == nullptr) a block of the stream-ordered pool.
+static int partials(hipStream_t s, size_t elems, double* ws, size_t ws_elems, double** out) {
+  if (ws) {
+    if (elems > ws_elems) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "GEMM partials need %zu bytes, the scratch holds %zu",
+               elems * sizeof(double), ws_elems * sizeof(double));
+      set_error(buf);
+      return -2;
+    }
+    *out = ws;
+    return 0;
+  }
+  void* P = nullptr;
+  MSDP_HIP(hipMallocAsync(&P, sizeof(double) * elems, s));
+  *out = static_cast<double*>(P);
+  return 0;
+}
+
 static int gemm_stream_k(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
                          const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                         double* C, int64_t ldc, const SkGrid& sg) {
-  void* P = nullptr;
-  MSDP_HIP(hipMallocAsync(&P, sizeof(double) * 2 * (size_t)sg.G * BT * BT, s));
-  double* Pd = static_cast<double*>(P);
+                         double* C, int64_t ldc, const SkGrid& sg, double* ws, size_t ws_elems) {
+  double* Pd = nullptr;
+  MSDP_TRY(partials(s, 2 * (size_t)sg.G * BT * BT, ws, ws_elems, &Pd));
   const dim3 g(sg.G);
   if (!ta && !tb) gemm_sk_f64_kernel<false, false><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
   else if (ta && !tb) gemm_sk_f64_kernel<true, false><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
@@ -542,14 +561,15 @@ static int gemm_stream_k(hipStream_t s, bool ta, bool tb, int M, int N, int K, d
   else gemm_sk_f64_kernel<true, true><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, Pd);
   gemm_sk_fixup_kernel<<<sg.ntm * sg.ntn, 256, 0, s>>>(M, N, alpha, beta, C, ldc, sg, Pd);
   MSDP_HIP(hipGetLastError());
-  MSDP_HIP(hipFreeAsync(P, s));
+  if (!ws) MSDP_HIP(hipFreeAsync(Pd, s));
   return 0;
 }
 
 static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                         double* C, int64_t ldc, int batch, int64_t sA, int64_t sB, int64_t sC,
-                        bool lower, const int* guard) {
+                        bool lower, const int* guard, double* ws = nullptr,
+                        size_t ws_elems = 0) {
   if (M <= 0 || N <= 0 || batch <= 0) return 0;
   if (K <= 0) {
     // C = beta C
@@ -567,7 +587,8 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   SkGrid sg;
   if (stream_k_shape(M, N, K, batch, lower, guard, SK_OCC * device_cus(), sg)) {
     if (prof) prof_before(s);
-    MSDP_TRY(gemm_stream_k(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg));
+    MSDP_TRY(gemm_stream_k(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, ws,
+                           ws_elems));
     if (prof) prof_after(s, 2.0 * M * N * (double)K);
     return 0;
   }
@@ -579,11 +600,10 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
                                        K / SPLITK_MIN_CHUNK);
     const int kc = ((K + parts - 1) / parts + GK - 1) / GK * GK;
     parts = (K + kc - 1) / kc;
-    void* P = nullptr;
-    MSDP_HIP(hipMallocAsync(&P, sizeof(double) * (size_t)M * N * parts, s));
+    double* Pd = nullptr;
+    MSDP_TRY(partials(s, (size_t)M * N * parts, ws, ws_elems, &Pd));
     if (prof) prof_before(s);
     dim3 g3(cdiv(M, GT), cdiv(N, GT), parts);
-    double* Pd = static_cast<double*>(P);
     if (!ta && !tb) gemm_splitk_kernel<false, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
     else if (ta && !tb) gemm_splitk_kernel<true, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
     else if (!ta && tb) gemm_splitk_kernel<false, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
@@ -592,7 +612,7 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
                                                                      lower);
     if (prof) prof_after(s, 2.0 * M * N * (double)K * (lower ? 0.5 : 1.0));
     MSDP_HIP(hipGetLastError());
-    MSDP_HIP(hipFreeAsync(P, s));
+    if (!ws) MSDP_HIP(hipFreeAsync(Pd, s));
     return 0;
   }
   if (prof) prof_before(s);
@@ -652,6 +672,47 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, con
   }
   return gemm_guarded(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC,
                       lower, nullptr);
+}
+
+// --------------------------------------------------------------------------
+// GEMM that allocates nothing: the partials of the stream-K and split-K
+// paths live in one file-scope device array (NOALLOC_SCRATCH_BYTES per
+// device, held until the process ends).  Its address on each device is
+// looked up once, with that device current, and kept; host code never
+// passes the array's own name (a host-side stand-in) to a kernel.  Every
+// call on a device shares that device's array: calls that overlap on one
+// device race.  Nothing in it needs to start at zero (the kernels write
+// every partial they read).
+// --------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static int noalloc_scratch(double** out) {
+  static std::mutex mu;
+  static std::vector<double*> ptrs;
+  int dev = 0;
+  MSDP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    ptrs[dev] = static_cast<double*>(p);
+  }
+  *out = ptrs[dev];
+  return 0;
+}
+
+size_t gemm_noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+int gemm_noalloc(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc) {
+  double* ws = nullptr;
+  MSDP_TRY(noalloc_scratch(&ws));
+  return gemm_guarded(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, 0, 0, 0, false,
+                      nullptr, ws, NOALLOC_SCRATCH_BYTES / sizeof(double));
 }
 
 // ==========================================================================

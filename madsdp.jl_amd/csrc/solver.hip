@@ -2234,6 +2234,13 @@ int msdp_k_gemm(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int3
                 int64_t ldc) {
   return gemm((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
+int msdp_k_gemm_noalloc(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K,
+                        double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                        double beta, double* C, int64_t ldc) {
+  return gemm_noalloc((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta,
+                      C, ldc);
+}
+size_t msdp_k_gemm_noalloc_scratch_bytes(void) { return gemm_noalloc_scratch_bytes(); }
 int msdp_k_syrk_lower(void* stream, int32_t trans, int32_t N, int32_t K, double alpha,
                       const double* A, int64_t lda, double beta, double* C, int64_t ldc) {
   // C = alpha X X' + beta C with X = op(A)' (N x K): trans -> X = A' (A is K x N)

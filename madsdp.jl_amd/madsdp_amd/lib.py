@@ -106,6 +106,10 @@ SIGNATURES = {
     "msdp_server_destroy": (C.c_int, [P]),
     "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
+    "msdp_k_gemm_noalloc": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_double, P, C.c_int64, P, C.c_int64, C.c_double, P,
+                                      C.c_int64]),
+    "msdp_k_gemm_noalloc_scratch_bytes": (C.c_size_t, []),
     "msdp_k_syrk_lower": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_double, P, C.c_int64,
                                     C.c_double, P, C.c_int64]),
     "msdp_k_potrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),
