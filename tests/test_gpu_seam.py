@@ -50,6 +50,9 @@ class HostDriver:
         self.S = [b.S.copy() for b in ref.blocks]
         self.y, self.x, self.z = ref.y.copy(), ref.x.copy(), ref.z.copy()
         self.nu = sum(model.dims) + model.n_scalar
+        self.dX = [np.zeros_like(X) for X in self.X]
+        self.dS = [np.zeros_like(X) for X in self.X]
+        self.dx, self.dz = np.zeros(model.n_scalar), np.zeros(model.n_scalar)
         ops = [O._BlockOps(model.A[i], model.m) for i in range(model.nblocks)]
         self.Aj = [[o.dense_matrix(j) for j in range(model.m)] for o in ops]
 
@@ -96,7 +99,7 @@ class HostDriver:
 
     def complementarity(self, a=0.0, b=0.0):
         s = 0.0
-        for X, S, dX, dS in zip(self.X, self.S, getattr(self, "dX", self.X), getattr(self, "dS", self.S)):
+        for X, S, dX, dS in zip(self.X, self.S, self.dX, self.dS):
             s += np.vdot(X + a * dX, S + b * dS)
         if self.model.n_scalar:
             s += (self.x + a * self.dx) @ (self.z + b * self.dz)
