@@ -46,7 +46,9 @@ enum {
   MSDP_INIT_POINT = 1,          /* InitialPoint    components.jl:243-251 */
   MSDP_INIT_DUAL_SHIFT = 2,     /* DualShift       components.jl:228-232 */
   MSDP_SCHUR_DENSE_CHOLESKY = 0,/* DenseCholesky   components.jl:273 */
-  MSDP_SCHUR_QR = 1             /* QRSchur         components.jl:300 */
+  MSDP_SCHUR_QR = 1,            /* QRSchur         components.jl:300 */
+  MSDP_SCHUR_LDLT = 2           /* MadNLPSchur     components.jl:275-283, kkt.jl:7-32:
+                                   Bunch-Kaufman LDL' on the device */
 };
 
 /* MadSDPOptions (src/options.jl:3-15) and the component parameters. */
@@ -79,7 +81,7 @@ typedef struct msdp_options {
    * over the blocks): below it the refinement costs more than the FP32
    * assembly saves and the solve stays FP64; it also stays FP64 when m
    * exceeds the sum of the blocks' n(n+1)/2 and n_s (a singular H) */
-  double lowp_min_gflop;           /* 50 */
+  double lowp_min_gflop;           /* 500 */
 } msdp_options;
 
 /* iteration data of MadSDPSolver (src/solver.jl:37-48) */
@@ -250,6 +252,17 @@ size_t msdp_k_cholesky_factor_scratch(int32_t n);
 int msdp_k_trtri(void* stream, int32_t n, const double* L, int64_t ldl, double* Linv,
                  int64_t ldi, double* work);
 size_t msdp_k_trtri_work(int32_t n);
+/* symmetric indefinite LDL' with Bunch-Kaufman pivoting of the lower triangle
+ * in place (LAPACK sytrf 'L', behind MadNLPSchur's BUNCHKAUFMAN solver,
+ * src/kkt.jl:15-25): ipiv as LAPACK's (1-based, negative for 2x2 pivots),
+ * work 2n doubles, ctl msdp_k_sytrf_ctl_bytes() of device memory, *info_dev =
+ * first exactly zero pivot (1-based) or 0 */
+int msdp_k_sytrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* ipiv, void* ctl,
+                 double* work, int32_t* info_dev);
+size_t msdp_k_sytrf_ctl_bytes(void);
+/* b = A^-1 b with that factorisation (LAPACK sytrs 'L') */
+int msdp_k_sytrs(void* stream, int32_t n, const double* A, int64_t lda, const int32_t* ipiv,
+                 double* b);
 /* eigenvalues ascending (+ vectors) of a symmetric matrix (LAPACK syevd / LRO.eigen!) */
 int msdp_k_syevd(void* stream, int32_t n, const double* A, int64_t lda, double* w, double* V,
                  int64_t ldv, int32_t vectors, double* work, int32_t* iwork);

@@ -171,6 +171,12 @@ int gemm_noalloc(hipStream_t s, bool ta, bool tb, int M, int N, int K, double al
                  const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                  double* C, int64_t ldc);
 size_t gemm_noalloc_scratch_bytes();
+// Bunch-Kaufman LDL' of the lower triangle (ldlt.hip): work 2n doubles, ctl
+// sytrf_bk_ctl_bytes(); *info_dev = first zero pivot (1-based) or 0
+int sytrf_bk(hipStream_t s, int n, double* A, int64_t lda, int* ipiv, void* ctl, double* work,
+             int* info_dev);
+size_t sytrf_bk_ctl_bytes();
+int sytrs_bk(hipStream_t s, int n, const double* A, int64_t lda, const int* ipiv, double* b);
 // Lower Cholesky in place (reads the lower triangle).  *info_dev receives 0 or
 // the 1-based index of the first non-positive pivot; kernels after a failure
 // return immediately.

@@ -15,6 +15,10 @@
 //  lambda_min : tridiagonalisation + Sturm-sequence multisection.
 #include "common.h"
 
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include <hip/hip_ext.h>
 
 #include <algorithm>
@@ -740,10 +744,82 @@ static int launch_trd1(hipStream_t s, int count, int n, const double* A0, int64_
   return 0;
 }
 
+static int sytrd_columns(hipStream_t s, int n, const TrdWork& t);
+
+// The column sequence of the blocked reduction (n > TRD1_MAX) has no host
+// read and no data-dependent launch: for a given n and workspace it is the
+// same 2n + n/32 kernels every time.  It is captured once per (device, n,
+// workspace) as a HIP graph and replayed, which removes the host launch cost
+// of its ~6,000 dependent kernels (n = 3000).  Not under live profiling
+// (whose events are stamped per launch), not on the legacy null stream
+// (which cannot be captured).
+namespace {
+struct TrdGraphKey {
+  int device, n;
+  const double* work;
+  bool operator==(const TrdGraphKey& o) const {
+    return device == o.device && n == o.n && work == o.work;
+  }
+};
+struct TrdGraphCache {
+  std::mutex mu;
+  std::vector<std::pair<TrdGraphKey, hipGraphExec_t>> entries;
+};
+TrdGraphCache g_trd_graphs;
+constexpr size_t TRD_GRAPHS_MAX = 64;
+constexpr int TRD_GRAPH_MAX_N = 4096;
+}  // namespace
+
+static int sytrd_columns_graph(hipStream_t s, int n, const TrdWork& t, bool& done) {
+  done = false;
+  // n <= TRD_GRAPH_MAX_N: the panel updates' products then take the plain
+  // tile kernels (no pool allocation inside the capture)
+  if (s == nullptr || n > TRD_GRAPH_MAX_N || prof_active(PROF_TRD, s) || prof_active(PROF_GEMM, s))
+    return 0;
+  TrdGraphKey key{0, n, t.A};
+  MSDP_HIP(hipGetDevice(&key.device));
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_trd_graphs.mu);
+    for (auto& e : g_trd_graphs.entries)
+      if (e.first == key) exec = e.second;
+    if (!exec && g_trd_graphs.entries.size() >= TRD_GRAPHS_MAX) return 0;
+  }
+  if (!exec) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MSDP_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return 0;  // the caller is capturing: launch eagerly
+    MSDP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = sytrd_columns(s, n, t);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != 0) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail_hip(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+    std::lock_guard<std::mutex> lk(g_trd_graphs.mu);
+    g_trd_graphs.entries.emplace_back(key, exec);
+  }
+  MSDP_HIP(hipGraphLaunch(exec, s));
+  done = true;
+  return 0;
+}
+
 static int sytrd(hipStream_t s, int n, const double* A, int64_t lda, const TrdWork& t) {
   if (n <= TRD1_MAX) return launch_trd1(s, 1, n, A, lda, t, A, 0);
   MSDP_TRY(copy_mat(s, n, n, A, lda, t.A, n));
   MSDP_TRY(fill(s, (int64_t)n * n, 0.0, t.V));
+  bool done = false;
+  MSDP_TRY(sytrd_columns_graph(s, n, t, done));
+  if (done) return 0;
+  return sytrd_columns(s, n, t);
+}
+
+static int sytrd_columns(hipStream_t s, int n, const TrdWork& t) {
   // column step: rows (w_{k-1}, column k) then the reflector and products
   auto rows = [&](int k, int j, int k0, int npp) -> int {
     const int start = k >= 0 ? k : k0 + j;

@@ -328,6 +328,13 @@ struct msdp_solver {
   int schur_solve(double* v);
   int factorize_schur(bool& ok);
   double dense_assembly_gflop() const;
+  int hsolve(double* v);
+  bool ldlt() const { return opt.schur == MSDP_SCHUR_LDLT; }
+  // MadNLPSchur: the pivots and the step control live in Ht (unused there)
+  int* ldlt_ipiv() const { return reinterpret_cast<int*>(Ht); }
+  void* ldlt_ctl() const {
+    return reinterpret_cast<char*>(Ht) + ((size_t)m * sizeof(int) + 255) / 256 * 256;
+  }
   bool lowp_eligible() const;
   int schur_scaling(Blk& bk, SchurScaling& sc);
   int apply_E(Blk& b, const double* M, bool diag, double* out);
@@ -514,7 +521,8 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&Cs, mm * nss));
   MSDP_TRY(alloc(&H, mm * mm));
   MSDP_TRY(alloc(&Hs, mm * mm));
-  MSDP_TRY(alloc(&Ht, mm * mm));
+  // Ht: the transposed Cholesky factor, or MadNLPSchur's pivots and control
+  MSDP_TRY(alloc(&Ht, std::max<size_t>(mm * mm, (mm * sizeof(int) + 1024) / sizeof(double))));
   MSDP_TRY(alloc(&Hdinv, potrf_work_elems((int)mm)));
   if (m >= 512) {
     MSDP_TRY(alloc(&Hdi, (size_t)(m / 512) * 512 * 512));
@@ -959,7 +967,7 @@ double msdp_solver::dense_assembly_gflop() const {
 bool msdp_solver::lowp_eligible() const {
   double dim = (double)ns;
   for (const auto& bk : blk) dim += 0.5 * (double)bk.n * (bk.n + 1);
-  return (double)m <= dim && dense_assembly_gflop() >= opt.lowp_min_gflop;
+  return !ldlt() && (double)m <= dim && dense_assembly_gflop() >= opt.lowp_min_gflop;
 }
 
 // factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
@@ -1015,6 +1023,22 @@ int msdp_solver::factorize_schur(bool& ok) {
   for (int t = 0; t < opt.max_schur_reg; ++t) {
     MSDP_TRY(copy_mat(st, (int)m, (int)m, Hs, m, H, m));
     if (reg > 0) MSDP_TRY(add_diag(st, (int)m, H, m, nullptr, reg));
+    if (ldlt()) {
+      // MadNLPSchur: Bunch-Kaufman LDL' (src/kkt.jl:7-32); it fails only on
+      // an exactly zero pivot (src/schur_solvers.jl:27-37)
+      MSDP_TRY(sytrf_bk(st, (int)m, H, m, ldlt_ipiv(), ldlt_ctl(), Hdinv, info));
+      int hinfo = 0;
+      MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+      MSDP_TRY(stream_sync(st));
+      if (hinfo == 0) {
+        reg_used = reg > 0 ? reg : 0.0;
+        h_approx = false;
+        ok = true;
+        return 0;
+      }
+      reg = reg == 0 ? 1e-12 : 10 * reg;
+      continue;
+    }
     MSDP_TRY(potrf(st, (int)m, H, m, info, Hdinv, true));
     int hinfo = 0;
     MSDP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1494,7 +1518,7 @@ int msdp_solver::newton_direction(bool& ok) {  // :89-110
   MSDP_TRY(factorize_schur(ok));
   if (!ok) return 0;
   MSDP_HIP(hipMemcpyAsync(dc_d, dc_g, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dc_d, wm2));
+  MSDP_TRY(hsolve(dc_d));
   MSDP_TRY(dot(st, m, dc_g, dc_d, slot(11), red));
   MSDP_TRY(read_slots());
   dc_lambda = std::sqrt(std::max(hsc[11], 0.0));
@@ -1518,7 +1542,7 @@ int msdp_solver::dc_initialize() {  // :112-143
   MSDP_TRY(newton_direction(ok));
   if (!ok) { set_error("MadSDP: failed to factorize the dual Hessian at y0."); return -3; }
   MSDP_HIP(hipMemcpyAsync(dc_dy, b, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dc_dy, wm2));
+  MSDP_TRY(hsolve(dc_dy));
   MSDP_TRY(dot(st, m, b, dc_dy, slot(12), red));
   MSDP_TRY(dot(st, m, b, dc_d, slot(13), red));
   MSDP_TRY(read_slots());
@@ -1542,7 +1566,7 @@ int msdp_solver::dc_initialize() {  // :112-143
 int msdp_solver::predictor_step(bool& ok) {  // :232-264
   MSDP_TRY(axpby(st, m, 1.0, y, 1.0, dc_d, dc_yhat));
   MSDP_HIP(hipMemcpyAsync(dc_dy, b, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, dc_dy, wm2));
+  MSDP_TRY(hsolve(dc_dy));
   MSDP_TRY(axpby(st, m, dc_t, dc_dy, 0.0, dc_dy, dc_dy));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
   int64_t off = 0;
@@ -1694,11 +1718,18 @@ int msdp_solver::apply_H(const double* v, double* out) {
   return 0;
 }
 
+// The solve with the last factorization of H: Cholesky (DenseCholesky) or
+// Bunch-Kaufman LDL' (MadNLPSchur).
+int msdp_solver::hsolve(double* v) {
+  if (ldlt()) return sytrs_bk(st, (int)m, H, m, ldlt_ipiv(), v);
+  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+}
+
 // schur_solve! (schur_solvers.jl:21).  With an FP32-assembled H, the FP64
 // solution is recovered by iterative refinement against the FP64 operator;
 // if it stalls above 1e-9 relative, H is re-assembled in FP64.
 int msdp_solver::schur_solve(double* v) {
-  if (!h_approx) return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+  if (!h_approx) return hsolve(v);
   MSDP_TRY(form_block_inverses());
   const double* di = hdi_valid ? Hdi : nullptr;
   MSDP_HIP(hipMemcpyAsync(rh, v, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
@@ -1727,7 +1758,7 @@ int msdp_solver::schur_solve(double* v) {
   MSDP_TRY(factorize_schur(ok));
   if (!ok) { set_error("FP64 re-factorisation of the Schur complement failed"); return -4; }
   MSDP_HIP(hipMemcpyAsync(v, rh, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+  return hsolve(v);
 }
 
 // ==========================================================================
@@ -1917,7 +1948,7 @@ void msdp_default_options(msdp_options* o) {
   o->precision = 0;
   o->mu_switch = 1e-5;
   o->refine_max_iter = 10;
-  o->lowp_min_gflop = 50.0;
+  o->lowp_min_gflop = 500.0;
 }
 
 int msdp_solver_create(msdp_solver** out, int device, int32_t nblocks, const int64_t* dims,
@@ -2160,7 +2191,7 @@ int msdp_solver_take_step(msdp_solver* s, double a, double bt) { GUARD_READY(s);
 int msdp_solver_schur_solve(msdp_solver* s, double* rhs) {
   GUARD_READY(s);
   MSDP_HIP(hipMemcpyAsync(s->wm, rhs, sizeof(double) * s->m, hipMemcpyHostToDevice, s->st));
-  MSDP_TRY(potrs(s->st, (int)s->m, s->H, s->Ht, s->m, s->Hdinv, s->wm, s->wm2));
+  MSDP_TRY(s->hsolve(s->wm));
   MSDP_HIP(hipMemcpyAsync(rhs, s->wm, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->st));
   MSDP_TRY(stream_sync(s->st));
   return 0;
@@ -2266,6 +2297,15 @@ int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm
 size_t msdp_k_trtri_work(int32_t n) { return trtri_work_elems(n); }
 int msdp_k_trtri(void* stream, int32_t n, const double* L, int64_t ldl, double* Li, int64_t ldi, double* work) {
   return trtri_lower((hipStream_t)stream, n, L, ldl, Li, ldi, work);
+}
+int msdp_k_sytrf(void* stream, int32_t n, double* A, int64_t lda, int32_t* ipiv, void* ctl,
+                 double* work, int32_t* info) {
+  return sytrf_bk((hipStream_t)stream, n, A, lda, ipiv, ctl, work, info);
+}
+size_t msdp_k_sytrf_ctl_bytes(void) { return sytrf_bk_ctl_bytes(); }
+int msdp_k_sytrs(void* stream, int32_t n, const double* A, int64_t lda, const int32_t* ipiv,
+                 double* b) {
+  return sytrs_bk((hipStream_t)stream, n, A, lda, ipiv, b);
 }
 size_t msdp_k_syevd_work(int32_t n) { return syevd_work_elems(n); }
 size_t msdp_k_syevd_iwork(int32_t n) { return syevd_iwork_elems(n); }

@@ -6,6 +6,6 @@ Julia API (`MadSDPSolver`, `solve!`, the component types) over it.
 """
 from .model import BlockConstraints, Model  # noqa: F401
 from .solver import (AdaptiveMu, CholeskyBacktrack, DenseCholesky, DualCentering,  # noqa: F401
-                     DualShift, EigenStep, HKM, InitialPoint, MadSDPSolver, Mehrotra, NT,
+                     DualShift, EigenStep, HKM, InitialPoint, MadNLPSchur, MadSDPSolver, Mehrotra, NT,
                      PrimalDual, QRSchur, ScaledIdentity, Status, madsdp)
 from .serve import Server, serve  # noqa: F401,E402

@@ -11,8 +11,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "libmadsdp_hip.so"
-SOURCES = ["dense.hip", "gemm32.hip", "eigen.hip", "ops.hip", "qr.hip", "solver.hip", "server.hip",
-           "profile.hip"]
+SOURCES = ["dense.hip", "gemm32.hip", "eigen.hip", "ops.hip", "qr.hip", "ldlt.hip", "solver.hip",
+           "server.hip", "profile.hip"]
 ARCH = os.environ.get("MSDP_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -118,6 +118,9 @@ SIGNATURES = {
     "msdp_k_cholesky_factor_scratch": (C.c_size_t, [C.c_int32]),
     "msdp_k_trtri": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P]),
     "msdp_k_trtri_work": (C.c_size_t, [C.c_int32]),
+    "msdp_k_sytrf": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P, P, P]),
+    "msdp_k_sytrf_ctl_bytes": (C.c_size_t, []),
+    "msdp_k_sytrs": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P]),
     "msdp_k_syevd": (C.c_int, [P, C.c_int32, P, C.c_int64, P, P, C.c_int64, C.c_int32, P, P]),
     "msdp_k_syevd_work": (C.c_size_t, [C.c_int32]),
     "msdp_k_syevd_iwork": (C.c_size_t, [C.c_int32]),

@@ -102,6 +102,12 @@ class DenseCholesky:
     """Dense Cholesky of the Schur complement (components.jl:273)."""
 
 
+class MadNLPSchur:
+    """`MadNLPSchur()` (components.jl:275-283): the Schur complement factored by
+    a symmetric indefinite LDL' with Bunch-Kaufman pivoting (MadNLP's LAPACK
+    solver in BUNCHKAUFMAN mode, kkt.jl:15-25), here on the device."""
+
+
 class QRSchur:
     """QR of the NT-scaled constraint matrix (components.jl:285-300)."""
 
@@ -205,7 +211,7 @@ class MadSDPSolver:
         else:
             opt.initial_point = 2
             opt.initial_shift, opt.max_shift = p.initial_shift, p.max_shift
-        opt.schur = 1 if isinstance(schur, QRSchur) else 0
+        opt.schur = 1 if isinstance(schur, QRSchur) else 2 if isinstance(schur, MadNLPSchur) else 0
         L.check(self.lib.msdp_solver_configure(self._h, C.byref(opt)), "configure")
         self.opt = opt
 

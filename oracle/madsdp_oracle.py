@@ -128,6 +128,10 @@ class DenseCholesky:            # components.jl:273
     pass
 
 
+class MadNLPSchur:              # components.jl:275-283 (LDL' Bunch-Kaufman, kkt.jl:7-32)
+    pass
+
+
 class QRSchur:                  # components.jl:300
     pass
 
@@ -516,6 +520,19 @@ class Solver:
             return self._qr_factorize()
         opt = self.opt
         reg = opt.regularize_schur
+        if isinstance(self.schur, MadNLPSchur):
+            # MadNLP's LAPACK solver in BUNCHKAUFMAN mode (kkt.jl:15-25): sytrf on
+            # the lower triangle; an exactly zero pivot (info > 0) is the
+            # SingularException schur_solvers.jl:27-37 turns into `false`
+            for _ in range(opt.max_schur_reg):
+                H = self.assemble_schur(reg)
+                ldu, piv, info = sla.lapack.dsytrf(H, lower=1)
+                if info == 0 and np.all(np.isfinite(ldu)):
+                    self.H = H
+                    self.cho = (ldu, piv)
+                    return True
+                reg = 1e-12 if reg == 0 else 10 * reg
+            return False
         for _ in range(opt.max_schur_reg):
             H = self.assemble_schur(reg)
             Lh, info = sla.lapack.dpotrf(H, lower=1, clean=1)
@@ -527,6 +544,10 @@ class Solver:
         return False
 
     def schur_solve(self, rhs):
+        if isinstance(self.schur, MadNLPSchur):           # sytrs (kkt.jl:27-32)
+            ldu, piv = self.cho
+            x, info = sla.lapack.dsytrs(ldu, piv, rhs, lower=1)
+            return x
         L = self.cho
         t = sla.solve_triangular(L, rhs, lower=True, check_finite=False)
         return sla.solve_triangular(L, t, lower=True, trans="T", check_finite=False)

@@ -180,6 +180,66 @@ def test_syevd(env, n, kind):
     np.testing.assert_allclose(A @ Vh, Vh * wh, rtol=0, atol=1e-11 * scale * max(1, n / 64))
 
 
+def test_syevd_graph_replay(env):
+    """n > 640 on a created stream: the blocked tridiagonalisation's column
+    sequence is captured as a HIP graph on the first call and replayed on the
+    next ones (same workspace).  A second matrix in between, then the first
+    again: bit-identical to the first call, every call exact against numpy.
+    n = 3000 is the max-cut workload's block."""
+    torch, lib, _ = env
+    stream = torch.cuda.Stream()
+    st = C.c_void_p(stream.cuda_stream)
+    for n in (1100, 3000):
+        rng = np.random.default_rng(n + 1)
+        mats = []
+        for _ in range(2):
+            A = rng.standard_normal((n, n))
+            mats.append((A + A.T) / 2)
+        w = torch.zeros(n, dtype=torch.float64, device="cuda")
+        V = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+        work = torch.zeros(int(lib.msdp_k_syevd_work(n)), dtype=torch.float64, device="cuda")
+        iwork = torch.zeros(int(lib.msdp_k_syevd_iwork(n)), dtype=torch.int32, device="cuda")
+        outs = []
+        for k in (0, 1, 0):
+            A = mats[k]
+            dA = dev(torch, A)
+            print(f"syevd graph n={n} call matrix {k}", flush=True)
+            with torch.cuda.stream(stream):
+                assert lib.msdp_k_syevd(st, n, P(dA), n, P(w), P(V), n, 1, P(work), P(iwork)) == 0
+            stream.synchronize()
+            wh, Vh = w.cpu().numpy(), host(V, (n, n))
+            outs.append((wh, Vh))
+            ref = np.linalg.eigvalsh(A)
+            scale = max(1.0, np.abs(ref).max())
+            np.testing.assert_allclose(wh, ref, rtol=0, atol=1e-12 * scale * n / 64)
+            np.testing.assert_allclose(A @ Vh, Vh * wh, rtol=0, atol=1e-11 * scale * n / 64)
+        assert np.array_equal(outs[0][0], outs[2][0]) and np.array_equal(outs[0][1], outs[2][1])
+
+
+def test_syevd_deflation_global_path(env):
+    """n = 4500: the last divide-and-conquer merge has K > 4096 entries, so
+    the deflation and secular kernels keep their vectors in global memory
+    instead of LDS."""
+    torch, lib, st = env
+    n = 4500
+    rng = np.random.default_rng(45)
+    A = rng.standard_normal((n, n))
+    A = (A + A.T) / 2
+    dA = dev(torch, A)
+    w = torch.zeros(n, dtype=torch.float64, device="cuda")
+    V = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    work = torch.zeros(int(lib.msdp_k_syevd_work(n)), dtype=torch.float64, device="cuda")
+    iwork = torch.zeros(int(lib.msdp_k_syevd_iwork(n)), dtype=torch.int32, device="cuda")
+    print(f"syevd n={n}", flush=True)
+    assert lib.msdp_k_syevd(st, n, P(dA), n, P(w), P(V), n, 1, P(work), P(iwork)) == 0
+    torch.cuda.synchronize()
+    wh, Vh = w.cpu().numpy(), host(V, (n, n))
+    ref = np.linalg.eigvalsh(A)
+    scale = max(1.0, np.abs(ref).max())
+    np.testing.assert_allclose(wh, ref, rtol=0, atol=1e-12 * scale * n / 64)
+    np.testing.assert_allclose(A @ Vh, Vh * wh, rtol=0, atol=1e-11 * scale * n / 64)
+
+
 @pytest.mark.parametrize("n,kind", [(3, "random"), (64, "random"), (65, "random"), (200, "random"),
                                     (400, "step"), (500, "random"), (640, "cluster"),
                                     (641, "random"), (1000, "random"), (1000, "cluster"),

@@ -301,3 +301,34 @@ def test_workload_instances_feasible():
         s.x = np.ones(model.n_scalar)
         s.update_residuals()
         assert np.abs(s.Rp).max() <= 1e-10 * max(1, np.abs(model.b).max())
+
+
+@pytest.mark.parametrize("model,expected", [(lambda: K.maxcut(K.W_LRO), -18.0), (K.mixed, 1.0),
+                                            (K.trivial, 2.0)], ids=["maxcut", "mixed", "trivial"])
+def test_madnlp_schur_kats(model, expected):
+    """runtests.jl:186-202 runs MadNLPSchur on the same problems."""
+    s = O.Solver(model(), schur=O.MadNLPSchur()).solve()
+    assert s.status == O.Status.SOLVE_SUCCEEDED
+    assert abs(s.primal_objective() - expected) <= 1e-4
+
+
+def test_madnlp_schur_mc_psd_start_and_dual_centering():
+    """runtests.jl:186-202 (dual feasibility kept from the mc_psd start) and
+    dual_runtests.jl:88-94 (DualCentering with MadNLPSchur)."""
+    X0, y0 = K.mc_psd_start(K.W_LRO)
+    s = O.Solver(K.maxcut(K.W_LRO), direction=O.HKM(), step=O.CholeskyBacktrack(),
+                 centering=O.AdaptiveMu(), initial_point=O.InitialPoint(X0, y0),
+                 schur=O.MadNLPSchur()).solve()
+    assert s.status == O.Status.SOLVE_SUCCEEDED and s.inf_du < 1e-12
+    s = O.Solver(K.maxcut(K.W_LRO), algorithm=O.DualCentering(), schur=O.MadNLPSchur(),
+                 tol=1e-6, max_iter=300).solve()
+    assert abs(s.dual_objective() + 18.0) <= 1e-3
+
+
+def test_madnlp_schur_solve_is_exact():
+    """The Bunch-Kaufman solve against a dense solve of the regularised H."""
+    s = O.Solver(instances.dense_sdp(8, 20, 3, 6), schur=O.MadNLPSchur())
+    s.initialize()
+    assert s.prepare() and s.factorize_schur()
+    rhs = np.arange(1.0, 21.0)
+    np.testing.assert_allclose(s.schur_solve(rhs), np.linalg.solve(s.H, rhs), rtol=1e-10)
