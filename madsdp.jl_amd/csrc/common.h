@@ -3,6 +3,7 @@
 // (Julia's `Matrix{Float64}` layout), leading dimension given explicitly.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <functional>
 #include <vector>
@@ -11,6 +12,15 @@
 #include <string>
 
 namespace msdp {
+
+// roctx range over one stage of the interior-point iteration, shown by
+// rocprofv3 --marker-trace (a no-op when no tool is attached)
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 

@@ -823,6 +823,7 @@ int msdp_solver::rec_flush() {
 // ==========================================================================
 // update_residuals! (step.jl:147-182)
 int msdp_solver::update_residuals() {
+  Range range_("madsdp:update_residuals");
   if (m > 0) {
     MSDP_TRY(fill(st, m, 0.0, Rp));
     for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.X, Rp));
@@ -855,6 +856,7 @@ int msdp_solver::update_residuals() {
 // _prepare! (primal_dual.jl:29-37): prepare_block!, prepare_direction!,
 // prepare_step! for every block, then zinv.
 int msdp_solver::prepare(bool& ok) {
+  Range range_("madsdp:prepare");
   MSDP_TRY(fill(st, 1, 0.0, slot(2)));
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * (4 * nb + 8), st));
   MSDP_TRY(rec_begin());
@@ -972,6 +974,7 @@ bool msdp_solver::lowp_eligible() const {
 
 // factorize_schur! (solver.jl:299-314) + factorize!(DenseCholesky)
 int msdp_solver::factorize_schur(bool& ok) {
+  Range range_("madsdp:factorize_schur");
   ok = false;
   if (opt.schur == MSDP_SCHUR_QR) return qr_factorize(ok);
   if (m == 0) { ok = true; return 0; }
@@ -1083,6 +1086,7 @@ int msdp_solver::apply_E(Blk& bk, const double* M, bool diag, double* out) {
 // solve_directions! (primal_dual.jl:44-54) = rhs! (step.jl:198-224),
 // schur_solve! (schur_solvers.jl:21), directions! (step.jl:229-259)
 int msdp_solver::solve_directions(double mu_, bool corrected) {
+  Range range_("madsdp:solve_directions");
   if (opt.schur == MSDP_SCHUR_QR) return qr_solve_directions(mu_, corrected);
   // rhs
   if (m > 0) {
@@ -1140,6 +1144,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
 
 // correction! (step.jl:263-271; directions.jl:90-100 NT, 123-127 HKM)
 int msdp_solver::correction() {
+  Range range_("madsdp:correction");
   MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
@@ -1165,6 +1170,7 @@ int msdp_solver::correction() {
 
 // step_lengths (step.jl:186-198 EigenStep, 200-228 CholeskyBacktrack)
 int msdp_solver::step_lengths(double& a, double& bt) {
+  Range range_("madsdp:step_lengths");
   if (opt.step == MSDP_STEP_EIGEN) {
     MSDP_TRY(rec_begin());
     for (int i = 0; i < nb; ++i) {
@@ -1283,6 +1289,7 @@ int msdp_solver::total_complementarity(double& out) {
 
 // _take_step! (primal_dual.jl:56-72)
 int msdp_solver::take_step(double a, double bt) {
+  Range range_("madsdp:take_step");
   alpha = a;
   beta = bt;
   MSDP_TRY(rec_begin());
@@ -1307,6 +1314,7 @@ int msdp_solver::take_step(double a, double bt) {
 
 // iterate! (primal_dual.jl:85-121)
 int msdp_solver::iterate() {
+  Range range_("madsdp:iterate");
   if (dual_centering()) return dc_iterate();
   bool ok1 = false, ok2 = false;
   MSDP_TRY(prepare(ok1));
@@ -1347,6 +1355,7 @@ int msdp_solver::iterate() {
 
 // update_termination! (primal_dual.jl:123-135, dual_centering.jl:293-303)
 int msdp_solver::update_termination() {
+  Range range_("madsdp:update_termination");
   if (dual_centering()) MSDP_TRY(update_residuals());
   MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
@@ -1467,6 +1476,7 @@ int msdp_solver::initial_point() {
 
 // initialize! (solver.jl:286-293) + initialize_algorithm!
 int msdp_solver::initialize() {
+  Range range_("madsdp:initialize");
   MSDP_TRY(finalize());
   MSDP_TRY(initial_point());
   iter = 0;
@@ -1618,6 +1628,7 @@ int msdp_solver::predictor_step(bool& ok) {  // :232-264
 }
 
 int msdp_solver::dc_iterate() {  // :266-289
+  Range range_("madsdp:dc_iterate");
   bool ok = false;
   MSDP_TRY(newton_direction(ok));
   if (!ok) { status = ERROR_IN_STEP_COMPUTATION; return 0; }
@@ -1766,6 +1777,7 @@ int msdp_solver::schur_solve(double* v) {
 // ==========================================================================
 // factorize_schur! (qr_schur.jl:24-88)
 int msdp_solver::qr_factorize(bool& ok) {
+  Range range_("madsdp:qr_factorize");
   ok = false;
   int64_t rows = ns;
   for (auto& bk : blk) rows += (int64_t)bk.n * (bk.n + 1) / 2;
@@ -1824,6 +1836,7 @@ int msdp_solver::qr_factorize(bool& ok) {
 
 // solve_directions! (qr_schur.jl:90-160)
 int msdp_solver::qr_solve_directions(double mu_, bool corrected) {
+  Range range_("madsdp:qr_solve_directions");
   const int64_t rows = qrows;
   MSDP_TRY(fill(st, rows, 0.0, qbase));
   int64_t off = 0;

@@ -27,14 +27,33 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> Path:
-    objdir = PKG.parent.parent / "_build"
-    objdir.mkdir(exist_ok=True)
+# host-only AddressSanitizer build of the library's non-kernel code (the
+# solver driver, the serving loop's threads, the recorder's pinned ring):
+# every -fsanitize= applies to the host compilation only
+ASAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+ASAN_LIB = PKG / "libmadsdp_hip_asan.so"
+
+
+def asan_runtime() -> str:
+    """Path of the compiler's shared ASan runtime.  The ASan build loads it as
+    a dependency; an uninstrumented Python then needs
+    ASAN_OPTIONS=verify_asan_link_order=0 (the runtime is not first in the
+    library list)."""
+    r = subprocess.run([f"{ROCM}/llvm/bin/clang", "-print-file-name=libclang_rt.asan-x86_64.so"],
+                       capture_output=True, text=True)
+    return r.stdout.strip()
+
+
+def build(verbose: bool = False, asan: bool = False) -> Path:
+    objdir = PKG.parent.parent / "_build" / ("asan" if asan else "")
+    objdir.mkdir(parents=True, exist_ok=True)
     cc = hipcc()
+    extra = ASAN_FLAGS if asan else []
+    lib = ASAN_LIB if asan else LIB
 
     def one(src: str) -> Path:
         obj = objdir / (Path(src).stem + ".o")
-        cmd = [cc, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [cc, *FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")
@@ -44,16 +63,20 @@ def build(verbose: bool = False) -> Path:
 
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(one, SOURCES))
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
-           f"-L{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib"]
+    tmp = lib.with_suffix(".so.tmp")
+    link_extra = []
+    if asan:  # the shared ASan runtime as a dependency of the library, found by rpath
+        link_extra = ["-Xarch_host", "-fsanitize=address", "-shared-libasan",
+                      f"-Wl,-rpath,{os.path.dirname(asan_runtime())}"]
+    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), *link_extra, "-o",
+           str(tmp), f"-L{ROCM}/lib", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM}/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, asan="--asan" in sys.argv))
     sys.exit(0)

@@ -10,7 +10,10 @@ import ctypes as C
 import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libmadsdp_hip.so"
+# MADSDP_HIP_LIB selects another build of the same library (the host-ASan
+# one, libmadsdp_hip_asan.so)
+LIB_PATH = Path(os.environ.get("MADSDP_HIP_LIB") or
+                Path(__file__).resolve().parent / "libmadsdp_hip.so")
 
 c_double_p = C.POINTER(C.c_double)
 c_int64_p = C.POINTER(C.c_int64)
