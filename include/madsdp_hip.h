@@ -91,6 +91,10 @@ typedef struct msdp_info {
   int32_t iter, status, n_newton, n_predictor;
   double dc_t, dc_lambda;
   int32_t lowp_factorizations, fp64_fallbacks, refine_steps, pad;
+  /* device bytes the solver holds now (its allocations and the recorder's
+   * argument buffer; GEMM partials taken from the stream-ordered pool for the
+   * duration of one product are not counted) -- compare msdp_solver_footprint */
+  int64_t device_bytes;
 } msdp_info;
 
 /* which iterate array msdp_solver_get/set moves */
@@ -164,6 +168,14 @@ int msdp_solver_profile_end(msdp_solver* s, double* total_ms, int64_t* launches,
 /* the same over every stream of the process (the serving loop's concurrent
  * solves); class 3 = the one-workgroup tridiagonalisation (n <= 640).  No
  * counterpart in the reference. */
+/* Kernel-selection parameters (no counterpart in the reference): the GEMM
+ * tile / split-K / stream-K choices, by name -- gemm_wide_min_tiles,
+ * gemm_wide_deep_tiles, splitk_workgroups, splitk_min_k, streamk_occupancy,
+ * streamk_min_slabs.  The autotuner's table (madsdp_amd/tuning_gfx950.json)
+ * is applied through msdp_set_tuning once, before any solver runs; -2 for an
+ * unknown name or a value out of range. */
+int msdp_set_tuning(const char* name, double value);
+int msdp_get_tuning(const char* name, double* value);
 int msdp_profile_begin(int32_t kernel_class);
 int msdp_profile_end(double* total_ms, int64_t* launches, double* work);
 /* Run the solver on a caller-owned HIP stream (before msdp_solver_initialize;
@@ -187,9 +199,11 @@ int msdp_solver_release(msdp_solver* s);
  * src/solver.jl:385-396).  Requests are prepared solvers (problem data set,
  * options configured, nothing on the device yet) with an arrival time in
  * seconds after msdp_server_start.  `slots` worker threads each own a HIP
- * stream; a request is admitted in submission order once it has arrived, a
- * slot is free and its footprint fits in the memory budget (a request larger
- * than the budget runs alone).  An admitted request is uploaded, solved on its
+ * stream; a free slot takes the first request, in submission order, that has
+ * arrived and whose footprint fits in the memory budget next to the running
+ * solves (first fit: a request that does not fit is passed over by smaller
+ * ones for at most one second, then waits at the head until memory frees; a
+ * request larger than the budget runs alone).  An admitted request is uploaded, solved on its
  * slot's stream concurrently with the other slots, and released.  The caller
  * keeps ownership of the solver handles and must not use a submitted one
  * until its request is done. */

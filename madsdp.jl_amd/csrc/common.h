@@ -181,6 +181,11 @@ int gemm_noalloc(hipStream_t s, bool ta, bool tb, int M, int N, int K, double al
                  const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                  double* C, int64_t ldc);
 size_t gemm_noalloc_scratch_bytes();
+// bytes of stream-K partials one product may take from the pool
+size_t gemm_sk_partials_bytes();
+// kernel-selection parameters by name (dense.hip)
+int set_tuning(const char* name, double value);
+int get_tuning(const char* name, double* value);
 // Bunch-Kaufman LDL' of the lower triangle (ldlt.hip): work 2n doubles, ctl
 // sytrf_bk_ctl_bytes(); *info_dev = first zero pivot (1-based) or 0
 int sytrf_bk(hipStream_t s, int n, double* A, int64_t lda, int* ipiv, void* ctl, double* work,

@@ -10,6 +10,8 @@
 #include <array>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -483,7 +485,55 @@ __global__ void splitk_reduce_kernel(int M, int N, int parts, const double* P, d
   *c = beta == 0.0 ? alpha * v : alpha * v + beta * (*c);
 }
 
-constexpr int SPLITK_WGS = 1024, SPLITK_MIN_K = 256, SPLITK_MIN_CHUNK = 64;
+// --------------------------------------------------------------------------
+// Kernel-selection parameters.  Defaults are compiled in; an offline sweep
+// on the target (madsdp_amd/autotune.py) writes a table that the host layer
+// applies once, before any solver runs, through msdp_set_tuning.  Within a
+// process they do not change, so results stay deterministic.
+// --------------------------------------------------------------------------
+namespace {
+struct TuneParam {
+  const char* name;
+  std::atomic<int> value;
+  int lo, hi;
+};
+TuneParam g_tune[] = {
+    {"gemm_wide_min_tiles", {256}, 1, 1 << 20},   // 128-tiles used when this many run ...
+    {"gemm_wide_deep_tiles", {96}, 1, 1 << 20},   // ... or this many when K >= 1024
+    {"splitk_workgroups", {1024}, 64, 1 << 16},   // split-K: workgroups aimed at
+    {"splitk_min_k", {256}, 64, 1 << 20},         // split-K only for K at least this
+    {"streamk_occupancy", {2}, 1, 2},              // stream-K workgroups per CU
+    {"streamk_min_slabs", {8}, 1, 1024},           // stream-K: slabs per workgroup at least
+};
+enum { T_WIDE_MIN = 0, T_WIDE_DEEP, T_SPLITK_WGS, T_SPLITK_MIN_K, T_SK_OCC, T_SK_MIN_UNITS };
+inline int tune(int k) { return g_tune[k].value.load(std::memory_order_relaxed); }
+}  // namespace
+
+int set_tuning(const char* name, double value) {
+  for (auto& p : g_tune) {
+    if (std::strcmp(p.name, name) != 0) continue;
+    if (!(value >= p.lo && value <= p.hi) || value != std::floor(value)) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return -2;
+    }
+    p.value.store((int)value, std::memory_order_relaxed);
+    return 0;
+  }
+  set_error(std::string("no tuning parameter ") + name);
+  return -2;
+}
+
+int get_tuning(const char* name, double* value) {
+  for (auto& p : g_tune)
+    if (std::strcmp(p.name, name) == 0) {
+      *value = p.value.load(std::memory_order_relaxed);
+      return 0;
+    }
+  set_error(std::string("no tuning parameter ") + name);
+  return -2;
+}
+
+constexpr int SPLITK_MIN_CHUNK = 64;
 
 // Compute units of the current device (the stream-K grid: one workgroup per
 // CU, so the partition -- and with it the summation order -- is fixed per
@@ -506,8 +556,7 @@ static int device_cus() {
 
 // Plain products (no batch, no triangle, no guard) whose 128-tile grid would
 // leave a partial last wave go through the stream-K kernel: at least
-// SK_MIN_UNITS slabs per workgroup, and more tiles than CUs.
-constexpr int SK_MIN_UNITS = 8;
+// streamk_min_slabs slabs per workgroup, and more tiles than CUs.
 
 static bool stream_k_shape(int M, int N, int K, int batch, bool lower, const int* guard, int G,
                            SkGrid& sg) {
@@ -518,16 +567,12 @@ static bool stream_k_shape(int M, int N, int K, int batch, bool lower, const int
   sg.G = G;
   sg.U = (int64_t)sg.ntm * sg.ntn * sg.I;
   const int64_t T = (int64_t)sg.ntm * sg.ntn;
-  return T > G && T % G != 0 && sg.U >= (int64_t)SK_MIN_UNITS * G;
+  return T > G && T % G != 0 && sg.U >= (int64_t)tune(T_SK_MIN_UNITS) * G;
 }
 
-#ifndef MSDP_SK_OCC
-#define MSDP_SK_OCC 2
-#endif
-// stream-K workgroups per CU (two of the 256-thread, 67 KB-LDS workgroups fit)
-constexpr int SK_OCC = MSDP_SK_OCC;
-
-size_t gemm_sk_partials_bytes() { return sizeof(double) * 2 * (size_t)SK_OCC * device_cus() * BT * BT; }
+// stream-K partials at the largest occupancy (two of the 256-thread,
+// 67 KB-LDS workgroups fit per CU)
+size_t gemm_sk_partials_bytes() { return sizeof(double) * 2 * (size_t)2 * device_cus() * BT * BT; }
 
 // Partials of the stream-K and split-K paths: the caller's workspace `ws` of
 // `ws_elems` doubles, or (ws == nullptr) a block of the stream-ordered pool.
@@ -585,7 +630,7 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   }
   const bool prof = prof_active(PROF_GEMM, s);
   SkGrid sg;
-  if (stream_k_shape(M, N, K, batch, lower, guard, SK_OCC * device_cus(), sg)) {
+  if (stream_k_shape(M, N, K, batch, lower, guard, tune(T_SK_OCC) * device_cus(), sg)) {
     if (prof) prof_before(s);
     MSDP_TRY(gemm_stream_k(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, ws,
                            ws_elems));
@@ -593,10 +638,10 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     return 0;
   }
   const int64_t tiles64 = (int64_t)cdiv(M, GT) * cdiv(N, GT) * batch;
-  if (!guard && batch == 1 && tiles64 < 128 && K >= SPLITK_MIN_K) {
-    // split K so that about SPLITK_WGS workgroups run, chunks of at least
-    // SPLITK_MIN_CHUNK
-    int parts = (int)std::min<int64_t>(std::max<int64_t>(2, SPLITK_WGS / tiles64),
+  if (!guard && batch == 1 && tiles64 < 128 && K >= tune(T_SPLITK_MIN_K)) {
+    // split K so that about splitk_workgroups workgroups run, chunks of at
+    // least SPLITK_MIN_CHUNK
+    int parts = (int)std::min<int64_t>(std::max<int64_t>(2, tune(T_SPLITK_WGS) / tiles64),
                                        K / SPLITK_MIN_CHUNK);
     const int kc = ((K + parts - 1) / parts + GK - 1) / GK * GK;
     parts = (K + kc - 1) / kc;
@@ -621,7 +666,8 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   // the 64 x 64 kernel's four times as many workgroups finish sooner
   const int64_t tiles128 = (int64_t)cdiv(M, BT) * cdiv(N, BT) * batch;
   const int64_t run128 = lower ? tiles128 / 2 : tiles128;
-  if (M >= BT && N >= BT && (run128 >= 256 || (K >= 1024 && run128 >= 96))) {
+  if (M >= BT && N >= BT &&
+      (run128 >= tune(T_WIDE_MIN) || (K >= 1024 && run128 >= tune(T_WIDE_DEEP)))) {
     dim3 g2(cdiv(M, BT), cdiv(N, BT), batch);
     int lmode = lower ? 1 : 0;
     if (lower && batch == 1 && M == N) {

@@ -1,8 +1,9 @@
 // Serving loop: continuous batching of independent SDP solves on one device.
 //
-// Each of `slots` worker threads owns a HIP stream and takes the next request
-// (in submission order) once it has arrived and its memory footprint fits in
-// the budget next to the solves already running; the solve is uploaded, run
+// Each of `slots` worker threads owns a HIP stream and takes the first request
+// (in submission order) that has arrived and whose memory footprint fits in
+// the budget next to the solves already running (a request that does not fit
+// is passed over for at most HOLD_S, then waits at the head); the solve is uploaded, run
 // and released on the slot's stream, so the admitted solves overlap on the
 // device while requests that arrive meanwhile wait for a slot and memory.
 // Every kernel of a solve is deterministic and runs on that solve's stream
@@ -30,6 +31,11 @@ struct Req {
 };
 
 }  // namespace
+
+// admission: how far past the oldest waiting request first fit may look,
+// and how long a request that does not fit may be passed over
+constexpr size_t ADMIT_WINDOW = 256;
+constexpr double HOLD_S = 1.0;
 
 struct msdp_server {
   int device = 0, slots = 1;
@@ -61,18 +67,28 @@ void msdp_server::work(int slot) {
       std::unique_lock<std::mutex> lk(mu);
       for (;;) {
         if (next < reqs.size()) {
-          Req& h = reqs[next];
+          // first fit in submission order among the requests that have
+          // arrived: a request whose footprint does not fit yet is passed
+          // over by later, smaller ones, unless it has waited HOLD_S already
+          // (then it is admitted first, as soon as memory frees)
           const double now = elapsed();
-          if (now < h.arrival) {
-            cv.wait_for(lk, std::chrono::duration<double>(h.arrival - now));
-            continue;
+          double wait_s = -1.0;
+          for (size_t i = next; i < reqs.size() && i < next + ADMIT_WINDOW; ++i) {
+            Req& h = reqs[i];
+            if (h.admitted >= 0) continue;
+            if (now < h.arrival) {
+              wait_s = h.arrival - now;
+              break;
+            }
+            if (active == 0 || in_use + h.footprint <= budget) {
+              r = &h;
+              break;
+            }
+            if (now - h.arrival > HOLD_S) break;
           }
-          if (active == 0 || in_use + h.footprint <= budget) {
-            r = &h;
-            ++next;
-            break;
-          }
-          cv.wait(lk);  // memory frees when a running solve finishes
+          if (r) break;
+          if (wait_s > 0) cv.wait_for(lk, std::chrono::duration<double>(wait_s));
+          else cv.wait(lk);  // memory frees when a running solve finishes
           continue;
         }
         if (closing) return;
@@ -84,6 +100,7 @@ void msdp_server::work(int slot) {
       peak_active = std::max(peak_active, active);
       r->admitted = elapsed();
       r->slot = slot;
+      while (next < reqs.size() && reqs[next].admitted >= 0) ++next;
     }
     int rc = msdp_solver_set_stream(r->s, streams[slot]);
     if (rc == 0) rc = msdp_solver_upload(r->s);

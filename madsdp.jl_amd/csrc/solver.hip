@@ -96,6 +96,7 @@ struct msdp_solver {
   hipEvent_t rec_ev[REC_RING] = {};
   char* rec_dev = nullptr;
   size_t rec_cap = 0;
+  int64_t alloc_bytes = 0;  // device bytes of the solver's own allocations
   int rec_slot = 0;
   int nb = 0;
   int64_t m = 0, ns = 0;
@@ -184,6 +185,7 @@ struct msdp_solver {
     void* q = nullptr;
     MSDP_HIP(hipMallocAsync(&q, std::max<size_t>(count, 1) * sizeof(T), st));
     allocs.push_back(q);
+    alloc_bytes += (int64_t)(std::max<size_t>(count, 1) * sizeof(T));
     *p = reinterpret_cast<T*>(q);
     return 0;
   }
@@ -192,6 +194,7 @@ struct msdp_solver {
     (void)hipStreamSynchronize(st);
     for (void* p : allocs) (void)hipFreeAsync(p, st);
     allocs.clear();
+    alloc_bytes = 0;
     (void)hipStreamSynchronize(st);
   }
   int release();
@@ -638,7 +641,17 @@ int64_t msdp_solver::footprint() const {
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
   bytes += (mm / 512) * (512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdiw
   bytes += ((int64_t)syevd_work_elems((int)nmax) + trtri_work_elems((int)nmax) + 3 * tot) * d;
-  return bytes + (1 << 20);
+  // the auxiliary streams' scratch of multi-block problems (setup_aux)
+  if (nb >= 2 && nmax <= JACOBI_VB_MAX) {
+    const int64_t tw = (int64_t)std::max(trtri_work_elems((int)nmax), potrf_work_elems((int)nmax));
+    bytes += NAUX * (tw + (int64_t)reduce_scratch_elems(0) + 64 + swork_b / d) * d;
+  }
+  // GEMM partials from the stream-ordered pool: split-K (at most 4M doubles)
+  // and, when a product's 128-tiles outnumber the CUs, stream-K
+  bytes += (int64_t)32 << 20;
+  if (std::max<int64_t>(m, nmax) > 2800) bytes += (int64_t)gemm_sk_partials_bytes();
+  // the recorder's device argument buffer (grown on demand, 1 MiB at least)
+  return bytes + ((int64_t)4 << 20);
 }
 
 // Copies the solution (X, S, y, x, z) to the host, frees the device state and
@@ -2147,6 +2160,14 @@ int msdp_solver_profile_end(msdp_solver* s, double* ms, int64_t* launches, doubl
   GUARD0(s);
   return prof_end(ms, launches, work);
 }
+int msdp_set_tuning(const char* name, double value) {
+  if (!name) { set_error("null name"); return -2; }
+  return set_tuning(name, value);
+}
+int msdp_get_tuning(const char* name, double* value) {
+  if (!name || !value) { set_error("null argument"); return -2; }
+  return get_tuning(name, value);
+}
 int msdp_profile_begin(int32_t cls) {
   if (cls < PROF_GEMM || cls > PROF_TRD1) { set_error("no such kernel class"); return -2; }
   return prof_begin(cls, nullptr);
@@ -2162,6 +2183,7 @@ int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
   o->inf_pr = s->inf_pr; o->inf_du = s->inf_du; o->inf_compl = s->inf_compl; o->obj_val = s->obj_val;
   o->pobj = s->pobj; o->dobj = s->dobj; o->iter = s->iter; o->status = s->status;
   o->n_newton = s->n_newton; o->n_predictor = s->n_predictor; o->dc_t = s->dc_t; o->dc_lambda = s->dc_lambda;
+  o->device_bytes = s->alloc_bytes + (int64_t)s->rec_cap;
   o->lowp_factorizations = s->lowp_count; o->fp64_fallbacks = s->fallback_count;
   o->refine_steps = s->refine_steps; o->pad = 0;
   return 0;

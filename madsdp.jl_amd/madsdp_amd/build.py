@@ -77,6 +77,28 @@ def build(verbose: bool = False, asan: bool = False) -> Path:
     return lib
 
 
+ASAN_DRIVER_SRC = PKG.parent.parent / "tests" / "asan" / "driver.cpp"
+ASAN_DRIVER = ASAN_DRIVER_SRC.with_suffix("")
+
+
+def build_asan_driver() -> Path:
+    """The host-ASan test driver (tests/asan/driver.cpp) against the ASan
+    build of the library: instrumented executable, the ASan runtime loaded
+    first as its dependency."""
+    rt_dir = os.path.dirname(asan_runtime())
+    cmd = [f"{ROCM}/llvm/bin/clang++", "-O1", "-g", "-std=c++17", "-fsanitize=address",
+           "-shared-libasan", "-fno-omit-frame-pointer", f"-I{INCLUDE}", str(ASAN_DRIVER_SRC),
+           str(ASAN_LIB), f"-Wl,-rpath,{PKG}", f"-Wl,-rpath,{rt_dir}", "-o", str(ASAN_DRIVER)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ASan driver build failed:\n{r.stderr[-4000:]}")
+    return ASAN_DRIVER
+
+
 if __name__ == "__main__":
-    print(build(verbose=True, asan="--asan" in sys.argv))
+    if "--asan" in sys.argv:
+        print(build(verbose=True, asan=True))
+        print(build_asan_driver())
+    else:
+        print(build(verbose=True))
     sys.exit(0)

@@ -44,7 +44,7 @@ class Info(C.Structure):
         ("iter", C.c_int32), ("status", C.c_int32), ("n_newton", C.c_int32),
         ("n_predictor", C.c_int32), ("dc_t", C.c_double), ("dc_lambda", C.c_double),
         ("lowp_factorizations", C.c_int32), ("fp64_fallbacks", C.c_int32),
-        ("refine_steps", C.c_int32), ("pad", C.c_int32),
+        ("refine_steps", C.c_int32), ("pad", C.c_int32), ("device_bytes", C.c_int64),
     ]
 
 
@@ -94,6 +94,8 @@ SIGNATURES = {
     "msdp_solver_synchronize": (C.c_int, [P]),
     "msdp_solver_profile_begin": (C.c_int, [P, C.c_int32]),
     "msdp_solver_profile_end": (C.c_int, [P, c_double_p, c_int64_p, c_double_p]),
+    "msdp_set_tuning": (C.c_int, [C.c_char_p, C.c_double]),
+    "msdp_get_tuning": (C.c_int, [C.c_char_p, c_double_p]),
     "msdp_profile_begin": (C.c_int, [C.c_int32]),
     "msdp_profile_end": (C.c_int, [c_double_p, c_int64_p, c_double_p]),
     "msdp_solver_set_stream": (C.c_int, [P, P]),
@@ -151,7 +153,26 @@ def load():
         f.restype = res
         f.argtypes = args
     _lib = lib
+    apply_tuning(lib)
     return lib
+
+
+TUNING_PATH = Path(__file__).resolve().parent / "tuning_gfx950.json"
+
+
+def apply_tuning(lib, path=None):
+    """Applies the autotuner's table (madsdp_amd/autotune.py) if present;
+    MADSDP_HIP_TUNING=0 keeps the compiled-in defaults."""
+    import json
+    path = Path(path or TUNING_PATH)
+    if os.environ.get("MADSDP_HIP_TUNING", "1") == "0" or not path.exists():
+        return {}
+    params = json.loads(path.read_text())["params"]
+    for name, value in params.items():
+        rc = lib.msdp_set_tuning(name.encode(), float(value))
+        if rc < 0:
+            raise RuntimeError(f"{path}: {lib.msdp_last_error().decode()}")
+    return params
 
 
 class HipError(RuntimeError):

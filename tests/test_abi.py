@@ -38,3 +38,24 @@ def test_options_struct_defaults():
     lib.msdp_default_options(ctypes.byref(opt))
     assert opt.tol == 1e-7 and opt.max_iter == 100 and opt.regularize_schur == 1e-12
     assert opt.max_schur_reg == 8 and opt.tau == 0.95 and opt.dc_A == 2.0
+
+
+def test_tuning_parameters():
+    """Kernel-selection parameters: known names read and set within range,
+    unknown names and out-of-range values rejected; the autotuner's table, if
+    present, names only known parameters."""
+    import json
+    from madsdp_amd import lib as L
+    lib = L.load()
+    v = ctypes.c_double()
+    assert lib.msdp_get_tuning(b"streamk_occupancy", ctypes.byref(v)) == 0 and v.value in (1, 2)
+    old = v.value
+    assert lib.msdp_set_tuning(b"streamk_occupancy", 1.0) == 0
+    assert lib.msdp_get_tuning(b"streamk_occupancy", ctypes.byref(v)) == 0 and v.value == 1
+    assert lib.msdp_set_tuning(b"streamk_occupancy", old) == 0
+    assert lib.msdp_set_tuning(b"streamk_occupancy", 3.0) < 0
+    assert lib.msdp_set_tuning(b"no_such_parameter", 1.0) < 0
+    if L.TUNING_PATH.exists():
+        params = json.loads(L.TUNING_PATH.read_text())["params"]
+        for name, value in params.items():
+            assert lib.msdp_get_tuning(name.encode(), ctypes.byref(v)) == 0

@@ -107,3 +107,37 @@ def test_released_solver_rejects_device_calls(gpu):
     assert s.X(0).shape == (20, 20)
     with pytest.raises(RuntimeError):
         s.iterate()
+
+
+def test_footprint_bounds_allocations(gpu):
+    """The admission estimate is an upper bound of what a solver allocates:
+    single dense block with scalars, many small blocks (auxiliary-stream
+    scratch, recorder buffer), max-cut, and the mixed path's FP32 copies."""
+    cases = [(instances.dense_sdp(40, 300, 20, 5), {}),
+             (instances.block_sdp(instances.block_sizes(12, 4, 60, 3), 200, 6, 3), {}),
+             (instances.maxcut(300, 0.05, 2), {}),
+             (instances.dense_sdp(30, 200, 30, 4), dict(precision="mixed", lowp_min_gflop=0.0))]
+    for model, kw in cases:
+        s = M.MadSDPSolver(model, **kw)
+        est = s.footprint()
+        s.solve()
+        used = s.info().device_bytes
+        print(f"footprint {est / 2**20:.1f} MiB, allocated {used / 2**20:.1f} MiB", flush=True)
+        assert 0 < used <= est
+
+
+def test_first_fit_admission(gpu):
+    """A request that does not fit next to the running one is passed over by
+    smaller ones that do (first fit), and still runs once memory frees."""
+    big = [instances.dense_sdp(60, 700, 0, 60 + k) for k in range(2)]
+    small = [instances.maxcut(20, 0.3, 70 + k) for k in range(2)]
+    solvers = [M.MadSDPSolver(m) for m in big + small]
+    fb = min(s.footprint() for s in solvers[:2])
+    fs = max(s.footprint() for s in solvers[2:])
+    budget = max(s.footprint() for s in solvers[:2]) + 2 * fs + fs // 2
+    assert budget < 2 * fb, "the two large requests must not fit together"
+    res, summary = M.serve(solvers, [0.0] * 4, slots=4, mem_budget=budget)
+    assert all(s.status == M.Status.SOLVE_SUCCEEDED for s in solvers)
+    big2 = res[1]
+    assert big2.admitted >= res[0].done - 1e-3          # it waited for memory
+    assert all(r.admitted < big2.admitted for r in res[2:])  # the small ones went ahead
