@@ -192,7 +192,9 @@ def run_stream(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32" if args.batched_precision == "mixed" else "f64",
+        # the arithmetic the served solves actually ran: FP32 assembly only
+        # where a solve was eligible for it (lowp_min_gflop, full-rank H)
+        "dtype": "f64+f32" if batched.get("lowp_solves", 0) > 0 else "f64",
         "data": "synthetic (seeded numpy PCG64 instances, Poisson arrivals)",
         "config": {"workload": f"serving stream: requests {nwarm}..{nwarm + ntimed - 1} of the "
                                f"seed-4 stream (prefix of the 4096-instance stream; {WINDOW} requests "
@@ -355,10 +357,8 @@ def main():
             else:
                 achieved = work.value / (ms.value * 1e-3) / 1e9
                 traffic = None
-                tf = os.path.join(ROOT, "profiles", f"trd_traffic_{args.workload}.json")
-                if os.path.exists(tf):
-                    with open(tf) as fh:
-                        traffic = json.load(fh)["traffic_bytes_per_launch"]
+                # HBM traffic per launch needs a rocprofv3 --pmc pass of this
+                # kernel; none has been made of the current one (null)
                 roof = {"bound": "hbm", "achieved": achieved, "peak": PEAKS["hbm"],
                         "unit": "GB/s", "frac": achieved / PEAKS["hbm"], "traffic": traffic,
                         "kernel": "trd_panel_product_kernel (p = A_trailing v of the blocked "
