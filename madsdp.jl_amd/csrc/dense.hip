@@ -479,8 +479,21 @@ __global__ void splitk_reduce_kernel(int M, int N, int parts, const double* P, d
   if (e >= (int64_t)M * N) return;
   const int i = (int)(e % M), j = (int)(e / M);
   if (lower && i < j) return;
-  double v = 0.0;
-  for (int q = 0; q < parts; ++q) v += P[e + (int64_t)q * M * N];
+  // four interleaved sums keep four loads in flight (deep splits are
+  // latency-bound otherwise), combined in a fixed order
+  const int64_t mn = (int64_t)M * N;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int q = 0;
+  for (; q + 4 <= parts; q += 4) {
+    a0 += P[e + (int64_t)q * mn];
+    a1 += P[e + (int64_t)(q + 1) * mn];
+    a2 += P[e + (int64_t)(q + 2) * mn];
+    a3 += P[e + (int64_t)(q + 3) * mn];
+  }
+  if (q < parts) a0 += P[e + (int64_t)q * mn];
+  if (q + 1 < parts) a1 += P[e + (int64_t)(q + 1) * mn];
+  if (q + 2 < parts) a2 += P[e + (int64_t)(q + 2) * mn];
+  const double v = (a0 + a1) + (a2 + a3);
   double* c = C + i + (int64_t)j * ldc;
   *c = beta == 0.0 ? alpha * v : alpha * v + beta * (*c);
 }
@@ -504,8 +517,10 @@ TuneParam g_tune[] = {
     {"splitk_min_k", {256}, 64, 1 << 20},         // split-K only for K at least this
     {"streamk_occupancy", {2}, 1, 2},              // stream-K workgroups per CU
     {"streamk_min_slabs", {8}, 1, 1024},           // stream-K: slabs per workgroup at least
+    {"streamk_max_waves", {4}, 1, 1 << 20},        // stream-K: tiles below this many waves
 };
-enum { T_WIDE_MIN = 0, T_WIDE_DEEP, T_SPLITK_WGS, T_SPLITK_MIN_K, T_SK_OCC, T_SK_MIN_UNITS };
+enum { T_WIDE_MIN = 0, T_WIDE_DEEP, T_SPLITK_WGS, T_SPLITK_MIN_K, T_SK_OCC, T_SK_MIN_UNITS,
+       T_SK_MAX_WAVES };
 inline int tune(int k) { return g_tune[k].value.load(std::memory_order_relaxed); }
 }  // namespace
 
@@ -556,7 +571,9 @@ static int device_cus() {
 
 // Plain products (no batch, no triangle, no guard) whose 128-tile grid would
 // leave a partial last wave go through the stream-K kernel: at least
-// streamk_min_slabs slabs per workgroup, and more tiles than CUs.
+// streamk_min_slabs slabs per workgroup, more tiles than CUs, and fewer than
+// streamk_max_waves waves of them (with many waves the last one's idle
+// share is small and the fixup's traffic is not).
 
 static bool stream_k_shape(int M, int N, int K, int batch, bool lower, const int* guard, int G,
                            SkGrid& sg) {
@@ -567,7 +584,8 @@ static bool stream_k_shape(int M, int N, int K, int batch, bool lower, const int
   sg.G = G;
   sg.U = (int64_t)sg.ntm * sg.ntn * sg.I;
   const int64_t T = (int64_t)sg.ntm * sg.ntn;
-  return T > G && T % G != 0 && sg.U >= (int64_t)tune(T_SK_MIN_UNITS) * G;
+  return T > G && T % G != 0 && sg.U >= (int64_t)tune(T_SK_MIN_UNITS) * G &&
+         T < (int64_t)tune(T_SK_MAX_WAVES) * G;
 }
 
 // stream-K partials at the largest occupancy (two of the 256-thread,

@@ -144,7 +144,7 @@ struct msdp_solver {
   int lowp_count = 0, fallback_count = 0, refine_steps = 0;
   // QRSchur workspace (qr_schur.jl:5-10): B with the reflectors, scale, tau
   double *qB = nullptr, *qscale = nullptr, *qtau = nullptr, *qbase = nullptr, *qy = nullptr,
-         *qc = nullptr, *qt = nullptr, *qrd = nullptr;
+         *qc = nullptr, *qt = nullptr, *qrd = nullptr, *qT = nullptr, *qwork = nullptr;
   int* qbad = nullptr;
   int64_t qrows = 0;
   // DualCentering vectors
@@ -650,6 +650,14 @@ int64_t msdp_solver::footprint() const {
   // and, when a product's 128-tiles outnumber the CUs, stream-K
   bytes += (int64_t)32 << 20;
   if (std::max<int64_t>(m, nmax) > 2800) bytes += (int64_t)gemm_sk_partials_bytes();
+  if (opt.schur == MSDP_SCHUR_QR) {
+    // the dense scaled constraint matrix, its T factors and workspace
+    int64_t rows = ns;
+    for (const auto& b : blk) rows += (int64_t)b.n * (b.n + 1) / 2;
+    rows = std::max<int64_t>(rows, 1);
+    bytes += (rows * mm + 3 * rows + 5 * mm + 2 + (int64_t)qr_t_elems((int)mm) +
+              (int64_t)qr_work_elems(rows, (int)mm)) * d + 4;
+  }
   // the recorder's device argument buffer (grown on demand, 1 MiB at least)
   return bytes + ((int64_t)4 << 20);
 }
@@ -1803,6 +1811,8 @@ int msdp_solver::qr_factorize(bool& ok) {
     MSDP_TRY(alloc(&qy, rr));
     MSDP_TRY(alloc(&qscale, mm));
     MSDP_TRY(alloc(&qtau, mm));
+    MSDP_TRY(alloc(&qT, qr_t_elems((int)mm)));
+    MSDP_TRY(alloc(&qwork, qr_work_elems((int64_t)rr, (int)mm)));
     MSDP_TRY(alloc(&qc, mm));
     MSDP_TRY(alloc(&qt, mm));
     MSDP_TRY(alloc(&qrd, 2));
@@ -1832,14 +1842,16 @@ int msdp_solver::qr_factorize(bool& ok) {
     off += (int64_t)n * (n + 1) / 2;
   }
   MSDP_TRY(qr_scalar_rows(st, (int)m, (int)ns, Clin, x, z, qB, rows, off));
-  MSDP_TRY(qr_factor(st, rows, (int)m, qB, qscale, qtau, qbad, qrd));
+  // more multipliers than rows (qr_schur.jl:83-85); the zero-column check
+  // (:74-77) reports the same failure, so its order does not matter
+  if (rows < m) return 0;
+  MSDP_TRY(qr_factor(st, rows, (int)m, qB, qscale, qtau, qT, qwork, qbad, qrd));
   int bad = 0;
   double rd[2] = {0, 0};
   MSDP_HIP(hipMemcpyAsync(&bad, qbad, sizeof(int), hipMemcpyDeviceToHost, st));
   MSDP_HIP(hipMemcpyAsync(rd, qrd, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   MSDP_TRY(stream_sync(st));
   if (bad) return 0;              // zero or non-finite column (qr_schur.jl:74-77)
-  if (rows < m) return 0;         // more multipliers than rows (:83-85)
   // A dense Householder QR leaves roundoff where SPQR's exact zero pivot would
   // be; a pivot below this tolerance counts as zero (:87).
   if (!(rd[1] > 64 * EPS * std::max(1.0, rd[0]))) return 0;
@@ -1864,7 +1876,7 @@ int msdp_solver::qr_solve_directions(double mu_, bool corrected) {
   if (m > 0) {
     // qbase = Q' base; c = R'^-1 (scale .* Rp); t = c - qbase[1:m]
     MSDP_HIP(hipMemcpyAsync(qy, qbase, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
-    MSDP_TRY(qr_apply_vec(st, rows, (int)m, qB, qtau, qy, true));
+    MSDP_TRY(qr_apply_vec(st, rows, (int)m, qB, qT, qwork, qy, true));
     MSDP_TRY(qr_mul(st, (int)m, qscale, Rp, qc));
     MSDP_TRY(qr_trsv(st, (int)m, qB, rows, qc, true));
     MSDP_TRY(qr_combine(st, (int)m, qc, qy, qt));
@@ -1875,7 +1887,7 @@ int msdp_solver::qr_solve_directions(double mu_, bool corrected) {
     // direction = base + Q [t; 0]
     MSDP_TRY(fill(st, rows, 0.0, qy));
     MSDP_HIP(hipMemcpyAsync(qy, qt, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-    MSDP_TRY(qr_apply_vec(st, rows, (int)m, qB, qtau, qy, false));
+    MSDP_TRY(qr_apply_vec(st, rows, (int)m, qB, qT, qwork, qy, false));
     MSDP_TRY(axpby(st, rows, 1.0, qbase, 1.0, qy, qy));
   } else {
     MSDP_HIP(hipMemcpyAsync(qy, qbase, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));

@@ -5,7 +5,8 @@
 Times the FP64 products the workloads issue (the max-cut n = 3000 NT
 products and panel updates, the per-block products of 200..600-wide blocks,
 the skinny split-K products of the eigen back-transform, a deep product of
-the dense Schur assembly) through msdp_k_gemm on the current device, and
+the dense Schur assembly, the QR panel's rank-64 trailing update and its
+V'C product) through msdp_k_gemm on the current device, and
 picks each parameter of msdp_set_tuning by coordinate descent over the summed
 time: the 128-tile thresholds, the split-K width and depth, the stream-K
 occupancy and minimum work per workgroup.  The table it writes is applied by
@@ -27,6 +28,7 @@ CANDIDATES = {
     "splitk_min_k": [128, 256, 512],
     "streamk_occupancy": [1, 2],
     "streamk_min_slabs": [4, 8, 16],
+    "streamk_max_waves": [2, 4, 8],
 }
 
 # (M, N, K, transA, transB, weight): weight ~ how often a solve issues it
@@ -36,7 +38,8 @@ SHAPES = [
     (600, 600, 600, 0, 0, 6), (500, 500, 500, 0, 1, 6), (400, 400, 400, 0, 0, 6),
     (300, 300, 300, 0, 1, 6), (200, 200, 200, 0, 0, 6),
     (640, 64, 3000, 1, 0, 4), (250, 64, 2048, 1, 0, 4), (64, 64, 3000, 1, 0, 4),
-    (1024, 1024, 4096, 1, 0, 2),
+    (1024, 1024, 4096, 1, 0, 2), (8000, 8000, 8000, 0, 0, 1),
+    (31775, 7936, 64, 0, 0, 1), (64, 7936, 31775, 1, 0, 1),
 ]
 
 

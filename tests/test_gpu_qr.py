@@ -116,3 +116,24 @@ def test_qr_maxcut_parity(gpu):
     tr = s.trace()
     for k, r in enumerate(ref.trace):
         assert abs(tr[k, 0] - r.pobj) <= 1e-9 * max(1, abs(r.pobj))
+
+
+@pytest.mark.parametrize("n,m,ns,seed", [(50, 1100, 10, 7), (90, 1150, 0, 8)],
+                         ids=["dense_50_1100_lp10", "dense_90_1150"])
+def test_qr_blocked_parity(gpu, n, m, ns, seed):
+    """The blocked factorisation (18 panels, the last one partial; B of
+    1285 x 1100 and 4095 x 1150) and the per-panel Q'y, Qy and R solves:
+    per-iteration parity with the oracle's LAPACK QR to 1e-9."""
+    from madsdp_amd import instances
+    model = instances.dense_sdp(n, m, ns, seed)
+    print(f"QR blocked n={n} m={m} ns={ns}", flush=True)
+    s = M.MadSDPSolver(model, schur=M.QRSchur()).solve()
+    ref = O.Solver(model, schur=O.QRSchur()).solve()
+    assert s.status == M.Status.SOLVE_SUCCEEDED and s.iter == ref.iter, (s.status, s.iter, ref.iter)
+    tr = s.trace()
+    for k, r in enumerate(ref.trace):
+        sc = max(1.0, abs(r.pobj), abs(r.dobj))
+        assert abs(tr[k, 0] - r.pobj) <= 1e-9 * sc and abs(tr[k, 1] - r.dobj) <= 1e-9 * sc, k
+        assert abs(tr[k, 2] - r.inf_pr) <= 1e-9 * max(1, r.inf_pr), k
+        assert abs(tr[k, 3] - r.inf_du) <= 1e-9 * max(1, r.inf_du), k
+    np.testing.assert_allclose(s.y, ref.y, rtol=1e-7, atol=1e-9)

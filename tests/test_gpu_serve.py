@@ -112,11 +112,13 @@ def test_released_solver_rejects_device_calls(gpu):
 def test_footprint_bounds_allocations(gpu):
     """The admission estimate is an upper bound of what a solver allocates:
     single dense block with scalars, many small blocks (auxiliary-stream
-    scratch, recorder buffer), max-cut, and the mixed path's FP32 copies."""
+    scratch, recorder buffer), max-cut, the mixed path's FP32 copies and the
+    QR path's dense constraint matrix, T factors and workspace."""
     cases = [(instances.dense_sdp(40, 300, 20, 5), {}),
              (instances.block_sdp(instances.block_sizes(12, 4, 60, 3), 200, 6, 3), {}),
              (instances.maxcut(300, 0.05, 2), {}),
-             (instances.dense_sdp(30, 200, 30, 4), dict(precision="mixed", lowp_min_gflop=0.0))]
+             (instances.dense_sdp(30, 200, 30, 4), dict(precision="mixed", lowp_min_gflop=0.0)),
+             (instances.dense_sdp(40, 300, 20, 5), dict(schur=M.QRSchur()))]
     for model, kw in cases:
         s = M.MadSDPSolver(model, **kw)
         est = s.footprint()
