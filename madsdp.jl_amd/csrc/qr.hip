@@ -106,6 +106,19 @@ __global__ __launch_bounds__(256) void qr_panel_step_kernel(
   const int g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6, G = gridDim.x;
   const int64_t lo = (int64_t)g * QR_CH, hi = std::min<int64_t>(lo + QR_CH, rows);
   const int64_t i = lo + t;
+  // rows that take reflector k, and rows below the next pivot (the next
+  // step's sums); with k >= 0 the second set lies inside the first
+  const bool upd = k >= 0 && i < hi && i > k;
+  const bool below = i < hi && i > nk;
+  const int ncol = j1 - nk;
+  // this thread's entries of column k and of the first eight columns from nk
+  // are loaded before the partial sums are reduced (only this thread writes
+  // them in this launch)
+  const double vraw = upd ? B[i + (int64_t)k * ldb] : 0.0;
+  double c[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    c[u] = (u < ncol && (upd || below)) ? B[i + (int64_t)(nk + u) * ldb] : 0.0;
   double sc = 0.0;
   if (k >= 0) {
     const int nv = j1 - k;  // ss and the dots with columns k+1 .. j1-1
@@ -145,25 +158,18 @@ __global__ __launch_bounds__(256) void qr_panel_step_kernel(
   } else if (i == nk && i < hi) {
     for (int q = 0; q < j1 - nk; ++q) prow_out[q] = B[nk + (int64_t)(nk + q) * ldb];
   }
-  // rows that take reflector k, and rows below the next pivot (the next
-  // step's sums); with k >= 0 the second set lies inside the first
-  const bool upd = k >= 0 && i < hi && i > k;
-  const bool below = i < hi && i > nk;
-  double v = 0.0;
-  if (upd) {
-    v = sc * B[i + (int64_t)k * ldb];
-    B[i + (int64_t)k * ldb] = v;
-  }
+  const double v = sc * vraw;
+  if (upd) B[i + (int64_t)k * ldb] = v;
   // columns nk .. j1-1: updated (k >= 0), the new pivot row copied, and each
   // row's terms of ss' = sum x^2 and d'_j = sum x B[., j], x = B[., nk],
   // summed over the wavefront at once; eight columns' loads in flight
-  const int ncol = j1 - nk;
   double x = 0.0;
   for (int q0 = 0; q0 < ncol; q0 += 8) {
-    double c[8];
+    if (q0 > 0) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      c[u] = (q0 + u < ncol && (upd || below)) ? B[i + (int64_t)(nk + q0 + u) * ldb] : 0.0;
+      for (int u = 0; u < 8; ++u)
+        c[u] = (q0 + u < ncol && (upd || below)) ? B[i + (int64_t)(nk + q0 + u) * ldb] : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int q = q0 + u;
