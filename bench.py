@@ -268,6 +268,9 @@ def main():
                          "(the dominant kernel of the max-cut solve), gemm = the FP64 GEMM class, "
                          "gemm32 = the FP32 GEMM class")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--schur", default="cholesky", choices=["cholesky", "qr", "ldlt"],
+                    help="single-instance workloads: Schur solver (DenseCholesky, QRSchur "
+                         "or MadNLPSchur)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rate", type=float, default=50.0, help="stream: Poisson arrival rate (1/s)")
@@ -305,7 +308,7 @@ def main():
 
     import ctypes as C
     import numpy as np
-    from madsdp_amd import MadSDPSolver, Status
+    from madsdp_amd import DenseCholesky, MadNLPSchur, MadSDPSolver, QRSchur, Status
 
     if args.copy_calibration > 0:
         import torch
@@ -316,7 +319,9 @@ def main():
         torch.cuda.synchronize()
         del a, b
     model, config = build_model(args.workload)
-    s = MadSDPSolver(model, device=0, precision=args.precision)
+    schur = {"cholesky": DenseCholesky, "qr": QRSchur, "ldlt": MadNLPSchur}[args.schur]()
+    s = MadSDPSolver(model, device=0, precision=args.precision, schur=schur)
+    config = dict(config, schur=type(schur).__name__)
     for _ in range(args.warmup):
         s.solve()
         print(f"warmup solve: status {s.status.name}, {s.iter} iterations", file=sys.stderr, flush=True)
@@ -347,8 +352,8 @@ def main():
                 peak = PEAKS["mfma"] if cls == 0 else PEAKS["mfma32"]
                 roof = {"bound": "mfma", "achieved": achieved, "peak": peak,
                         "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                        "kernel": "FP64 GEMM class (gemm128_f64_kernel, gemm_f64_kernel, "
-                                  "gemm_splitk_kernel, vendor DGEMM for large plain products)"
+                        "kernel": "FP64 GEMM class (gemm128_f64_kernel, gemm_sk_f64_kernel, "
+                                  "gemm_f64_kernel, gemm_splitk_kernel)"
                         if cls == 0
                         else "gemm128_f32_kernel (FP32 MFMA 16x16x4)",
                         "launches": launches.value, "avg_launch_us": 1e3 * ms.value / launches.value,

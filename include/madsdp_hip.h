@@ -95,6 +95,9 @@ typedef struct msdp_info {
    * argument buffer; GEMM partials taken from the stream-ordered pool for the
    * duration of one product are not counted) -- compare msdp_solver_footprint */
   int64_t device_bytes;
+  /* the diagonal shift the last Schur factorisation took: 0, or the rung of
+   * the regularisation ladder (src/solver.jl:299-314) it succeeded at */
+  double schur_reg;
 } msdp_info;
 
 /* which iterate array msdp_solver_get/set moves */
@@ -136,6 +139,9 @@ int msdp_solver_set_initial_point(msdp_solver* s, const double* X, const double*
 int msdp_solver_initialize(msdp_solver* s);          /* initialize!        solver.jl:286 */
 int msdp_solver_iterate(msdp_solver* s);             /* iterate!           solver.jl:316 */
 int msdp_solver_update_termination(msdp_solver* s);  /* update_termination! solver.jl:349 */
+/* solver.mu = mu (the field MadSDPSolver holds, src/solver.jl:37-48; the
+ * reference's test/termination.jl:22 assigns it before update_termination!) */
+int msdp_solver_set_mu(msdp_solver* s, double mu);
 int msdp_solver_solve(msdp_solver* s);               /* solve!             solver.jl:385 */
 int msdp_solver_get_info(const msdp_solver* s, msdp_info* out);
 /* number of iteration records and the per-iteration trace (pobj, dobj,
@@ -171,7 +177,7 @@ int msdp_solver_profile_end(msdp_solver* s, double* total_ms, int64_t* launches,
 /* Kernel-selection parameters (no counterpart in the reference): the GEMM
  * tile / split-K / stream-K choices, by name -- gemm_wide_min_tiles,
  * gemm_wide_deep_tiles, splitk_workgroups, splitk_min_k, streamk_occupancy,
- * streamk_min_slabs.  The autotuner's table (madsdp_amd/tuning_gfx950.json)
+ * streamk_min_slabs, streamk_max_waves.  The autotuner's table (madsdp_amd/tuning_gfx950.json)
  * is applied through msdp_set_tuning once, before any solver runs; -2 for an
  * unknown name or a value out of range. */
 int msdp_set_tuning(const char* name, double value);

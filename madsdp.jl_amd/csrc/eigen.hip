@@ -15,6 +15,8 @@
 //  lambda_min : tridiagonalisation + Sturm-sequence multisection.
 #include "common.h"
 
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -774,7 +776,14 @@ static int sytrd_columns_graph(hipStream_t s, int n, const TrdWork& t, bool& don
   done = false;
   // n <= TRD_GRAPH_MAX_N: the panel updates' products then take the plain
   // tile kernels (no pool allocation inside the capture)
-  if (s == nullptr || n > TRD_GRAPH_MAX_N || prof_active(PROF_TRD, s) || prof_active(PROF_GEMM, s))
+  // MADSDP_HIP_GRAPHS=0 turns the replay off (eager launches, e.g. for
+  // kernel tracers that do not follow captured launches)
+  static const bool graphs_off = [] {
+    const char* v = std::getenv("MADSDP_HIP_GRAPHS");
+    return v && std::strcmp(v, "0") == 0;
+  }();
+  if (graphs_off || s == nullptr || n > TRD_GRAPH_MAX_N || prof_active(PROF_TRD, s) ||
+      prof_active(PROF_GEMM, s))
     return 0;
   TrdGraphKey key{0, n, t.A};
   MSDP_HIP(hipGetDevice(&key.device));

@@ -2134,6 +2134,11 @@ int msdp_solver_initialize(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->
 int msdp_solver_iterate(msdp_solver* s) { GUARD_READY(s); MSDP_HIP(hipSetDevice(s->device)); return s->iterate(); }
 int msdp_solver_update_termination(msdp_solver* s) { GUARD_READY(s); MSDP_HIP(hipSetDevice(s->device)); return s->update_termination(); }
 int msdp_solver_solve(msdp_solver* s) { GUARD(s); MSDP_HIP(hipSetDevice(s->device)); return s->solve(); }
+int msdp_solver_set_mu(msdp_solver* s, double mu) {
+  GUARD_READY(s);
+  s->mu = mu;
+  return 0;
+}
 int msdp_solver_synchronize(msdp_solver* s) {
   GUARD0(s);
   if (s->st && !s->released) MSDP_TRY(stream_sync(s->st));
@@ -2198,6 +2203,7 @@ int msdp_solver_get_info(const msdp_solver* s, msdp_info* o) {
   o->device_bytes = s->alloc_bytes + (int64_t)s->rec_cap;
   o->lowp_factorizations = s->lowp_count; o->fp64_fallbacks = s->fallback_count;
   o->refine_steps = s->refine_steps; o->pad = 0;
+  o->schur_reg = s->reg_used;
   return 0;
 }
 

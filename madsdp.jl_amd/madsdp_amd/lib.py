@@ -45,6 +45,7 @@ class Info(C.Structure):
         ("n_predictor", C.c_int32), ("dc_t", C.c_double), ("dc_lambda", C.c_double),
         ("lowp_factorizations", C.c_int32), ("fp64_fallbacks", C.c_int32),
         ("refine_steps", C.c_int32), ("pad", C.c_int32), ("device_bytes", C.c_int64),
+        ("schur_reg", C.c_double),
     ]
 
 
@@ -77,6 +78,7 @@ SIGNATURES = {
     "msdp_solver_initialize": (C.c_int, [P]),
     "msdp_solver_iterate": (C.c_int, [P]),
     "msdp_solver_update_termination": (C.c_int, [P]),
+    "msdp_solver_set_mu": (C.c_int, [P, C.c_double]),
     "msdp_solver_solve": (C.c_int, [P]),
     "msdp_solver_get_info": (C.c_int, [P, C.POINTER(Info)]),
     "msdp_solver_trace": (C.c_int, [P, C.c_int32, c_double_p, c_int32_p]),

@@ -264,6 +264,10 @@ class MadSDPSolver:
     def update_residuals(self):
         L.check(self.lib.msdp_solver_update_residuals(self._h), "update_residuals!")
 
+    def set_mu(self, mu: float):
+        """`solver.mu = mu` (the field test/termination.jl:22 assigns)."""
+        L.check(self.lib.msdp_solver_set_mu(self._h, float(mu)), "mu")
+
     def solve(self, verbose: bool = False, host_loop: bool = False):
         """`solve!(solver)` (src/solver.jl:385-396).  The loop runs inside
         the library unless `host_loop`, which runs the same loop here."""
@@ -329,7 +333,7 @@ class MadSDPSolver:
 
     def __getattr__(self, name):
         if name in ("mu", "sigma", "alpha", "beta", "inf_pr", "inf_du", "inf_compl", "obj_val",
-                    "iter", "n_newton", "n_predictor"):
+                    "iter", "n_newton", "n_predictor", "schur_reg"):
             return getattr(self.info(), name)
         raise AttributeError(name)
 
