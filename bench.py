@@ -276,6 +276,8 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="stream: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=16, help="stream: concurrent solves")
     ap.add_argument("--mem-budget", type=int, default=128, help="stream: device GiB for admission")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="stream: GPU_MAX_HW_QUEUES for the run (0: one per slot, 4..32)")
     ap.add_argument("--batched-precision", default="mixed", choices=["mixed", "fp64"],
                     help="stream: precision of the batched path")
     ap.add_argument("--compare-precision", type=int, default=1,
@@ -293,7 +295,7 @@ def main():
         # one hardware queue per concurrent solve (HIP's default is 4); set
         # before the HIP runtime starts
         prev = os.environ.get("GPU_MAX_HW_QUEUES")
-        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, args.slots)))
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, args.hw_queues or args.slots)))
         print(f"GPU_MAX_HW_QUEUES {prev} -> {os.environ['GPU_MAX_HW_QUEUES']}; "
               f"{len(os.sched_getaffinity(0))} host cores", file=sys.stderr)
         rec = run_stream(args)

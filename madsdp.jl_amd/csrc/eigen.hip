@@ -1079,7 +1079,7 @@ __global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) 
 // Eigen-decomposition with vectors, n <= 128, one workgroup per matrix:
 // Householder tridiagonalisation in LDS (reflectors kept in the item's Vtmp,
 // tau_k on its diagonal), implicit QL with Wilkinson-type shifts on the
-// tridiagonal (one thread forms each sweep's rotations, every thread applies
+// tridiagonal (wavefront 0 forms each sweep's rotations, every thread applies
 // them to its row of Z held in the same LDS), the back-transformation
 // V = H_0 ... H_{n-3} Z, and the columns sorted by eigenvalue.  Replaces the
 // cyclic Jacobi sweeps for the NT scaling of small blocks.
@@ -1165,18 +1165,28 @@ __device__ void syev_small(const EigBatch& it) {
     const int i = x % n, j = x / n;
     As[i][j] = i == j ? 1.0 : 0.0;
   }
-  // implicit QL (e[i] couples d[i] and d[i+1]); thread 0 keeps l, iter
+  // implicit QL (e[i] couples d[i] and d[i+1]).  Wavefront 0 runs the
+  // control in every lane (same values): the split search is one ballot per
+  // 64 entries, and the chase carries d[i], e[i] in registers, loading the
+  // next pair one rotation ahead (d[i] is not written by rotation i).
   int l = 0, iter = 0;
   for (;;) {
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
+      const int lane = tid;
       int state = 0, cnt = 0, top = 0;
       for (;;) {
         if (l >= n) { state = 2; break; }
-        int m = l;
-        for (; m < n - 1; ++m) {
-          const double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= DEPS * dd) break;
+        // first m >= l with |e[m]| <= eps (|d[m]| + |d[m+1]|), else n - 1
+        int m = n - 1;
+        for (int b = l; b < n - 1; b += 64) {
+          const int mm = b + lane;
+          const bool hit = mm < n - 1 && fabs(e[mm]) <= DEPS * (fabs(d[mm]) + fabs(d[mm + 1]));
+          const unsigned long long mask = __ballot(hit);
+          if (mask) {
+            m = b + __ffsll((long long)mask) - 1;
+            break;
+          }
         }
         if (m == l) {  // d[l] converged
           ++l;
@@ -1188,28 +1198,33 @@ __device__ void syev_small(const EigBatch& it) {
         double r = hypot(g, 1.0);
         g = d[m] - d[l] + e[l] / (g + copysign(r, g));
         double s = 1.0, c = 1.0, pp = 0.0;
-        int i = m - 1;
         bool split = false;
-        for (; i >= l; --i) {
-          const double f = s * e[i], b = c * e[i];
+        double dip1 = d[m], ei = e[m - 1], di = d[m - 1];
+        for (int i = m - 1; i >= l; --i) {
+          double en = 0.0, dn = 0.0;
+          if (i > l) { en = e[i - 1]; dn = d[i - 1]; }
+          const double f = s * ei, b = c * ei;
           r = hypot(f, g);
           e[i + 1] = r;
           if (r == 0.0) {
-            d[i + 1] -= pp;
+            d[i + 1] = dip1 - pp;
             e[m] = 0.0;
             split = true;
             break;
           }
           s = f / r;
           c = g / r;
-          g = d[i + 1] - pp;
-          r = (d[i] - g) * s + 2.0 * c * b;
+          g = dip1 - pp;
+          r = (di - g) * s + 2.0 * c * b;
           pp = s * r;
           d[i + 1] = g + pp;
           g = c * r - b;
           cs[cnt] = c;
           sn[cnt] = s;
           ++cnt;
+          dip1 = di;
+          ei = en;
+          di = dn;
         }
         top = m - 1;
         if (!split) {
@@ -1219,9 +1234,11 @@ __device__ void syev_small(const EigBatch& it) {
         }
         break;  // apply this sweep's rotations
       }
-      s_state = state;
-      s_cnt = cnt;
-      s_top = top;
+      if (lane == 0) {
+        s_state = state;
+        s_cnt = cnt;
+        s_top = top;
+      }
     }
     __syncthreads();
     const int cnt = s_cnt, top = s_top;
