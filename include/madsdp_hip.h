@@ -127,6 +127,15 @@ int msdp_solver_set_constraints_coo(msdp_solver* s, int32_t block, int64_t nnz,
  * mats is m_i consecutive n_i x n_i matrices (BufferedModelForSchur dense path) */
 int msdp_solver_set_constraints_dense(msdp_solver* s, int32_t block, int64_t m_i,
                                       const int64_t* con_idx, const double* mats);
+/* Low-rank constraint matrices of one block, A_j = sum_r w_r f_r f_r' (the
+ * rank-one / factored matrices LowRankOpt keeps, reference README.md:148):
+ * con_idx (m_i, strictly increasing), rank_ptr (m_i + 1, from 0) the column
+ * range of each constraint in factors (n x rank_ptr[m_i], column-major), and
+ * weights (rank_ptr[m_i]).  A(X), A*(y) and the Schur contribution are then
+ * computed from F' X F, F diag(w y) F' and F' L F, F' R F. */
+int msdp_solver_set_constraints_lowrank(msdp_solver* s, int32_t block, int64_t m_i,
+                                        const int64_t* con_idx, const int64_t* rank_ptr,
+                                        const double* factors, const double* weights);
 /* LRO.cons_constant (b), d_lin, C_lin (m x n_s column-major) */
 int msdp_solver_set_rhs(msdp_solver* s, const double* b);
 int msdp_solver_set_scalar_data(msdp_solver* s, const double* d_lin, const double* C_lin);

@@ -7,7 +7,7 @@ namespace msdp {
 
 // Device view of the constraint matrices touching one PSD block.
 struct BlockOps {
-  enum Kind { EMPTY = 0, COO = 1, DENSE = 2 };
+  enum Kind { EMPTY = 0, COO = 1, DENSE = 2, LOWRANK = 3 };
   int kind = EMPTY;
   int n = 0;
   bool one_hot = false;       // every constraint is v e_r e_r'
@@ -32,6 +32,16 @@ struct BlockOps {
   double* Hbuf = nullptr;      // ncon x ncon block of H before the scatter
   int jt_split = 1;            // A*(y): constraint ranges summed separately
   double* jt_part = nullptr;   // [jt_split x n^2] partial sums
+  // LOWRANK: A_j = sum_{r in con_ptr[j]..con_ptr[j+1]} lam_r f_r f_r' with the
+  // factor columns f_r of F (n x R, column-major), fcon[r] = the constraint
+  // index of column r; FW (n x R) and PL, PR (R x R) are work buffers
+  int R = 0;
+  double* F = nullptr;
+  double* lam = nullptr;
+  int* fcon = nullptr;
+  double* FW = nullptr;
+  double* PL = nullptr;
+  double* PR = nullptr;
 };
 
 // number of constraint ranges of the dense A*(y) for a block of side n

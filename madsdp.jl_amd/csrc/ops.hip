@@ -4,7 +4,10 @@
 //   A*(y)_i   = sum_j y_j A_ij                  LRO.unsafe_jtprod  (src/step.jl:30,97)
 //   H_jk     += <A_ij, L A_ik R>                LRO.schur_complement! (src/solver.jl:304)
 // Constraint storage per block: COO entries sorted by constraint (with a
-// by-position index for A*), or dense matrices of the touching constraints.
+// by-position index for A*), dense matrices of the touching constraints, or
+// low-rank factors A_j = sum_r lam_r f_r f_r' (LowRankOpt's rank-one
+// structure, reference README.md:148): A(X)_j = sum_r lam_r f_r' X f_r,
+// A*(y) = F diag(lam y) F', H_jk = sum_{r,s} lam_r lam_s (f_r' L f_s)(f_s' R f_r).
 #include "ops.h"
 
 #include <algorithm>
@@ -36,9 +39,31 @@ __global__ __launch_bounds__(256) void jprod_dense_kernel(int64_t n2, const int*
   if (threadIdx.x == 0) out[dcon[j]] += s;
 }
 
+// out[con_list[j]] += sum_{r in j} lam_r f_r' Y[:, r]   (Y = X F)
+__global__ __launch_bounds__(256) void jprod_lowrank_kernel(int n, const int* con_list,
+                                                            const int* con_ptr, const double* F,
+                                                            const double* Y, const double* lam,
+                                                            double* out) {
+  __shared__ double red[4];
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int r = con_ptr[j]; r < con_ptr[j + 1]; ++r) {
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) t += F[i + (int64_t)r * n] * Y[i + (int64_t)r * n];
+    s += lam[r] * block_reduce<0>(t, red);
+  }
+  if (threadIdx.x == 0) out[con_list[j]] += s;
+}
+
 int jprod(hipStream_t st, const BlockOps& b, const double* X, double* out) {
   MSDP_TRY(rec_forbid("jprod"));
   if (b.kind == BlockOps::EMPTY) return 0;
+  if (b.kind == BlockOps::LOWRANK) {
+    MSDP_TRY(gemm(st, false, false, b.n, b.R, b.n, 1.0, X, b.n, b.F, b.n, 0.0, b.FW, b.n));
+    jprod_lowrank_kernel<<<b.ncon, 256, 0, st>>>(b.n, b.con_list, b.con_ptr, b.F, b.FW, b.lam, out);
+    MSDP_HIP(hipGetLastError());
+    return 0;
+  }
   if (b.kind == BlockOps::COO) {
     jprod_coo_kernel<<<cdiv(b.ncon, 256), 256, 0, st>>>(b.ncon, b.con_list, b.con_ptr, b.row,
                                                          b.col, b.val, X, b.n, out);
@@ -96,10 +121,26 @@ int jtprod_split(int n, int ncon) {
   return std::max(1, std::min(split, 64));
 }
 
+// FW[:, r] = lam_r y[fcon[r]] F[:, r]
+__global__ void lowrank_scale_kernel(int n, int R, const double* F, const double* lam,
+                                     const int* fcon, const double* y, double* FW) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * R) return;
+  const int r = (int)(e / n);
+  FW[e] = F[e] * (lam[r] * y[fcon[r]]);
+}
+
 int jtprod(hipStream_t st, const BlockOps& b, const double* y, double* out) {
   MSDP_TRY(rec_forbid("jtprod"));
   const int64_t n2 = (int64_t)b.n * b.n;
   if (b.kind == BlockOps::EMPTY) return fill(st, n2, 0.0, out);
+  if (b.kind == BlockOps::LOWRANK) {
+    lowrank_scale_kernel<<<cdiv((int64_t)b.n * b.R, 256), 256, 0, st>>>(b.n, b.R, b.F, b.lam,
+                                                                        b.fcon, y, b.FW);
+    MSDP_HIP(hipGetLastError());
+    MSDP_TRY(gemm(st, false, true, b.n, b.n, b.R, 1.0, b.FW, b.n, b.F, b.n, 0.0, out, b.n));
+    return symmetrize(st, b.n, out, b.n);
+  }
   if (b.kind == BlockOps::COO) {
     MSDP_TRY(fill(st, n2, 0.0, out));
     jtprod_coo_kernel<<<cdiv(b.npos, 256), 256, 0, st>>>(b.npos, b.pos_lin, b.pos_ptr, b.pos_ent,
@@ -161,6 +202,20 @@ __global__ void svec_pack_kernel(int n, int cb, const double* P, double* B, int6
   B[k + (int64_t)j * ldb] = r == c ? v : 1.4142135623730951 * v;
 }
 
+// Low-rank: H[j,k] += sum_{r in j, s in k} lam_r lam_s PL[r,s] PR[s,r] with
+// PL = F' L F and PR = F' R F (R x R, ld R)
+__global__ void schur_lowrank_kernel(int ncon, const int* con_list, const int* con_ptr,
+                                     const double* lam, const double* PL, const double* PR, int R,
+                                     double* H, int64_t ldh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (a >= ncon || a < b) return;
+  double s = 0.0;
+  for (int r = con_ptr[a]; r < con_ptr[a + 1]; ++r)
+    for (int q = con_ptr[b]; q < con_ptr[b + 1]; ++q)
+      s += lam[r] * lam[q] * PL[r + (int64_t)q * R] * PR[q + (int64_t)r * R];
+  H[con_list[a] + (int64_t)con_list[b] * ldh] += s;
+}
+
 __global__ void scatter_lower_kernel(int mi, const int* dcon, const double* Hb, double* H,
                                      int64_t ldh) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
@@ -182,6 +237,22 @@ int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, doubl
   MSDP_TRY(rec_forbid("schur_block"));
   const int n = b.n;
   if (b.kind == BlockOps::EMPTY || b.ncon == 0) return 0;
+  if (b.kind == BlockOps::LOWRANK) {
+    // PL = F' (L F); PR = F' (R F), the same buffer when L == R
+    MSDP_TRY(gemm(st, false, false, n, b.R, n, 1.0, sc.L, n, b.F, n, 0.0, b.FW, n));
+    MSDP_TRY(gemm(st, true, false, b.R, b.R, n, 1.0, b.F, n, b.FW, n, 0.0, b.PL, b.R));
+    const double* PR = b.PL;
+    if (sc.R != sc.L) {
+      MSDP_TRY(gemm(st, false, false, n, b.R, n, 1.0, sc.R, n, b.F, n, 0.0, b.FW, n));
+      MSDP_TRY(gemm(st, true, false, b.R, b.R, n, 1.0, b.F, n, b.FW, n, 0.0, b.PR, b.R));
+      PR = b.PR;
+    }
+    dim3 g(cdiv(b.ncon, 128), b.ncon);
+    schur_lowrank_kernel<<<g, 128, 0, st>>>(b.ncon, b.con_list, b.con_ptr, b.lam, b.PL, PR, b.R,
+                                           H, m);
+    MSDP_HIP(hipGetLastError());
+    return 0;
+  }
   if (b.kind == BlockOps::COO) {
     dim3 g(cdiv(b.ncon, 128), b.ncon);
     if (b.one_hot)

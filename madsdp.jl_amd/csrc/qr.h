@@ -7,6 +7,8 @@ int qr_pack_dense(hipStream_t s, int n, int cnt, const double* P, const int* con
                   int64_t ldb, int64_t off);
 int qr_pack_coo(hipStream_t s, const BlockOps& b, const double* G, double* B, int64_t ldb,
                 int64_t off);
+int qr_pack_lowrank(hipStream_t s, const BlockOps& b, const double* G, double* B, int64_t ldb,
+                    int64_t off);
 int qr_scalar_rows(hipStream_t s, int m, int ns, const double* Clin, const double* x,
                    const double* z, double* B, int64_t ldb, int64_t off);
 // panel width of the blocked factorisation and rows per workgroup of the

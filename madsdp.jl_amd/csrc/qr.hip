@@ -56,6 +56,23 @@ __global__ void qr_coo_kernel(int n, int ncon, const int* con_list, const int* c
   B[off + k + (int64_t)con_list[j] * ldb] = a == b ? s : 1.4142135623730951 * s;
 }
 
+// Low-rank constraints: G' A_j G = sum_r lam_r g_r g_r' with g_r = G' f_r
+// (GF = G' F, n x R):  B[off + k, con_j] = w_k sum_{r in j} lam_r GF[a, r] GF[b, r]
+__global__ void qr_lowrank_kernel(int n, int ncon, const int* con_list, const int* con_ptr,
+                                  const double* lam, const double* GF, double* B, int64_t ldb,
+                                  int64_t off) {
+  const int64_t d = (int64_t)n * (n + 1) / 2;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (k >= d || j >= ncon) return;
+  int a, b;
+  svec_rc(k, a, b);
+  double s = 0.0;
+  for (int r = con_ptr[j]; r < con_ptr[j + 1]; ++r)
+    s += lam[r] * GF[a + (int64_t)r * n] * GF[b + (int64_t)r * n];
+  B[off + k + (int64_t)con_list[j] * ldb] = a == b ? s : 1.4142135623730951 * s;
+}
+
 // scalar rows: B[off + l, j] = sqrt(x_l / z_l) C_lin[j, l]
 __global__ void qr_scalar_rows_kernel(int m, int ns, const double* Clin, const double* x,
                                       const double* z, double* B, int64_t ldb, int64_t off) {
@@ -456,6 +473,17 @@ int qr_pack_coo(hipStream_t s, const BlockOps& b, const double* G, double* B, in
   const int64_t d = (int64_t)n * (n + 1) / 2;
   qr_coo_kernel<<<dim3(cdiv(d, 256), b.ncon), 256, 0, s>>>(n, b.ncon, b.con_list, b.con_ptr,
                                                             b.row, b.col, b.val, G, B, ldb, off);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+int qr_pack_lowrank(hipStream_t s, const BlockOps& b, const double* G, double* B, int64_t ldb,
+                    int64_t off) {
+  const int n = b.n;
+  const int64_t d = (int64_t)n * (n + 1) / 2;
+  MSDP_TRY(gemm(s, true, false, n, b.R, n, 1.0, G, n, b.F, n, 0.0, b.FW, n));
+  qr_lowrank_kernel<<<dim3(cdiv(d, 256), b.ncon), 256, 0, s>>>(n, b.ncon, b.con_list, b.con_ptr,
+                                                                b.lam, b.FW, B, ldb, off);
   MSDP_HIP(hipGetLastError());
   return 0;
 }

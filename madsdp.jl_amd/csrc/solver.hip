@@ -59,7 +59,10 @@ struct Blk {
   std::vector<int> h_dcon;
   std::vector<double> h_mats;
   std::vector<double> h_C;
-  bool has_coo = false, has_dense = false;
+  // low-rank: constraint indices, column ranges, factors (n x R), weights
+  std::vector<int> h_lcon, h_lptr;
+  std::vector<double> h_lF, h_lw;
+  bool has_coo = false, has_dense = false, has_lowrank = false;
 };
 
 }  // namespace
@@ -444,6 +447,29 @@ int msdp_solver::finalize() {
       if (!ident) MSDP_TRY(alloc(&o.Hbuf, (size_t)mi * mi));
       o.jt_split = jtprod_split(b.n, mi);
       if (o.jt_split > 1) MSDP_TRY(alloc(&o.jt_part, (size_t)o.jt_split * n2));
+    } else if (b.has_lowrank && !b.h_lcon.empty()) {
+      const int mi = (int)b.h_lcon.size();
+      const int R = b.h_lptr[mi];
+      std::vector<int> fcon(R);
+      for (int j = 0; j < mi; ++j)
+        for (int r = b.h_lptr[j]; r < b.h_lptr[j + 1]; ++r) fcon[r] = b.h_lcon[j];
+      o.kind = BlockOps::LOWRANK;
+      o.ncon = mi;
+      o.R = R;
+      MSDP_TRY(alloc(&o.con_list, mi));
+      MSDP_TRY(alloc(&o.con_ptr, mi + 1));
+      MSDP_TRY(alloc(&o.fcon, R));
+      MSDP_TRY(alloc(&o.F, (size_t)b.n * R));
+      MSDP_TRY(alloc(&o.lam, R));
+      MSDP_TRY(alloc(&o.FW, (size_t)b.n * R));
+      MSDP_TRY(alloc(&o.PL, (size_t)R * R));
+      MSDP_TRY(alloc(&o.PR, (size_t)R * R));
+      MSDP_HIP(hipMemcpyAsync(o.con_list, b.h_lcon.data(), sizeof(int) * mi, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.con_ptr, b.h_lptr.data(), sizeof(int) * (mi + 1), hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.fcon, fcon.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.F, b.h_lF.data(), sizeof(double) * b.n * R, hipMemcpyHostToDevice, st));
+      MSDP_HIP(hipMemcpyAsync(o.lam, b.h_lw.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+      MSDP_TRY(stream_sync(st));  // fcon is a local
     } else if (b.has_coo && !b.h_con.empty()) {
       // sort entries by constraint (stable), then build the ranges
       const int64_t nnz = (int64_t)b.h_con.size();
@@ -606,6 +632,10 @@ int msdp_solver::finalize() {
     std::vector<int>().swap(bb.h_con);
     std::vector<int>().swap(bb.h_row);
     std::vector<int>().swap(bb.h_col);
+    std::vector<int>().swap(bb.h_lcon);
+    std::vector<int>().swap(bb.h_lptr);
+    std::vector<double>().swap(bb.h_lF);
+    std::vector<double>().swap(bb.h_lw);
   }
   std::vector<double>().swap(h_b);
   std::vector<double>().swap(h_dlin);
@@ -634,6 +664,9 @@ int64_t msdp_solver::footprint() const {
       swork_b = std::max(swork_b, std::max(2 * n2 * d, std::min<int64_t>(2 * n2 * mi * d, 64ll << 20)));
     } else if (b.has_coo) {
       bytes += (int64_t)b.h_con.size() * (4 * 3 + 8 + 8 + 4 + 4) + 64;
+    } else if (b.has_lowrank && !b.h_lcon.empty()) {
+      const int64_t mi = (int64_t)b.h_lcon.size(), R = b.h_lptr[mi];
+      bytes += 2 * b.n * R * d + 2 * R * R * d + R * (d + 4) + (2 * mi + 1) * 4 + 64;
     }
   }
   bytes += swork_b;
@@ -1838,6 +1871,8 @@ int msdp_solver::qr_factorize(bool& ok) {
       }
     } else if (o.kind == BlockOps::COO) {
       MSDP_TRY(qr_pack_coo(st, o, bk.G, qB, rows, off));
+    } else if (o.kind == BlockOps::LOWRANK) {
+      MSDP_TRY(qr_pack_lowrank(st, o, bk.G, qB, rows, off));
     }
     off += (int64_t)n * (n + 1) / 2;
   }
@@ -2042,7 +2077,7 @@ int msdp_solver_set_constraints_coo(msdp_solver* s, int32_t block, int64_t nnz, 
   if (block < 0 || block >= s->nb) return as_err("block out of range");
   if (s->finalized) return as_err("problem already finalized");
   Blk& b = s->blk[block];
-  if (b.has_dense) return as_err("block already has dense constraints");
+  if (b.has_dense || b.has_lowrank) return as_err("block already has dense or low-rank constraints");
   for (int64_t e = 0; e < nnz; ++e) {
     if (con[e] < 0 || con[e] >= s->m || row[e] < 0 || row[e] >= b.n || col[e] < 0 || col[e] >= b.n)
       return as_err("constraint entry out of range");
@@ -2062,7 +2097,7 @@ int msdp_solver_set_constraints_dense(msdp_solver* s, int32_t block, int64_t m_i
   if (block < 0 || block >= s->nb) return as_err("block out of range");
   if (s->finalized) return as_err("problem already finalized");
   Blk& b = s->blk[block];
-  if (b.has_coo) return as_err("block already has coo constraints");
+  if (b.has_coo || b.has_lowrank) return as_err("block already has coo or low-rank constraints");
   for (int64_t j = 0; j < m_i; ++j) {
     if (con_idx[j] < 0 || con_idx[j] >= s->m) return as_err("constraint index out of range");
     if (j > 0 && con_idx[j] <= con_idx[j - 1]) return as_err("con_idx must be strictly increasing");
@@ -2070,6 +2105,30 @@ int msdp_solver_set_constraints_dense(msdp_solver* s, int32_t block, int64_t m_i
   b.h_dcon.assign(con_idx, con_idx + m_i);
   b.h_mats.assign(mats, mats + (size_t)m_i * b.n * b.n);
   b.has_dense = true;
+  return 0;
+}
+
+int msdp_solver_set_constraints_lowrank(msdp_solver* s, int32_t block, int64_t m_i,
+                                        const int64_t* con_idx, const int64_t* rank_ptr,
+                                        const double* factors, const double* weights) {
+  GUARD(s);
+  if (block < 0 || block >= s->nb) return as_err("block out of range");
+  if (s->finalized) return as_err("problem already finalized");
+  Blk& b = s->blk[block];
+  if (b.has_coo || b.has_dense) return as_err("block already has coo or dense constraints");
+  if (m_i < 0 || rank_ptr[0] != 0) return as_err("rank_ptr must start at 0");
+  for (int64_t j = 0; j < m_i; ++j) {
+    if (con_idx[j] < 0 || con_idx[j] >= s->m) return as_err("constraint index out of range");
+    if (j > 0 && con_idx[j] <= con_idx[j - 1]) return as_err("con_idx must be strictly increasing");
+    if (rank_ptr[j + 1] <= rank_ptr[j]) return as_err("each constraint needs at least one factor");
+  }
+  const int64_t R = rank_ptr[m_i];
+  if (R > INT32_MAX / 2) return as_err("too many factor columns");
+  b.h_lcon.assign(con_idx, con_idx + m_i);
+  b.h_lptr.assign(rank_ptr, rank_ptr + m_i + 1);
+  b.h_lF.assign(factors, factors + (size_t)R * b.n);
+  b.h_lw.assign(weights, weights + R);
+  b.has_lowrank = true;
   return 0;
 }
 

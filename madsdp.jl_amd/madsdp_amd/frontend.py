@@ -12,11 +12,14 @@ plumbing for the device solver:
 * `LroModel` holds the same fields: per-block cost matrices C_i (dense or
   scipy.sparse), the nblocks x m table A[i][j] of constraint matrices
   (scipy.sparse, dense or None for zero), b, d_lin and C_lin (m x n_s).
-* `ingest(lro, datasparsity)` maps it once to the device upload format
-  (`Model`): a block whose constraint matrices have more than `datasparsity`
-  nonzeros is stored dense (its touching matrices contiguous, the
-  BufferedModelForSchur dense path), otherwise as COO entries -- and a block
-  of diagonal one-hot constraints is recognised for the W o W Schur kernel.
+* `ingest(lro, datasparsity, detect_rank_one)` maps it once to the device
+  upload format (`Model`): a block whose constraint matrices are all given as
+  factors (`LowRankMatrix`) -- or, with `detect_rank_one`, are all rank one,
+  LowRankOpt's detection (README.md:148) -- keeps them as factors; otherwise a
+  block whose matrices have more than `datasparsity` nonzeros is stored dense
+  (its touching matrices contiguous, the BufferedModelForSchur dense path),
+  else as COO entries, and a block of diagonal one-hot constraints is
+  recognised for the W o W Schur kernel.
 * `Solver(lro).solve(**kws)` creates the device solver with the component
   keywords `SolverCore.solve!` forwards (src/moi.jl:53-60), solves, and
   returns `Stats` with the fields src/moi.jl:62-70 sets: status (the
@@ -32,7 +35,7 @@ from typing import Any, List, Optional, Sequence
 
 import numpy as np
 
-from .model import BlockConstraints, Model
+from .model import BlockConstraints, Model, rank_one_factor
 from .solver import MadSDPSolver, Status
 
 # MadNLP status -> SolverCore status symbol (src/moi.jl:16-27)
@@ -58,6 +61,25 @@ def solvercore_status(status) -> str:
         return "unknown"
 
 
+@dataclass
+class LowRankMatrix:
+    """A constraint matrix given by its factors, sum_r weights[r] f_r f_r'
+    with f_r = factors[:, r] (LowRankOpt's factored matrix types)."""
+    factors: Any
+    weights: Any
+
+    def toarray(self) -> np.ndarray:
+        F = np.asarray(self.factors, dtype=np.float64)
+        F = F.reshape(F.shape[0], -1)
+        return (F * np.asarray(self.weights, dtype=np.float64).reshape(1, -1)) @ F.T
+
+    def terms(self):
+        F = np.asarray(self.factors, dtype=np.float64)
+        F = F.reshape(F.shape[0], -1)
+        w = np.asarray(self.weights, dtype=np.float64).reshape(-1)
+        return [(w[r], F[:, r]) for r in range(w.size)]
+
+
 def _dense(M) -> np.ndarray:
     if M is None:
         return None
@@ -67,6 +89,8 @@ def _dense(M) -> np.ndarray:
 
 
 def _nnz(M) -> int:
+    if isinstance(M, LowRankMatrix):
+        return int(np.count_nonzero(M.toarray()))
     if hasattr(M, "nnz"):
         M = M.tocoo()
         return int(np.count_nonzero(M.data))
@@ -84,10 +108,10 @@ class LroModel:
     dims: List[int] = field(default_factory=list)
 
 
-def ingest(lro: LroModel, datasparsity: int = 8) -> Model:
+def ingest(lro: LroModel, datasparsity: int = 8, detect_rank_one: bool = True) -> Model:
     """LroModel -> device upload format, with the datasparsity switch of
-    `BufferedModelForSchur` (src/options.jl:12-14) deciding each block's
-    storage."""
+    `BufferedModelForSchur` (src/options.jl:12-14) and the rank-one
+    detection of LowRankOpt deciding each block's storage."""
     b = np.asarray(lro.b, dtype=np.float64).reshape(-1)
     m = b.size
     d_lin = np.zeros(0) if lro.d_lin is None else np.asarray(lro.d_lin, dtype=np.float64).reshape(-1)
@@ -106,7 +130,19 @@ def ingest(lro: LroModel, datasparsity: int = 8) -> Model:
                 continue
             touching.append((j, Aij))
         maxnnz = max((_nnz(A) for _, A in touching), default=0)
-        if maxnnz > datasparsity:
+        explicit = bool(touching) and all(isinstance(A, LowRankMatrix) for _, A in touching)
+        terms = None
+        if explicit:
+            terms = [A.terms() for _, A in touching]
+        elif detect_rank_one and touching and maxnnz > datasparsity:
+            facs = [A.terms() if isinstance(A, LowRankMatrix) and len(A.terms()) == 1
+                    else rank_one_factor(_dense(A)) for _, A in touching]
+            facs = [f[0] if isinstance(f, list) else f for f in facs]
+            if all(f is not None for f in facs):
+                terms = [[f] for f in facs]
+        if terms is not None:
+            blocks.append(BlockConstraints.lowrank(n, [j for j, _ in touching], terms))
+        elif maxnnz > datasparsity:
             mats = np.stack([_dense(A) for _, A in touching]) if touching else np.zeros((0, n, n))
             if not np.allclose(mats, np.swapaxes(mats, 1, 2)):
                 raise ValueError(f"block {i}: constraint matrices must be symmetric")

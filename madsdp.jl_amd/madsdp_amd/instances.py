@@ -114,6 +114,28 @@ def block_sdp(sizes, m: int, n_s: int, seed: int) -> Model:
     return Model(Cs, blocks, b, d, Clin)
 
 
+def lowrank_sdp(n: int, m: int, n_s: int, seed: int, rank: int = 1, signed: bool = False) -> Model:
+    """Constraints A_j = sum_{r<rank} w_jr f_jr f_jr' (f ~ N(0, 1/n), w = 1 or
+    +-1), stored as factors; X = I, x = 1 and S = I, z = 1 are strictly
+    feasible (b_j = sum_r w_jr |f_jr|^2 + sum(C_lin[j]), C = I + sum_j y0_j A_j)."""
+    rng = np.random.default_rng(seed)
+    F = rng.standard_normal((m, rank, n)) / math.sqrt(n)
+    w = np.where(rng.random((m, rank)) < 0.5, -1.0, 1.0) if signed else np.ones((m, rank))
+    Clin = rng.random((m, n_s))
+    y0 = rng.standard_normal(m)
+    C = np.eye(n)
+    b = np.zeros(m)
+    for j in range(m):
+        for r in range(rank):
+            C += (y0[j] * w[j, r]) * np.outer(F[j, r], F[j, r])
+        b[j] = math.fsum(np.concatenate([w[j, r] * F[j, r] * F[j, r] for r in range(rank)]
+                                        + [Clin[j]]))
+    d = _combination(y0, Clin) + 1.0
+    blk = BlockConstraints.lowrank(n, np.arange(m), [[(w[j, r], F[j, r]) for r in range(rank)]
+                                                     for j in range(m)])
+    return Model([C], [blk], b, d, Clin)
+
+
 def block_sizes(count: int, lo: int, hi: int, seed: int):
     rng = np.random.default_rng(seed)
     return [int(v) for v in rng.integers(lo, hi + 1, size=count)]

@@ -71,6 +71,8 @@ SIGNATURES = {
     "msdp_solver_set_cost_block": (C.c_int, [P, C.c_int32, c_double_p]),
     "msdp_solver_set_constraints_coo": (C.c_int, [P, C.c_int32, C.c_int64, c_int64_p, c_int64_p, c_int64_p, c_double_p]),
     "msdp_solver_set_constraints_dense": (C.c_int, [P, C.c_int32, C.c_int64, c_int64_p, c_double_p]),
+    "msdp_solver_set_constraints_lowrank": (C.c_int, [P, C.c_int32, C.c_int64, c_int64_p, c_int64_p,
+                                                      c_double_p, c_double_p]),
     "msdp_solver_set_rhs": (C.c_int, [P, c_double_p]),
     "msdp_solver_set_scalar_data": (C.c_int, [P, c_double_p, c_double_p]),
     "msdp_solver_configure": (C.c_int, [P, C.POINTER(Options)]),

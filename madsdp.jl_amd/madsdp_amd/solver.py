@@ -226,6 +226,14 @@ class MadSDPSolver:
                 mats = np.ascontiguousarray(blk.mats, dtype=np.float64)
                 idx = np.ascontiguousarray(blk.con_idx, dtype=np.int64)
                 L.check(lib.msdp_solver_set_constraints_dense(self._h, i, idx.size, L.iptr(idx), L.dptr(mats)))
+            elif blk.kind == "lowrank":
+                idx = np.ascontiguousarray(blk.con_idx, dtype=np.int64)
+                ptr = np.ascontiguousarray(blk.rank_ptr, dtype=np.int64)
+                F = np.asfortranarray(blk.factors, dtype=np.float64)
+                w = np.ascontiguousarray(blk.weights, dtype=np.float64)
+                L.check(lib.msdp_solver_set_constraints_lowrank(
+                    self._h, i, idx.size, L.iptr(idx), L.iptr(ptr),
+                    F.ctypes.data_as(L.c_double_p), L.dptr(w)))
             else:
                 con = np.ascontiguousarray(blk.con, dtype=np.int64)
                 row = np.ascontiguousarray(blk.row, dtype=np.int64)

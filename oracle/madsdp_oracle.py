@@ -168,6 +168,10 @@ class _BlockOps:
     """A(X), A*(y) and the Schur contribution of one PSD block."""
 
     def __init__(self, blk, m: int):
+        if blk.kind == "lowrank":
+            # A_j = sum_r w_r f_r f_r' taken by its definition, as dense matrices
+            # (LowRankOpt's factored matrices are ordinary A_ij to the algorithm)
+            blk = type(blk)("dense", blk.n, con_idx=blk.con_idx, mats=blk.dense_mats())
         self.blk = blk
         self.n = blk.n
         self.m = m
