@@ -160,8 +160,18 @@ function options(solver, ws::HIPWorkspace)
     return o
 end
 
+# LowRankOpt keeps factored constraint matrices as factor * Diagonal(scaling) *
+# factor' (its rank-one detection, README.md:148): (factors n x r, weights r),
+# or nothing for any other matrix type.
+function _factors(A)
+    (hasproperty(A, :factor) && hasproperty(A, :scaling)) || return nothing
+    F = Matrix{Float64}(reshape(A.factor, size(A, 1), :))
+    return F, Vector{Float64}(A.scaling)
+end
+
 # LowRankOpt.Model -> device, once (the accessors src/qr_schur.jl:25-37 uses).
-# A block whose constraint matrices have more than `datasparsity` nonzeros
+# A block whose constraint matrices are all factored goes up as factors;
+# otherwise one whose matrices have more than `datasparsity` nonzeros
 # (src/options.jl:12-14) goes up dense: its m_i touching matrices as one
 # contiguous m_i x n x n array; otherwise as COO entries of both triangles.
 function upload!(ws::HIPWorkspace, solver)
@@ -180,6 +190,17 @@ function upload!(ws::HIPWorkspace, solver)
         n = dims[i]
         touching = [(j, LRO.jac(model, j, idx)) for j = 1:solver.m]
         filter!(t -> !iszero(t[2]), touching)
+        facs = [_factors(A) for (_, A) in touching]
+        if !isempty(facs) && all(!isnothing, facs)
+            con = Int64[j - 1 for (j, _) in touching]
+            ptr = cumsum(Int64[0; [length(w) for (_, w) in facs]])
+            F = reduce(hcat, [f for (f, _) in facs])     # n x R, column-major
+            w = reduce(vcat, [w for (_, w) in facs])
+            check(ccall((:msdp_solver_set_constraints_lowrank, libmsdp), Cint,
+                        (Ptr{Cvoid}, Int32, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}),
+                        ws.handle, i - 1, length(con), con, ptr, F, w), "set_constraints_lowrank")
+            continue
+        end
         maxnnz = maximum((SparseArrays.nnz(SparseArrays.sparse(A)) for (_, A) in touching); init = 0)
         if maxnnz > κ
             con = Int64[j - 1 for (j, _) in touching]
