@@ -116,8 +116,8 @@ struct msdp_solver {
   // solves apply them as products) and the workspace that forms them
   double *Hdi = nullptr, *Hdiw = nullptr;
   bool hdi_valid = false;
-  // Forms the block inverses once per factorisation; worth it only when one
-  // factorisation serves many solves (iterative refinement).
+  // Forms the block inverses once per factorisation (every factorisation
+  // serves two or more solves).
   int form_block_inverses() {
     if (!Hdi || hdi_valid) return 0;
     // all full 512-row diagonal blocks in one batched triangular inverse
@@ -1787,7 +1787,11 @@ int msdp_solver::apply_H(const double* v, double* out) {
 // Bunch-Kaufman LDL' (MadNLPSchur).
 int msdp_solver::hsolve(double* v) {
   if (ldlt()) return sytrs_bk(st, (int)m, H, m, ldlt_ipiv(), v);
-  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2);
+  // every factorisation serves at least the predictor and corrector solves:
+  // the 512 x 512 diagonal-block inverses (one batched launch sequence) turn
+  // each super-block's dependent 64-block steps into one wide product
+  MSDP_TRY(form_block_inverses());
+  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2, hdi_valid ? Hdi : nullptr);
 }
 
 // schur_solve! (schur_solvers.jl:21).  With an FP32-assembled H, the FP64
