@@ -97,10 +97,16 @@ def stream_cpu_baseline(models, budget_s):
             one = {"value": k / (time.perf_counter() - t1), "cores": 1, "solves": k}
     except Exception:
         pass
-    return {"value": done / t_all, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (numpy/scipy OpenBLAS, {threads} threads) solving the first "
-                      f"{done} instances of the timed stream back to back in {t_all:.1f} s",
-            "single_thread": one}
+    multi = {"value": done / t_all, "cores": threads, "solves": done}
+    # the faster configuration is the baseline (one thread wins on these
+    # small instances: BLAS threading costs more than it brings), the other
+    # is reported beside it
+    best, other = (one, multi) if one and one["value"] > multi["value"] else (multi, one)
+    return {"value": best["value"], "unit": "solves/s", "cores": best["cores"], "kind": "port",
+            "sample": f"FP64 oracle (numpy/scipy OpenBLAS, {best['cores']} thread(s)) solving the "
+                      f"first {best['solves']} instances of the timed stream back to back "
+                      f"(the first {done} took {t_all:.1f} s with {threads} threads)",
+            "other_threads": other}
 
 
 def run_stream(args):
