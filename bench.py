@@ -273,7 +273,7 @@ def main():
                     help="kernel timed live for the roofline: trd = trd_panel_product_kernel "
                          "(the dominant kernel of the max-cut solve), gemm = the FP64 GEMM class, "
                          "gemm32 = the FP32 GEMM class")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "bf16x3"])
     ap.add_argument("--schur", default="cholesky", choices=["cholesky", "qr", "ldlt"],
                     help="single-instance workloads: Schur solver (DenseCholesky, QRSchur "
                          "or MadNLPSchur)")

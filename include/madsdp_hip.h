@@ -69,10 +69,14 @@ typedef struct msdp_options {
   double dc_beta, dc_delta, dc_A;  /* DualCentering 0.2, 1, 2 (dc_A < 0: theoretical) */
   int32_t dc_centering_max_iter;   /* 50 */
   /* precision of the Schur assembly: 0 = FP64 (the reference's arithmetic);
-   * 1 = mixed: FP32 MFMA assembly of the dense-constraint blocks while
+   * 1 = mixed FP32: FP32 MFMA assembly of the dense-constraint blocks while
    * mu >= mu_switch, FP64 Cholesky, and every Schur solve refined against
    * the FP64 operator (at most refine_max_iter steps, FP64 assembly if the
-   * refinement stalls).  Not in the reference: an extension of this library. */
+   * refinement stalls); 2 = mixed BF16x3: the same with the FP64 scaled
+   * constraint panel B split into bf16 hi + lo and H = B'B formed by three
+   * bf16 MFMA products (FP32 accumulation), the refinement applying the
+   * FP64 operator through B.  Not in the reference: an extension of this
+   * library. */
   int32_t precision;               /* 0 */
   double mu_switch;                /* 1e-5 */
   int32_t refine_max_iter;         /* 10 */
@@ -311,6 +315,21 @@ int msdp_k_eig_min2(void* stream, int32_t n, const double* A0, const double* A1,
 int msdp_k_gemm32(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
                   float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
                   double beta, double* C, int64_t ldc);
+/* lower triangle of C(FP64) = alpha A'A + beta C by BF16x3 split products
+ * (A is K x N FP64: hi = bf16(A), lo = bf16(A - hi), C = alpha (hi'hi + hi'lo
+ * + lo'hi) on the bf16 MFMA with FP32 accumulation; the mixed-precision
+ * Schur assembly H = B'B behind LRO.schur_complement!, src/solver.jl:304);
+ * work: msdp_k_syrk_bf16x3_work_bytes(N, K) bytes of device memory for the
+ * split operand; no counterpart in the reference */
+int msdp_k_syrk_bf16x3(void* stream, int32_t N, int32_t K, double alpha, const double* A,
+                       int64_t lda, double beta, double* C, int64_t ldc, void* work);
+size_t msdp_k_syrk_bf16x3_work_bytes(int32_t N, int32_t K);
+/* out = B'(B v) (+ out when accumulate) in FP64 for B rows x cols: the FP64
+ * Schur operator a BF16x3-assembled solve is refined against; work holds
+ * msdp_k_panel_apply_work(rows, cols) doubles; no counterpart in the reference */
+int msdp_k_panel_apply(void* stream, int32_t rows, int32_t cols, const double* B, int64_t ldb,
+                       const double* v, double* out, int32_t accumulate, double* work);
+size_t msdp_k_panel_apply_work(int32_t rows, int32_t cols);
 /* Hadamard Schur complement of one-hot constraints: H[i,j] += L[i,j] R[j,i]
  * for i >= j (max-cut H = W o W, src/components.jl:102-104) */
 int msdp_k_schur_onehot(void* stream, int32_t m, const double* L, int64_t ldl, const double* R,

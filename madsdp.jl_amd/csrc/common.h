@@ -394,6 +394,26 @@ int gemm32_to64(hipStream_t s, bool ta, bool tb, int M, int N, int K, float alph
 int d2f(hipStream_t s, int64_t count, const double* x, float* y);
 int svec_pack32(hipStream_t s, int n, int cb, const float* P, float* B, int64_t ldb);
 
+// --------------------------------------------------------------------------
+// BF16x3 split products (bf16x3.hip): the mixed-precision Schur assembly
+// H = B'B from hi = bf16(B), lo = bf16(B - hi) on the bf16 MFMA, and the
+// FP64 panel operator B'(B v) its refinement applies
+// --------------------------------------------------------------------------
+// leading dimension of a split panel with `rows` rows (a multiple of 32)
+int64_t bf16x3_ld(int64_t rows);
+int bf16_split(hipStream_t s, int rows, int cols, const double* B, int64_t ldb, __bf16* hi,
+               __bf16* lo, int64_t ldh);
+// C = alpha (hi'hi + hi'lo + lo'hi) + beta C, lower triangle of M x M;
+// hi, lo: K x M, K and lda multiples of 32
+int syrk_bf16x3(hipStream_t s, int M, int K, double alpha, const __bf16* hi, const __bf16* lo,
+                int64_t lda, double beta, double* C, int64_t ldc);
+size_t panel_apply_work_elems(int rows, int cols);
+// out (+)= B'(B v), B rows x cols FP64, deterministic
+int panel_apply(hipStream_t s, int rows, int cols, const double* B, int64_t ldb, const double* v,
+                double* out, bool accumulate, double* work);
+int gather_vec(hipStream_t s, int n, const int* con, const double* v, double* vb);
+int scatter_add_vec(hipStream_t s, int n, const int* con, const double* u, double* out);
+
 }  // namespace msdp
 
 namespace msdp {

@@ -65,6 +65,11 @@ int scatter_lower_add(hipStream_t st, int mi, const int* con, const double* Hb, 
                       int64_t ldh);
 int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
                 double* work, size_t work_elems);
+// rows of a dense block's scaled constraint panel: n(n+1)/2 (svec) or n^2 (vec)
+int64_t dense_panel_rows(int n, bool full_vec);
+// b.Bbuf[:, j] = svec(P_j) (vec for HKM) in FP64, P_j = op(Lm) A_j op(F2)
+int dense_panel(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* work,
+                size_t work_elems);
 
 enum { SOP_ZINV = 0, SOP_RHS = 1, SOP_DIRS = 2, SOP_CORR = 3, SOP_XZ = 4 };
 int scalar_op(hipStream_t st, int op, int ns, double mu, const double* x, const double* z,
@@ -78,9 +83,11 @@ int sqrt_min(hipStream_t st, int n, const double* lam, double* D, double* minlam
 int scale_by_diag(hipStream_t st, int n, const double* W, const double* M, double* out);
 // H y = b from H = L L' (L and its transpose Lt, the 64 x 64 diagonal-block
 // inverses dinv); with di512 (the inverses of the full 512 x 512 diagonal
-// blocks, consecutive) those blocks are applied as products.
+// blocks, consecutive) those blocks are applied as products, with di512t
+// (their transposes) the forward ones read row-wise too.
 int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
-          const double* dinv, double* b, double* tmp, const double* di512 = nullptr);
+          const double* dinv, double* b, double* tmp, const double* di512 = nullptr,
+          const double* di512t = nullptr);
 int scale_cols_vec(hipStream_t st, int rows, int cols, double* A, int64_t lda, const double* d);
 int vec_min(hipStream_t st, int n, const double* v, double* out);
 int schur_onehot_identity(hipStream_t st, int m, const double* L, int64_t ldl, const double* R,

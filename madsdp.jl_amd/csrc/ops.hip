@@ -232,6 +232,42 @@ int scatter_lower_add(hipStream_t st, int mi, const int* con, const double* Hb, 
   return 0;
 }
 
+int64_t dense_panel_rows(int n, bool full_vec) {
+  return full_vec ? (int64_t)n * n : (int64_t)n * (n + 1) / 2;
+}
+
+// The scaled constraint panel of a dense block in FP64: P_j = op(Lm) (A_j
+// op(F2)) by batched products, b.Bbuf[:, j] = svec(P_j) (vec(P_j) for HKM),
+// so that <A_j, L A_k R> = B[:, j]' B[:, k].
+int dense_panel(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* work,
+                size_t work_elems) {
+  MSDP_TRY(rec_forbid("dense_panel"));
+  const int n = b.n;
+  const int64_t n2 = (int64_t)n * n;
+  const int64_t drow = dense_panel_rows(n, sc.full_vec);
+  const int mi = b.ncon;
+  double* Bm = b.Bbuf;  // drow x mi
+  // per-chunk buffers for T_j and P_j
+  const int64_t per = 2 * n2;
+  int cb = (int)std::max<int64_t>(1, std::min<int64_t>(mi, (int64_t)(work_elems / per)));
+  if (cb > 64 && cb < mi) cb = cb / 64 * 64;  // whole 64-constraint batches (measured faster)
+  double* T = work;
+  double* P = work + (int64_t)cb * n2;
+  for (int j0 = 0; j0 < mi; j0 += cb) {
+    const int c = std::min(cb, mi - j0);
+    MSDP_TRY(gemm(st, false, sc.transF2, n, n, n, 1.0, b.mats + (int64_t)j0 * n2, n, sc.F2, n, 0.0,
+                  T, n, c, n2, 0, n2));
+    double* dst = sc.full_vec ? Bm + (int64_t)j0 * n2 : P;
+    MSDP_TRY(gemm(st, sc.transL, false, n, n, n, 1.0, sc.Lm, n, T, n, 0.0, dst, n, c, 0, n2, n2));
+    if (!sc.full_vec) {
+      dim3 g(cdiv(drow, 256), c);
+      svec_pack_kernel<<<g, 256, 0, st>>>(n, c, P, Bm + (int64_t)j0 * drow, drow);
+      MSDP_HIP(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
 int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
                 double* work, size_t work_elems) {
   MSDP_TRY(rec_forbid("schur_block"));
@@ -263,30 +299,12 @@ int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, doubl
     MSDP_HIP(hipGetLastError());
     return 0;
   }
-  // dense: P_j = op(Lm) (A_j op(F2)); B[:, j] = svec(P_j) or vec(P_j); Hb = B'B,
-// scattered into H unless H is null (then the caller scatters b.Hbuf later)
-  const int64_t n2 = (int64_t)n * n;
-  const int64_t drow = sc.full_vec ? n2 : (int64_t)n * (n + 1) / 2;
+  // dense: B[:, j] = svec(P_j) or vec(P_j) (dense_panel); Hb = B'B,
+  // scattered into H unless H is null (then the caller scatters b.Hbuf later)
+  MSDP_TRY(dense_panel(st, b, sc, work, work_elems));
+  const int64_t drow = dense_panel_rows(b.n, sc.full_vec);
   const int mi = b.ncon;
-  double* Bm = b.Bbuf;  // drow x mi
-  // per-chunk buffers for T_j and P_j
-  const int64_t per = 2 * n2;
-  int cb = (int)std::max<int64_t>(1, std::min<int64_t>(mi, (int64_t)(work_elems / per)));
-  if (cb > 64 && cb < mi) cb = cb / 64 * 64;  // whole 64-constraint batches (measured faster)
-  double* T = work;
-  double* P = work + (int64_t)cb * n2;
-  for (int j0 = 0; j0 < mi; j0 += cb) {
-    const int c = std::min(cb, mi - j0);
-    MSDP_TRY(gemm(st, false, sc.transF2, n, n, n, 1.0, b.mats + (int64_t)j0 * n2, n, sc.F2, n, 0.0,
-                  T, n, c, n2, 0, n2));
-    double* dst = sc.full_vec ? Bm + (int64_t)j0 * n2 : P;
-    MSDP_TRY(gemm(st, sc.transL, false, n, n, n, 1.0, sc.Lm, n, T, n, 0.0, dst, n, c, 0, n2, n2));
-    if (!sc.full_vec) {
-      dim3 g(cdiv(drow, 256), c);
-      svec_pack_kernel<<<g, 256, 0, st>>>(n, c, P, Bm + (int64_t)j0 * drow, drow);
-      MSDP_HIP(hipGetLastError());
-    }
-  }
+  double* Bm = b.Bbuf;
   if (b.dcon_identity) {
     MSDP_TRY(gemm(st, true, false, mi, mi, (int)drow, 1.0, Bm, drow, Bm, drow, 1.0, H, m, 1, 0, 0,
                   0, true));
@@ -562,47 +580,88 @@ __global__ __launch_bounds__(256) void trsv_blockinv_kernel(int K0, const double
   if (w == 0) out[K0 + i] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
-// out[K0 + i] = sum_{j >= i} D[j + 512 i] in[K0 + j]: the transposed inverse,
-// one wavefront per output (column i of D).
+// out[K0 + i] = sum_j D[j + 512 i] in[K0 + j] over j >= i (upper = 1: the
+// transposed inverse, D' applied) or j <= i (upper = 0: D holds the transposed
+// inverse, so column i is row i of the inverse), one wavefront per output:
+// the column is read contiguously and summed by wave_sum.
 __global__ __launch_bounds__(256) void trsv_blockinv_t_kernel(int K0, const double* D,
-                                                              const double* in, double* out) {
+                                                              const double* in, double* out,
+                                                              int upper) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const double* col = D + (int64_t)i * TRSV_SB;
+  const int j0 = upper ? i : 0, j1 = upper ? TRSV_SB : i + 1;
   double a0 = 0.0, a1 = 0.0;
-  int j = i + lane;
-  for (; j + 64 < TRSV_SB; j += 128) {
+  int j = j0 + lane;
+  for (; j + 64 < j1; j += 128) {
     a0 += col[j] * in[K0 + j];
     a1 += col[j + 64] * in[K0 + j + 64];
   }
-  if (j < TRSV_SB) a0 += col[j] * in[K0 + j];
+  if (j < j1) a0 += col[j] * in[K0 + j];
   const double acc = wave_sum(a0 + a1);
   if (lane == 0) out[K0 + i] = acc;
 }
 
+// target[r0 + r] -= sum_{c < KB} Mt[(K0 + c) + (r0 + r) ld] v[K0 + c]: the
+// super-block's panel applied through the transposed storage, where each
+// row's KB coefficients are one contiguous run.  One wavefront per row (the
+// run read in 64-wide coalesced steps, summed by wave_sum).
+__global__ __launch_bounds__(256) void trsv_rowdot_kernel(int r0, int rows, int K0, int KB,
+                                                          const double* __restrict__ Mt, int64_t ld,
+                                                          const double* __restrict__ v,
+                                                          double* __restrict__ target) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  const double* run = Mt + K0 + (int64_t)(r0 + r) * ld;
+  const double* x = v + K0;
+  double a0 = 0.0, a1 = 0.0;
+  int c = lane;
+  for (; c + 64 < KB; c += 128) {
+    a0 += run[c] * x[c];
+    a1 += run[c + 64] * x[c + 64];
+  }
+  if (c < KB) a0 += run[c] * x[c];
+  const double acc = wave_sum(a0 + a1);
+  if (lane == 0) target[r0 + r] -= acc;
+}
+
 int potrs(hipStream_t st, int m, const double* L, const double* Lt, int64_t ld,
-          const double* dinv, double* b, double* tmp, const double* di512) {
+          const double* dinv, double* b, double* tmp, const double* di512,
+          const double* di512t) {
   if (m <= 0) return 0;
   const int nsb = cdiv(m, TRSV_SB);
   // forward: L x = b (x -> tmp, b overwritten by the updates)
   for (int k = 0; k < nsb; ++k) {
     const int K0 = k * TRSV_SB, KB = std::min(TRSV_SB, m - K0);
-    if (di512 && KB == TRSV_SB)
-      trsv_blockinv_kernel<<<TRSV_SB / 64, 256, 0, st>>>(K0, di512 + (size_t)k * TRSV_SB * TRSV_SB, b, tmp);
-    else
+    if (di512 && KB == TRSV_SB) {
+      if (di512t)
+        trsv_blockinv_t_kernel<<<TRSV_SB / 4, 256, 0, st>>>(
+            K0, di512t + (size_t)k * TRSV_SB * TRSV_SB, b, tmp, 0);
+      else
+        trsv_blockinv_kernel<<<TRSV_SB / 64, 256, 0, st>>>(
+            K0, di512 + (size_t)k * TRSV_SB * TRSV_SB, b, tmp);
+    } else {
       trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, L, ld, dinv + (size_t)(K0 / 64) * 4096, b, tmp, 0);
+    }
     const int rest = m - K0 - KB;
-    if (rest > 0)
-      trsv_panel_kernel<<<cdiv(rest, 64), 256, 0, st>>>(K0 + KB, rest, K0, KB, L, ld, tmp, b);
+    if (rest > 0) {
+      // rows below: row r of L's columns K0.. is column r of L' (contiguous)
+      if (Lt)
+        trsv_rowdot_kernel<<<cdiv(rest, 4), 256, 0, st>>>(K0 + KB, rest, K0, KB, Lt, ld, tmp, b);
+      else
+        trsv_panel_kernel<<<cdiv(rest, 64), 256, 0, st>>>(K0 + KB, rest, K0, KB, L, ld, tmp, b);
+    }
   }
   // backward: L' z = x (z -> b, tmp overwritten by the updates)
   for (int k = nsb - 1; k >= 0; --k) {
     const int K0 = k * TRSV_SB, KB = std::min(TRSV_SB, m - K0);
     if (di512 && KB == TRSV_SB)
-      trsv_blockinv_t_kernel<<<TRSV_SB / 4, 256, 0, st>>>(K0, di512 + (size_t)k * TRSV_SB * TRSV_SB, tmp, b);
+      trsv_blockinv_t_kernel<<<TRSV_SB / 4, 256, 0, st>>>(K0, di512 + (size_t)k * TRSV_SB * TRSV_SB, tmp, b, 1);
     else
       trsv_super_kernel<<<1, 1024, 0, st>>>(K0, KB, Lt, ld, dinv + (size_t)(K0 / 64) * 4096, tmp, b, 1);
-    if (K0 > 0) trsv_panel_kernel<<<cdiv(K0, 64), 256, 0, st>>>(0, K0, K0, KB, Lt, ld, b, tmp);
+    // rows above: row r of L' 's columns K0.. is column r of L (contiguous)
+    if (K0 > 0) trsv_rowdot_kernel<<<cdiv(K0, 4), 256, 0, st>>>(0, K0, K0, KB, L, ld, b, tmp);
   }
   MSDP_HIP(hipGetLastError());
   return 0;

@@ -53,6 +53,12 @@ struct Blk {
   // mixed precision: FP32 copies of the dense constraints, the scaled
   // constraint panel and the scaling factors
   float *mats32 = nullptr, *B32 = nullptr, *Lm32 = nullptr, *F32 = nullptr;
+  // BF16x3 assembly: the split FP64 panel (hi, lo: bf16x3_ld(drow) x ncon);
+  // panel_fp64 marks that ops.Bbuf holds the FP64 panel of the current
+  // factorisation (the refinement applies H through it)
+  __bf16 *Bh = nullptr, *Bl = nullptr;
+  bool panel_fp64 = false;
+  int64_t panel_rows = 0;
   // host copies of the constraint data until finalize
   std::vector<int> h_con, h_row, h_col;
   std::vector<double> h_val;
@@ -114,7 +120,9 @@ struct msdp_solver {
   double *H = nullptr, *Hs = nullptr, *Ht = nullptr, *Hdinv = nullptr, *wm2 = nullptr;
   // inverses of the factor's full 512 x 512 diagonal blocks (triangular
   // solves apply them as products) and the workspace that forms them
-  double *Hdi = nullptr, *Hdiw = nullptr;
+  // and their transposes (the forward sweep reads rows of the inverses as
+  // contiguous columns)
+  double *Hdi = nullptr, *Hdit = nullptr, *Hdiw = nullptr;
   bool hdi_valid = false;
   // Forms the block inverses once per factorisation (every factorisation
   // serves two or more solves).
@@ -123,6 +131,8 @@ struct msdp_solver {
     // all full 512-row diagonal blocks in one batched triangular inverse
     MSDP_TRY(trtri_lower_batched(st, 512, (int)(m / 512), H, m, 512 * (m + 1), Hdi, 512,
                                  512 * 512, Hdiw, (int64_t)trtri_work_elems(512)));
+    for (int64_t k = 0; k < m / 512; ++k)
+      MSDP_TRY(transpose(st, 512, 512, Hdi + k * 512 * 512, 512, Hdit + k * 512 * 512, 512));
     hdi_valid = true;
     return 0;
   }
@@ -142,6 +152,9 @@ struct msdp_solver {
   float* swork32 = nullptr;
   size_t swork32_elems = 0;
   double *rh = nullptr, *rr = nullptr, *rdl = nullptr;
+  // BF16x3: panel-operator work (partials, t) and the gathered / product
+  // vectors of blocks that touch a subset of the constraints
+  double *pwork = nullptr, *pvb = nullptr, *pu = nullptr;
   double reg_used = 0.0;
   bool h_approx = false, force_fp64 = false;
   int lowp_count = 0, fallback_count = 0, refine_steps = 0;
@@ -289,6 +302,14 @@ struct msdp_solver {
     }
   };
   int64_t footprint() const;
+  // the dense assembly's chunk buffer: 512 MiB, or 64 MiB for problems whose
+  // auxiliary streams each hold a copy (setup_aux)
+  size_t swork_cap_bytes() const {
+    int64_t nmax = 0;
+    for (const auto& b : blk) nmax = std::max<int64_t>(nmax, b.n);
+    const bool aux_copies = nb >= 2 && nmax <= JACOBI_VB_MAX;
+    return aux_copies ? ((size_t)64 << 20) : ((size_t)512 << 20);
+  }
 
   double* slot(int k) { return sc + k; }
   // slots: 0..31 global, 32 + 8*i per block
@@ -330,6 +351,8 @@ struct msdp_solver {
   int nt_scaling(Blk& bk, int i);
   int ensure_mixed();
   int schur_block_fp32(Blk& bk, const SchurScaling& sc);
+  int schur_block_bf16x3(Blk& bk, const SchurScaling& sc);
+  bool bf16x3() const { return opt.precision == 2; }
   int apply_H(const double* v, double* out);
   int schur_solve(double* v);
   int factorize_schur(bool& ok);
@@ -555,6 +578,7 @@ int msdp_solver::finalize() {
   MSDP_TRY(alloc(&Hdinv, potrf_work_elems((int)mm)));
   if (m >= 512) {
     MSDP_TRY(alloc(&Hdi, (size_t)(m / 512) * 512 * 512));
+    MSDP_TRY(alloc(&Hdit, (size_t)(m / 512) * 512 * 512));
     MSDP_TRY(alloc(&Hdiw, (size_t)(m / 512) * trtri_work_elems(512)));
   }
   MSDP_TRY(alloc(&wm2, mm));
@@ -599,12 +623,14 @@ int msdp_solver::finalize() {
     }
   }
   // chunk buffers of the dense Schur assembly: as many constraints per chunk
-  // as fit in 64 MiB (fewer, larger batched products for small blocks)
+  // as fit in 512 MiB (fewer, larger batched products: at n = 250, 512
+  // constraints per chunk fill the chip with 2048 tiles), 64 MiB where the
+  // auxiliary streams keep a copy each
   size_t want = 0;
   for (auto& bb : blk)
     if (bb.ops.kind == BlockOps::DENSE)
       want = std::max(want, 2 * (size_t)bb.n * bb.n * (size_t)bb.ops.ncon);
-  want = std::min(want, (size_t)(64u << 20) / 8);
+  want = std::min(want, swork_cap_bytes() / 8);
   for (auto& bb : blk)
     if (bb.ops.kind == BlockOps::DENSE) want = std::max(want, 2 * (size_t)bb.n * bb.n);
   swork_elems = want;
@@ -661,7 +687,13 @@ int64_t msdp_solver::footprint() const {
       bytes += 2 * mi * n2 * d + mi * 4 + (mi == m ? 0 : mi * mi * d);
       bytes += (int64_t)jtprod_split(b.n, (int)mi) * n2 * d;
       if (opt.precision == 1) bytes += 2 * mi * n2 * 4 + 2 * n2 * 4;
-      swork_b = std::max(swork_b, std::max(2 * n2 * d, std::min<int64_t>(2 * n2 * mi * d, 64ll << 20)));
+      if (opt.precision == 2) {
+        const int64_t drow = dense_panel_rows(b.n, opt.direction != MSDP_DIR_NT);
+        bytes += 2 * bf16x3_ld(drow) * mi * 2 + (int64_t)panel_apply_work_elems((int)drow, (int)mi) * d +
+                 2 * mi * d;
+      }
+      swork_b = std::max(swork_b, std::max(2 * n2 * d, std::min<int64_t>(2 * n2 * mi * d,
+                                                                       (int64_t)swork_cap_bytes())));
     } else if (b.has_coo) {
       bytes += (int64_t)b.h_con.size() * (4 * 3 + 8 + 8 + 4 + 4) + 64;
     } else if (b.has_lowrank && !b.h_lcon.empty()) {
@@ -672,7 +704,7 @@ int64_t msdp_solver::footprint() const {
   bytes += swork_b;
   bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
-  bytes += (mm / 512) * (512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdiw
+  bytes += (mm / 512) * (2 * 512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdit, Hdiw
   bytes += ((int64_t)syevd_work_elems((int)nmax) + trtri_work_elems((int)nmax) + 3 * tot) * d;
   // the auxiliary streams' scratch of multi-block problems (setup_aux)
   if (nb >= 2 && nmax <= JACOBI_VB_MAX) {
@@ -1032,11 +1064,13 @@ int msdp_solver::factorize_schur(bool& ok) {
   ok = false;
   if (opt.schur == MSDP_SCHUR_QR) return qr_factorize(ok);
   if (m == 0) { ok = true; return 0; }
-  // mixed precision: FP32 assembly of the dense blocks while mu >= mu_switch
-  const bool lowp = opt.precision == 1 && !dual_centering() && mu >= opt.mu_switch && !force_fp64 &&
+  // mixed precision: FP32 or BF16x3 assembly of the dense blocks while
+  // mu >= mu_switch
+  const bool lowp = opt.precision != 0 && !dual_centering() && mu >= opt.mu_switch && !force_fp64 &&
                     lowp_eligible();
   bool any_lowp = false;
   if (lowp) MSDP_TRY(ensure_mixed());
+  for (auto& bk : blk) bk.panel_fp64 = false;
   MSDP_TRY(fill(st, m * m, 0.0, Hs));
   // Multi-block: the per-block Schur contributions of dense blocks that touch
   // a subset of the constraints are formed concurrently into their own
@@ -1063,7 +1097,7 @@ int msdp_solver::factorize_schur(bool& ok) {
     SchurScaling sc;
     MSDP_TRY(schur_scaling(bk, sc));
     if (lowp && bk.ops.kind == BlockOps::DENSE) {
-      MSDP_TRY(schur_block_fp32(bk, sc));
+      MSDP_TRY(bf16x3() ? schur_block_bf16x3(bk, sc) : schur_block_fp32(bk, sc));
       any_lowp = true;
       continue;
     }
@@ -1705,6 +1739,30 @@ int msdp_solver::dc_iterate() {  // :266-289
 // Mixed precision: FP32 MFMA Schur assembly + FP64 iterative refinement
 // ==========================================================================
 int msdp_solver::ensure_mixed() {
+  if (bf16x3()) {
+    size_t pw = 0, pv = 0;
+    bool need_pw = false;
+    for (auto& bk : blk) {
+      if (bk.ops.kind != BlockOps::DENSE) continue;
+      const int mi = bk.ops.ncon;
+      const int64_t drow = dense_panel_rows(bk.n, !nt());
+      pw = std::max(pw, panel_apply_work_elems((int)drow, mi));
+      if (!bk.ops.dcon_identity) pv = std::max(pv, (size_t)mi);
+      if (bk.Bh) continue;
+      need_pw = true;
+      const size_t ld = (size_t)bf16x3_ld(drow);
+      MSDP_TRY(alloc(&bk.Bh, ld * mi));
+      MSDP_TRY(alloc(&bk.Bl, ld * mi));
+    }
+    if (need_pw) {
+      MSDP_TRY(alloc(&pwork, pw));
+      if (pv > 0) {
+        MSDP_TRY(alloc(&pvb, pv));
+        MSDP_TRY(alloc(&pu, pv));
+      }
+    }
+    return 0;
+  }
   size_t want = 0;
   for (auto& bk : blk) {
     if (bk.ops.kind != BlockOps::DENSE || bk.mats32) continue;
@@ -1721,6 +1779,28 @@ int msdp_solver::ensure_mixed() {
     MSDP_TRY(alloc(&swork32, want));
     swork32_elems = want;
   }
+  return 0;
+}
+
+// <A_j, L A_k R> of a dense block by BF16x3 split products: the FP64 panel
+// B (svec / vec of P_j = op(Lm) A_j op(F2), the FP64 path's own), split into
+// hi = bf16(B) and lo = bf16(B - hi), then H += hi'hi + hi'lo + lo'hi on the
+// bf16 MFMA with FP32 accumulation.  The FP64 panel stays for the refinement.
+int msdp_solver::schur_block_bf16x3(Blk& bk, const SchurScaling& sc) {
+  const BlockOps& o = bk.ops;
+  const int mi = o.ncon;
+  MSDP_TRY(dense_panel(st, o, sc, swork, swork_elems));
+  const int64_t drow = dense_panel_rows(bk.n, sc.full_vec);
+  const int64_t ld = bf16x3_ld(drow);
+  MSDP_TRY(bf16_split(st, (int)drow, mi, o.Bbuf, drow, bk.Bh, bk.Bl, ld));
+  if (o.dcon_identity) {
+    MSDP_TRY(syrk_bf16x3(st, mi, (int)ld, 1.0, bk.Bh, bk.Bl, ld, 1.0, Hs, m));
+  } else {
+    MSDP_TRY(syrk_bf16x3(st, mi, (int)ld, 1.0, bk.Bh, bk.Bl, ld, 0.0, o.Hbuf, mi));
+    MSDP_TRY(scatter_lower_add(st, mi, o.con_list, o.Hbuf, Hs, m));
+  }
+  bk.panel_fp64 = true;
+  bk.panel_rows = drow;
   return 0;
 }
 
@@ -1763,6 +1843,19 @@ int msdp_solver::apply_H(const double* v, double* out) {
   MSDP_TRY(axpby(st, m, reg_used, v, 0.0, v, out));
   for (auto& bk : blk) {
     if (bk.ops.kind == BlockOps::EMPTY) continue;
+    if (bk.panel_fp64) {
+      // the block's H_jk = B[:, j]' B[:, k] with its FP64 panel
+      const BlockOps& o = bk.ops;
+      const int drow = (int)bk.panel_rows;
+      if (o.dcon_identity) {
+        MSDP_TRY(panel_apply(st, drow, o.ncon, o.Bbuf, drow, v, out, true, pwork));
+      } else {
+        MSDP_TRY(gather_vec(st, o.ncon, o.con_list, v, pvb));
+        MSDP_TRY(panel_apply(st, drow, o.ncon, o.Bbuf, drow, pvb, pu, false, pwork));
+        MSDP_TRY(scatter_add_vec(st, o.ncon, o.con_list, pu, out));
+      }
+      continue;
+    }
     const int n = bk.n;
     const double *L, *R;
     if (dual_centering()) { L = bk.Si; R = bk.Si; }
@@ -1791,7 +1884,8 @@ int msdp_solver::hsolve(double* v) {
   // the 512 x 512 diagonal-block inverses (one batched launch sequence) turn
   // each super-block's dependent 64-block steps into one wide product
   MSDP_TRY(form_block_inverses());
-  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2, hdi_valid ? Hdi : nullptr);
+  return potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2, hdi_valid ? Hdi : nullptr,
+               hdi_valid ? Hdit : nullptr);
 }
 
 // schur_solve! (schur_solvers.jl:21).  With an FP32-assembled H, the FP64
@@ -1801,9 +1895,10 @@ int msdp_solver::schur_solve(double* v) {
   if (!h_approx) return hsolve(v);
   MSDP_TRY(form_block_inverses());
   const double* di = hdi_valid ? Hdi : nullptr;
+  const double* dit = hdi_valid ? Hdit : nullptr;
   MSDP_HIP(hipMemcpyAsync(rh, v, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
   MSDP_TRY(dot(st, m, rh, rh, slot(15), red));
-  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2, di));
+  MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, v, wm2, di, dit));
   double prev = INFINITY, hn = 0.0, rn = 0.0;
   for (int it = 0;; ++it) {
     MSDP_TRY(apply_H(v, rr));
@@ -1814,7 +1909,7 @@ int msdp_solver::schur_solve(double* v) {
     rn = std::sqrt(hsc[16]);
     if (!(rn > 1e-12 * hn) || it >= opt.refine_max_iter || !(rn < 0.5 * prev)) break;
     prev = rn;
-    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, rr, wm2, di));
+    MSDP_TRY(potrs(st, (int)m, H, Ht, m, Hdinv, rr, wm2, di, dit));
     MSDP_TRY(axpby(st, m, 1.0, v, 1.0, rr, v));
     ++refine_steps;
   }
@@ -2178,6 +2273,8 @@ int msdp_solver_configure(msdp_solver* s, const msdp_options* o) {
   if (o->schur == MSDP_SCHUR_QR &&
       !(o->algorithm == MSDP_ALG_PRIMAL_DUAL && o->direction == MSDP_DIR_NT))
     return as_err("QRSchur requires PrimalDual with the NT direction.");
+  if (o->precision < 0 || o->precision > 2)
+    return as_err("precision must be 0 (FP64), 1 (mixed FP32) or 2 (mixed BF16x3)");
   s->opt = *o;
   return 0;
 }
@@ -2441,6 +2538,26 @@ int msdp_k_gemm32(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, in
                   double beta, double* C, int64_t ldc) {
   return gemm32_to64((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta,
                      C, ldc, false);
+}
+int msdp_k_syrk_bf16x3(void* stream, int32_t N, int32_t K, double alpha, const double* A,
+                       int64_t lda, double beta, double* C, int64_t ldc, void* work) {
+  hipStream_t st = (hipStream_t)stream;
+  if (N <= 0 || K <= 0) return 0;
+  const int64_t ld = bf16x3_ld(K);
+  __bf16* hi = reinterpret_cast<__bf16*>(work);
+  __bf16* lo = hi + ld * N;
+  MSDP_TRY(bf16_split(st, K, N, A, lda, hi, lo, ld));
+  return syrk_bf16x3(st, N, (int)ld, alpha, hi, lo, ld, beta, C, ldc);
+}
+size_t msdp_k_syrk_bf16x3_work_bytes(int32_t N, int32_t K) {
+  return N <= 0 || K <= 0 ? 0 : (size_t)(2 * bf16x3_ld(K) * N * 2);
+}
+int msdp_k_panel_apply(void* stream, int32_t rows, int32_t cols, const double* B, int64_t ldb,
+                       const double* v, double* out, int32_t accumulate, double* work) {
+  return panel_apply((hipStream_t)stream, rows, cols, B, ldb, v, out, accumulate != 0, work);
+}
+size_t msdp_k_panel_apply_work(int32_t rows, int32_t cols) {
+  return rows <= 0 || cols <= 0 ? 0 : panel_apply_work_elems(rows, cols);
 }
 int msdp_k_schur_onehot(void* stream, int32_t m, const double* L, int64_t ldl, const double* R, int64_t ldr,
                         double* H, int64_t ldh) {

@@ -11,7 +11,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "libmadsdp_hip.so"
-SOURCES = ["dense.hip", "gemm32.hip", "eigen.hip", "ops.hip", "qr.hip", "ldlt.hip", "solver.hip",
+SOURCES = ["dense.hip", "gemm32.hip", "bf16x3.hip", "eigen.hip", "ops.hip", "qr.hip", "ldlt.hip", "solver.hip",
            "server.hip", "profile.hip"]
 ARCH = os.environ.get("MSDP_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

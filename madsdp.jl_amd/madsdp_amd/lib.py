@@ -137,6 +137,11 @@ SIGNATURES = {
     "msdp_k_eig_min2": (C.c_int, [P, C.c_int32, P, P, C.c_int64, P, P, P]),
     "msdp_k_gemm32": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                 P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
+    "msdp_k_syrk_bf16x3": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, P, C.c_int64, C.c_double,
+                                     P, C.c_int64, P]),
+    "msdp_k_syrk_bf16x3_work_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "msdp_k_panel_apply": (C.c_int, [P, C.c_int32, C.c_int32, P, C.c_int64, P, P, C.c_int32, P]),
+    "msdp_k_panel_apply_work": (C.c_size_t, [C.c_int32, C.c_int32]),
     "msdp_k_schur_onehot": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, C.c_int64]),
 }
 

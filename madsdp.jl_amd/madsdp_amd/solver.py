@@ -129,11 +129,13 @@ class DualCentering:
 
 OPTION_FIELDS = ("tol", "max_iter", "max_wall_time", "mu_min", "regularize_schur", "max_schur_reg",
                  "mu_switch", "refine_max_iter", "lowp_min_gflop")
-# Schur assembly precision of the dense constraint blocks: "fp64" (plain) or
+# Schur assembly precision of the dense constraint blocks: "fp64" (plain),
 # "mixed" (FP32 MFMA assembly while mu >= mu_switch, FP64 Cholesky and
 # iterative refinement against the FP64 operator; FP64 below mu_switch, and
-# throughout when one dense assembly costs less than lowp_min_gflop GFLOP).
-PRECISIONS = {"fp64": 0, "mixed": 1}
+# throughout when one dense assembly costs less than lowp_min_gflop GFLOP) or
+# "bf16x3" (the same with H = B'B formed from the FP64 scaled panel B split
+# into bf16 hi + lo by three bf16 MFMA products, refined through B).
+PRECISIONS = {"fp64": 0, "mixed": 1, "bf16x3": 2}
 
 X_, S_, Y_, XLIN, ZLIN, DELX, DELS, DELY, DELXLIN, DELZLIN, RD, RP, RDLIN, W_, G_, SI, CORR, LS, LX, H_, D_ = range(21)
 

@@ -5,10 +5,10 @@ tests/golden/make_config_traces.py).
 * max-cut G(3000, 0.01), plain FP64: same status and iteration count, every
   per-iteration objective and residual within 1e-9 relative.
 * dense 250-block + 400 scalars, m = 8000, and 96 blocks of 8..120 + 200
-  scalars, m = 6000: the same for plain FP64; the mixed path (FP32 MFMA
-  assembly, FP64 refinement) reaches the same status in at most 2 extra
-  iterations, its final objective, residuals and y are stated against the
-  oracle's.
+  scalars, m = 6000: the same for plain FP64; the mixed paths (FP32 MFMA or
+  BF16x3 split-product assembly, FP64 refinement) reach the same status in at
+  most 2 extra iterations, their final objective, residuals and y are stated
+  against the oracle's.
 * the first 24 requests of the serving stream: the plain FP64 path matches
   the oracle per request (status, iterations, trace); served concurrently
   (continuous batching) each result equals the same instance solved alone
@@ -60,15 +60,15 @@ def check_plain(model, z):
     return s
 
 
-def check_mixed(model, z, **kw):
-    s = M.MadSDPSolver(model, precision="mixed", **kw).solve()
+def check_mixed(model, z, precision="mixed", **kw):
+    s = M.MadSDPSolver(model, precision=precision, **kw).solve()
     info = s.info()
     ref = z["trace"]
     tr = s.trace()
     e_obj = abs(tr[-1, 0] - ref[-1, 0]) / max(1.0, abs(ref[-1, 0]))
     e_y = np.abs(s.y - z["y"]).max() / max(1.0, np.abs(z["y"]).max())
-    print(f"mixed: status {int(s.status)}, {s.iter} it (oracle {int(z['iter'])}), "
-          f"FP32 assemblies {info.lowp_factorizations}, refinement steps {info.refine_steps}, "
+    print(f"{precision}: status {int(s.status)}, {s.iter} it (oracle {int(z['iter'])}), "
+          f"low-precision assemblies {info.lowp_factorizations}, refinement steps {info.refine_steps}, "
           f"FP64 fallbacks {info.fp64_fallbacks}; vs oracle: objective {e_obj:.2e}, "
           f"inf_pr {tr[-1, 2]:.2e} (oracle {ref[-1, 2]:.2e}), inf_du {tr[-1, 3]:.2e} "
           f"(oracle {ref[-1, 3]:.2e}), gap {tr[-1, 4]:.2e}, y {e_y:.2e}", flush=True)
@@ -101,6 +101,11 @@ def test_dense_250_mixed(gpu, dense_250):
     check_mixed(dense_250, _load("dense_250"))
 
 
+def test_dense_250_bf16x3(gpu, dense_250):
+    print("dense_250 bf16x3", flush=True)
+    check_mixed(dense_250, _load("dense_250"), precision="bf16x3")
+
+
 @pytest.fixture(scope="module")
 def blocks_96():
     return instances.blocks_96_m6000()
@@ -114,6 +119,13 @@ def test_blocks_96_plain(gpu, blocks_96):
 def test_blocks_96_mixed(gpu, blocks_96):
     print("blocks_96 mixed (FP32 assembly forced)", flush=True)
     check_mixed(blocks_96, _load("blocks_96"), lowp_min_gflop=0.0)
+
+
+def test_blocks_96_bf16x3(gpu, blocks_96):
+    """Constraints touching a subset of the blocks: the panel operator gathers
+    and scatters by the block's constraint list."""
+    print("blocks_96 bf16x3 (forced)", flush=True)
+    check_mixed(blocks_96, _load("blocks_96"), precision="bf16x3", lowp_min_gflop=0.0)
 
 
 def test_stream_prefix(gpu):

@@ -79,6 +79,67 @@ def test_syrk_lower(env, trans, N, K):
     assert np.array_equal(got[up], Cm[up])
 
 
+def _bf16(x):
+    """Round-to-nearest-even of float32(x) to bfloat16, as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("N,K", [(1, 1), (129, 31), (300, 1000), (700, 2500)])
+def test_syrk_bf16x3(env, N, K):
+    """C = alpha (hi'hi + hi'lo + lo'hi) + beta C on the lower triangle, the
+    upper triangle untouched.  Against the same three products formed in
+    FP64 from a numpy emulation of the split (hi = bf16(a), lo = bf16(a -
+    hi)) the only difference is the FP32 accumulation (a transposed operand
+    or a wrong lane would be off by O(1)); against A'A in FP64 the error is
+    the split's (about 2^-17 per entry)."""
+    torch, lib, st = env
+    rng = np.random.default_rng(N * 3 + K)
+    A = rng.standard_normal((K, N)) * np.exp(rng.uniform(-3, 3, size=(1, N)))
+    Cm = rng.standard_normal((N, N))
+    dA, dC = dev(torch, A), dev(torch, Cm)
+    work = torch.zeros(int(lib.msdp_k_syrk_bf16x3_work_bytes(N, K)) // 8 + 1, dtype=torch.float64,
+                       device="cuda")
+    print(f"syrk_bf16x3 N={N} K={K}", flush=True)
+    assert lib.msdp_k_syrk_bf16x3(st, N, K, 0.5, P(dA), K, 2.0, P(dC), N, P(work)) == 0
+    torch.cuda.synchronize()
+    got = host(dC, (N, N))
+    hi = _bf16(A)
+    lo = _bf16(A - hi)
+    emul = 0.5 * (hi.T @ hi + hi.T @ lo + lo.T @ hi) + 2.0 * Cm
+    scale = 0.5 * np.abs(hi).T @ np.abs(hi) + 2.0 * np.abs(Cm)
+    low = np.tril_indices(N)
+    # FP32 accumulation of K products (and the FP64 widening of alpha * acc)
+    assert np.all(np.abs(got - emul)[low] <= (4 * K + 8) * 2.0 ** -24 * scale[low])
+    exact = 0.5 * A.T @ A + 2.0 * Cm
+    err = np.abs(got - exact)[low] / scale[low]
+    print(f"  max error vs FP64 A'A: {err.max():.2e} (relative to |A|'|A|)", flush=True)
+    assert err.max() <= 2.0 ** -14
+    up = np.triu_indices(N, 1)
+    assert np.array_equal(got[up], Cm[up])
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (300, 77), (4000, 1100)])
+@pytest.mark.parametrize("acc", [0, 1])
+def test_panel_apply(env, rows, cols, acc):
+    """out (+)= B'(B v) in FP64 (the refinement's operator of a BF16x3 Schur
+    complement)."""
+    torch, lib, st = env
+    rng = np.random.default_rng(rows + cols + acc)
+    B = rng.standard_normal((rows, cols))
+    v = rng.standard_normal(cols)
+    o = rng.standard_normal(cols)
+    dB, dv, do = dev(torch, B), dev(torch, v), dev(torch, o)
+    work = torch.zeros(int(lib.msdp_k_panel_apply_work(rows, cols)), dtype=torch.float64, device="cuda")
+    print(f"panel_apply rows={rows} cols={cols} acc={acc}", flush=True)
+    assert lib.msdp_k_panel_apply(st, rows, cols, P(dB), rows, P(dv), P(do), acc, P(work)) == 0
+    torch.cuda.synchronize()
+    ref = B.T @ (B @ v) + (o if acc else 0.0)
+    scale = np.abs(B).T @ (np.abs(B) @ np.abs(v)) + np.abs(o)
+    assert np.all(np.abs(do.cpu().numpy() - ref) <= 1e-13 * (rows + cols) * scale)
+
+
 @pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 300])
 def test_cholesky_factor(env, n):
     torch, lib, st = env

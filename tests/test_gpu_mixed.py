@@ -1,7 +1,7 @@
-"""Mixed-precision Schur path: FP32 MFMA assembly of the dense constraint
-blocks while mu >= mu_switch, FP64 Cholesky, and iterative refinement of every
-Schur solve against the FP64 operator (falling back to an FP64 assembly when
-the refinement stalls).
+"""Mixed-precision Schur paths: FP32 MFMA ("mixed") or BF16x3 split-product
+("bf16x3") assembly of the dense constraint blocks while mu >= mu_switch,
+FP64 Cholesky, and iterative refinement of every Schur solve against the FP64
+operator (falling back to an FP64 assembly when the refinement stalls).
 
 The bar: the same termination status and tolerance as the plain FP64 path, in
 at most two more iterations, and the same optimum to the tolerance.
@@ -49,9 +49,12 @@ def test_gemm32(env, ta, tb, Mm, N, K):
     assert np.all(np.abs(got - ref) <= 4 * np.finfo(np.float32).eps * np.sqrt(K) * scale)
 
 
-def _pair(model, **kw):
+LOWP = ["mixed", "bf16x3"]
+
+
+def _pair(model, precision="mixed", **kw):
     a = M.MadSDPSolver(model, **kw).solve()
-    b = M.MadSDPSolver(model, precision="mixed", lowp_min_gflop=0.0, **kw).solve()
+    b = M.MadSDPSolver(model, precision=precision, lowp_min_gflop=0.0, **kw).solve()
     return a, b
 
 
@@ -68,26 +71,29 @@ def _check(a, b, tol=1e-7):
     assert inf.lowp_factorizations > 0
 
 
+@pytest.mark.parametrize("precision", LOWP)
 @pytest.mark.parametrize("direction", [M.NT(), M.HKM()])
-def test_mixed_dense(gpu, direction):
-    a, b = _pair(instances.dense_sdp(40, 300, 20, 5), direction=direction)
+def test_mixed_dense(gpu, direction, precision):
+    a, b = _pair(instances.dense_sdp(40, 300, 20, 5), precision, direction=direction)
     print(f"fp64 {a.iter} it, mixed {b.iter} it, lowp {b.info().lowp_factorizations}, "
           f"refine {b.info().refine_steps}, fallbacks {b.info().fp64_fallbacks}", flush=True)
     _check(a, b)
 
 
-def test_mixed_blocks(gpu):
+@pytest.mark.parametrize("precision", LOWP)
+def test_mixed_blocks(gpu, precision):
     model = instances.block_sdp(instances.block_sizes(8, 4, 90, 6), 400, 12, 7)
-    a, b = _pair(model)
+    a, b = _pair(model, precision)
     print(f"fp64 {a.iter} it, mixed {b.iter} it, lowp {b.info().lowp_factorizations}, "
           f"refine {b.info().refine_steps}, fallbacks {b.info().fp64_fallbacks}", flush=True)
     _check(a, b)
 
 
-def test_mixed_backtrack_adaptive(gpu):
+@pytest.mark.parametrize("precision", LOWP)
+def test_mixed_backtrack_adaptive(gpu, precision):
     model = instances.dense_sdp(30, 150, 0, 9)
     kw = dict(step=M.CholeskyBacktrack(), centering=M.AdaptiveMu())
-    a, b = _pair(model, **kw)
+    a, b = _pair(model, precision, **kw)
     _check(a, b)
 
 
@@ -117,16 +123,18 @@ def test_precision_option_checked():
         M.MadSDPSolver(instances.maxcut(5, 0.5, 1), precision="fp16")
 
 
-def test_mixed_dense_m1100(gpu):
+@pytest.mark.parametrize("precision", LOWP)
+def test_mixed_dense_m1100(gpu, precision):
     """m = 1100 (at most n(n+1)/2 + n_s, so H has full rank): the refined
     solves apply the inverses of the factor's full 512-row diagonal blocks."""
-    a, b = _pair(instances.dense_sdp(48, 1100, 12, 13))
+    a, b = _pair(instances.dense_sdp(48, 1100, 12, 13), precision)
     print(f"fp64 {a.iter} it, mixed {b.iter} it, lowp {b.info().lowp_factorizations}, "
           f"refine {b.info().refine_steps}, fallbacks {b.info().fp64_fallbacks}", flush=True)
     _check(a, b)
 
 
-def test_mixed_error_vs_oracle(gpu):
+@pytest.mark.parametrize("precision", LOWP)
+def test_mixed_error_vs_oracle(gpu, precision):
     """The mixed path's end point against the FP64 oracle's solve of the same
     instance: objective, primal and dual residuals, and X, y, S.  The errors
     are printed (and stated in the design notes); the bounds hold at the 1e-7
@@ -134,7 +142,7 @@ def test_mixed_error_vs_oracle(gpu):
     from oracle import madsdp_oracle as O
     model = instances.dense_sdp(30, 200, 30, 4)
     ref = O.Solver(model).solve()
-    b = M.MadSDPSolver(model, precision="mixed", lowp_min_gflop=0.0).solve()
+    b = M.MadSDPSolver(model, precision=precision, lowp_min_gflop=0.0).solve()
     assert ref.status == O.Status.SOLVE_SUCCEEDED and b.status == M.Status.SOLVE_SUCCEEDED
     assert b.info().lowp_factorizations > 0
     tr = b.trace()
