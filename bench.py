@@ -134,7 +134,9 @@ def run_stream(args):
 
     def one(ms, arr, slots, precision, profile=False):
         t = time.perf_counter()
-        solvers = [MadSDPSolver(model, device=0, precision=precision) for model in ms]
+        # concurrent solves share the device: one launch per product
+        solvers = [MadSDPSolver(model, device=0, precision=precision, shared_device=int(slots > 1))
+                   for model in ms]
         prep = time.perf_counter() - t
         lib = L.load()
         if profile:

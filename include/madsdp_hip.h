@@ -86,6 +86,13 @@ typedef struct msdp_options {
    * assembly saves and the solve stays FP64; it also stays FP64 when m
    * exceeds the sum of the blocks' n(n+1)/2 and n_s (a singular H) */
   double lowp_min_gflop;           /* 500 */
+  /* 1: the solve shares the device with other solves (a serving loop): every
+   * product is one launch (no split-K or stream-K partition of a product over
+   * the whole chip, which costs extra launches and pool allocations that
+   * concurrent solves pay for in dispatch contention).  Results differ from
+   * the shared_device = 0 path only in the summation order of those products.
+   * Not in the reference: an extension of this library. */
+  int32_t shared_device;           /* 0 */
 } msdp_options;
 
 /* iteration data of MadSDPSolver (src/solver.jl:37-48) */

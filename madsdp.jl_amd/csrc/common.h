@@ -38,6 +38,9 @@ int fail_hip(hipError_t e, const char* what, const char* file, int line);
 // blocking-sync event instead, leaving its core to other solves' threads.
 int stream_sync(hipStream_t s);
 void set_blocking_sync(bool on);
+// The calling thread's products share the device with other solves: each
+// product is one launch (no split-K / stream-K partition over the chip).
+bool& shared_device();
 #define MSDP_HIP(x)                                                      \
   do {                                                                   \
     hipError_t e_ = (x);                                                 \

@@ -20,6 +20,11 @@ namespace msdp {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
+bool& shared_device() {
+  static thread_local bool on = false;
+  return on;
+}
+
 Recorder*& rec_active() {
   static thread_local Recorder* r = nullptr;
   return r;
@@ -648,7 +653,9 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   }
   const bool prof = prof_active(PROF_GEMM, s);
   SkGrid sg;
-  if (stream_k_shape(M, N, K, batch, lower, guard, tune(T_SK_OCC) * device_cus(), sg)) {
+  // a device shared with other solves: one launch per product
+  const bool whole_chip = !shared_device();
+  if (whole_chip && stream_k_shape(M, N, K, batch, lower, guard, tune(T_SK_OCC) * device_cus(), sg)) {
     if (prof) prof_before(s);
     MSDP_TRY(gemm_stream_k(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, ws,
                            ws_elems));
@@ -656,7 +663,7 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     return 0;
   }
   const int64_t tiles64 = (int64_t)cdiv(M, GT) * cdiv(N, GT) * batch;
-  if (!guard && batch == 1 && tiles64 < 128 && K >= tune(T_SPLITK_MIN_K)) {
+  if (whole_chip && !guard && batch == 1 && tiles64 < 128 && K >= tune(T_SPLITK_MIN_K)) {
     // split K so that about splitk_workgroups workgroups run, chunks of at
     // least SPLITK_MIN_CHUNK
     int parts = (int)std::min<int64_t>(std::max<int64_t>(2, tune(T_SPLITK_WGS) / tiles64),

@@ -2076,9 +2076,18 @@ struct RecScope {
     }
   }
 };
+// the solver's shared_device option for the calling thread, for the call
+struct SharedScope {
+  bool prev;
+  explicit SharedScope(const msdp_solver* s) : prev(shared_device()) {
+    shared_device() = s->opt.shared_device != 0;
+  }
+  ~SharedScope() { shared_device() = prev; }
+};
 #define GUARD0(s) \
   if (!(s)) return as_err("null solver"); \
-  RecScope rec_scope_{const_cast<msdp_solver*>(s)};
+  RecScope rec_scope_{const_cast<msdp_solver*>(s)}; \
+  SharedScope shared_scope_{s};
 #define GUARD(s) \
   GUARD0(s)      \
   if ((s)->released) return as_err("solver released: its device state was freed after the solve");

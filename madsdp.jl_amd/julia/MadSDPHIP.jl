@@ -52,6 +52,7 @@ mutable struct Options
     dc_beta::Cdouble; dc_delta::Cdouble; dc_A::Cdouble; dc_centering_max_iter::Int32
     precision::Int32; mu_switch::Cdouble; refine_max_iter::Int32
     lowp_min_gflop::Cdouble
+    shared_device::Int32
     Options() = new()
 end
 

@@ -32,7 +32,7 @@ class Options(C.Structure):
         ("dc_beta", C.c_double), ("dc_delta", C.c_double), ("dc_A", C.c_double),
         ("dc_centering_max_iter", C.c_int32),
         ("precision", C.c_int32), ("mu_switch", C.c_double), ("refine_max_iter", C.c_int32),
-        ("lowp_min_gflop", C.c_double),
+        ("lowp_min_gflop", C.c_double), ("shared_device", C.c_int32),
     ]
 
 

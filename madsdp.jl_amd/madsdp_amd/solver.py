@@ -128,7 +128,7 @@ class DualCentering:
 
 
 OPTION_FIELDS = ("tol", "max_iter", "max_wall_time", "mu_min", "regularize_schur", "max_schur_reg",
-                 "mu_switch", "refine_max_iter", "lowp_min_gflop")
+                 "mu_switch", "refine_max_iter", "lowp_min_gflop", "shared_device")
 # Schur assembly precision of the dense constraint blocks: "fp64" (plain),
 # "mixed" (FP32 MFMA assembly while mu >= mu_switch, FP64 Cholesky and
 # iterative refinement against the FP64 operator; FP64 below mu_switch, and

@@ -38,6 +38,7 @@ def test_options_struct_defaults():
     lib.msdp_default_options(ctypes.byref(opt))
     assert opt.tol == 1e-7 and opt.max_iter == 100 and opt.regularize_schur == 1e-12
     assert opt.max_schur_reg == 8 and opt.tau == 0.95 and opt.dc_A == 2.0
+    assert opt.precision == 0 and opt.shared_device == 0
 
 
 def test_tuning_parameters():

@@ -24,7 +24,11 @@ def mix():
         (instances.block_sdp(instances.block_sizes(5, 4, 40, 6), 70, 6, 7), {}),
         (instances.dense_sdp(16, 50, 16, 8), dict(step=M.CholeskyBacktrack(), centering=M.AdaptiveMu())),
         (instances.maxcut(300, 0.03, 9), {}),
+        (instances.maxcut(400, 0.02, 12), dict(shared_device=1)),
+        (instances.dense_sdp(120, 1500, 120, 13), dict(shared_device=1)),
         (instances.dense_sdp(60, 400, 0, 10), dict(precision="mixed", lowp_min_gflop=0.0)),
+        (instances.dense_sdp(40, 300, 20, 14), dict(precision="bf16x3", lowp_min_gflop=0.0,
+                                                    shared_device=1)),
         (instances.maxcut(25, 0.3, 11), dict(algorithm=M.DualCentering())),
     ]
 
@@ -59,9 +63,14 @@ def test_same_alone_and_in_mix(gpu):
 
 
 def test_served_matches_oracle(gpu):
-    models = [instances.maxcut(60, 0.15, 21), instances.dense_sdp(24, 80, 12, 22)]
-    solvers = [M.MadSDPSolver(m) for m in models]
-    M.serve(solvers, [0.0, 0.0], slots=2)
+    """Served solves against the FP64 oracle per iteration, among them one
+    whose products are single launches on the shared device (n = 300: the
+    n x n products would otherwise be split over K)."""
+    models = [instances.maxcut(60, 0.15, 21), instances.dense_sdp(24, 80, 12, 22),
+              instances.maxcut(300, 0.03, 23)]
+    kws = [{}, {}, dict(shared_device=1)]
+    solvers = [M.MadSDPSolver(m, **kw) for m, kw in zip(models, kws)]
+    M.serve(solvers, [0.0] * len(models), slots=3)
     for m, s in zip(models, solvers):
         ref = O.Solver(m).solve()
         assert s.status == M.Status.SOLVE_SUCCEEDED and s.iter == ref.iter
