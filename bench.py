@@ -181,8 +181,17 @@ def run_stream(args):
     if prof and prof[1] > 0 and prof[0] > 0:
         ms_, n_, w_ = prof
         achieved = w_ / (ms_ * 1e-3) / 1e12
+        # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+        # this kernel (a plain one-stream serving of the stream's first requests:
+        # counter passes need one stream)
+        traffic, tsample = None, None
+        tf = os.path.join(ROOT, "profiles", "trd1_traffic_stream.json")
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                tj = json.load(fh)
+            traffic, tsample = tj["traffic_bytes_per_launch"], tj["sample"]
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAKS["mfma"], "unit": "TFLOP/s",
-                "frac": achieved / PEAKS["mfma"], "traffic": None,
+                "frac": achieved / PEAKS["mfma"], "traffic": traffic, "traffic_sample": tsample,
                 "kernel": "trd1_kernel (one-workgroup Householder tridiagonalisation, n <= 640: "
                           "the NT eigenproblem and the step-length eigenvalues of the stream's "
                           "blocks); work = (4/3) n^3 flops per matrix, FP64 vector peak",
