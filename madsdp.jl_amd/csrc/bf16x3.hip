@@ -69,13 +69,55 @@ int bf16_split(hipStream_t s, int rows, int cols, const double* B, int64_t ldb, 
 // while the current one is multiplied.
 constexpr int SK_T = 128, SK_KS = 32, SK_LD = 40;
 
+// Lower tiles of an nt x nt tile grid for a 1-D launch of lower_tile_blocks(nt)
+// workgroups, grouped so that the tiles one XCD runs together share their
+// panels: the tiles are enumerated by 8 x 8 super-tiles (column-major lower
+// order of super-tiles, column-major inside each, the diagonal super-tiles'
+// lower half only) and each XCD (blockIdx % 8) takes a contiguous range.  The
+// 64 workgroups an XCD holds at once then cover about one super-tile, which
+// reads 16 row / column panels instead of the 65 of a column run.
+constexpr int SK_SUPER = 8;
+__device__ inline bool lower_tile_super(int nt, int& tm, int& tn) {
+  const int T = nt * (nt + 1) / 2;
+  const int per = (T + 7) / 8;
+  const int w = blockIdx.x;
+  int t = (w & 7) * per + (w >> 3);
+  if (t >= T) return false;
+  const int ns = (nt + SK_SUPER - 1) / SK_SUPER;
+  for (int TN = 0; TN < ns; ++TN) {
+    const int b1 = min(SK_SUPER, nt - TN * SK_SUPER);  // tile columns in this super-column
+    for (int TM = TN; TM < ns; ++TM) {
+      const int a1 = min(SK_SUPER, nt - TM * SK_SUPER);
+      const int cnt = TM == TN ? a1 * (a1 + 1) / 2 : a1 * b1;
+      if (t >= cnt) {
+        t -= cnt;
+        continue;
+      }
+      if (TM != TN) {
+        tn = TN * SK_SUPER + t / a1;
+        tm = TM * SK_SUPER + t % a1;
+      } else {  // lower half of a diagonal super-tile, column by column
+        int b = 0;
+        while (t >= a1 - b) {
+          t -= a1 - b;
+          ++b;
+        }
+        tn = TN * SK_SUPER + b;
+        tm = TN * SK_SUPER + b + t;
+      }
+      return true;
+    }
+  }
+  return false;
+}
+
 __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(int M, int K, double alpha,
                                                              const __bf16* __restrict__ Ah,
                                                              const __bf16* __restrict__ Al,
                                                              int64_t lda, double beta,
                                                              double* __restrict__ C, int64_t ldc) {
   int tm, tn;
-  if (!lower_tile_xcd((M + SK_T - 1) / SK_T, tm, tn)) return;
+  if (!lower_tile_super((M + SK_T - 1) / SK_T, tm, tn)) return;
   // [buffer][hi, lo][column][k]: the tile's row panel (I) and column panel (J)
   __shared__ __attribute__((aligned(16))) __bf16 sI[2][2][SK_T * SK_LD];
   __shared__ __attribute__((aligned(16))) __bf16 sJ[2][2][SK_T * SK_LD];

@@ -975,9 +975,53 @@ __global__ __launch_bounds__(64) void chol_panel_kernel(int rows, double* A21, i
 size_t potrf_work_elems(int n) { return (size_t)cdiv(std::max(n, 1), 64) * 4096; }
 
 // count (1 or 2) matrices: the second at A + sA, its info at info[1]
+// Right-looking blocked Cholesky in 256-column panels for large n: inside a
+// panel the 64-column steps (diagonal block, rows below by substitution)
+// update only the panel's own later columns; the rest of the matrix takes one
+// rank-256 update per panel.  A rank-64 update per step re-reads and rewrites
+// the whole trailing triangle four times as often with a quarter of the
+// depth (measured 0.23 of the FP64 MFMA peak at n = 8000).
+constexpr int POTRF_NB = 256;
+constexpr int POTRF_BLOCKED_MIN = 4096;
+
+static int potrf_blocked(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
+                         int* info) {
+  for (int J0 = 0; J0 < n; J0 += POTRF_NB) {
+    const int J1 = std::min(n, J0 + POTRF_NB);
+    for (int j0 = J0; j0 < J1; j0 += 64) {
+      const int jb = std::min(64, J1 - j0);
+      double* A11 = A + j0 + (int64_t)j0 * lda;
+      chol_diag_kernel<<<count, 64, 0, s>>>(jb, A11, lda, sA, info, j0, 1, nullptr);
+      MSDP_HIP(hipGetLastError());
+      const int rest = n - j0 - jb;
+      if (rest <= 0) continue;  // jb == 64 below
+      double* A21 = A11 + jb;
+      chol_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(rest, A21, lda, A11, info, jb,
+                                                                   sA);
+      MSDP_HIP(hipGetLastError());
+      const int w = J1 - j0 - jb;  // the panel's later columns
+      if (w <= 0) continue;
+      double* A22 = A21 + (int64_t)jb * lda;
+      // the panel's square part (lower) and the rows below it
+      MSDP_TRY(gemm_guarded(s, false, true, w, w, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
+                            count, sA, sA, sA, true, info));
+      if (n > J1)
+        MSDP_TRY(gemm_guarded(s, false, true, n - J1, w, jb, -1.0, A21 + w, lda, A21, lda, 1.0,
+                              A22 + w, lda, count, sA, sA, sA, false, info));
+    }
+    if (n > J1) {
+      double* P = A + J1 + (int64_t)J0 * lda;
+      MSDP_TRY(gemm_guarded(s, false, true, n - J1, n - J1, J1 - J0, -1.0, P, lda, P, lda, 1.0,
+                            A + J1 + (int64_t)J1 * lda, lda, count, sA, sA, sA, true, info));
+    }
+  }
+  return 0;
+}
+
 static int potrf_count(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
                        int* info) {
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * count, s));
+  if (n >= POTRF_BLOCKED_MIN && !shared_device()) return potrf_blocked(s, n, A, lda, sA, count, info);
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int jb = std::min(64, n - j0);
     double* A11 = A + j0 + (int64_t)j0 * lda;
