@@ -1194,8 +1194,12 @@ __device__ void syev_small(const EigBatch& it) {
           continue;
         }
         if (++iter > 60) { state = 3; break; }
+        // sqrt of the sum of squares in place of hypot, and one reciprocal per
+        // rotation in place of two quotients: the chase below is a dependent
+        // chain, and these are its longest latencies (the operands are matrix
+        // entries far inside the range where squaring overflows or underflows)
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = hypot(g, 1.0);
+        double r = sqrt(fma(g, g, 1.0));
         g = d[m] - d[l] + e[l] / (g + copysign(r, g));
         double s = 1.0, c = 1.0, pp = 0.0;
         bool split = false;
@@ -1204,7 +1208,7 @@ __device__ void syev_small(const EigBatch& it) {
           double en = 0.0, dn = 0.0;
           if (i > l) { en = e[i - 1]; dn = d[i - 1]; }
           const double f = s * ei, b = c * ei;
-          r = hypot(f, g);
+          r = sqrt(fma(f, f, g * g));
           e[i + 1] = r;
           if (r == 0.0) {
             d[i + 1] = dip1 - pp;
@@ -1212,8 +1216,9 @@ __device__ void syev_small(const EigBatch& it) {
             split = true;
             break;
           }
-          s = f / r;
-          c = g / r;
+          const double ri = 1.0 / r;
+          s = f * ri;
+          c = g * ri;
           g = dip1 - pp;
           r = (di - g) * s + 2.0 * c * b;
           pp = s * r;
