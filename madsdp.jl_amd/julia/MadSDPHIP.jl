@@ -81,7 +81,9 @@ setarr!(h, which, block, src) =
 
 Schur complement on HIP device `device`.  `precision = :mixed` assembles large
 dense constraint blocks in FP32 on MFMA while μ ≥ `mu_switch` and refines
-every Schur solve against the FP64 operator.
+every Schur solve against the FP64 operator; `precision = :bf16x3` forms the
+same blocks' H = B'B from bf16 hi/lo splits of the FP64 scaled panel B and
+refines through B.
 """
 struct HIPSchur <: MadSDP.AbstractSchurSolver
     device::Int
@@ -105,7 +107,8 @@ end
 
 function MadSDP.schur_workspace(s::HIPSchur, ::Type{T}, m) where {T}
     T == Float64 || throw(ArgumentError("HIPSchur requires Float64 data."))
-    ws = HIPWorkspace{T}(C_NULL, s.device, s.precision == :mixed ? Int32(1) : Int32(0))
+    prec = s.precision == :mixed ? Int32(1) : s.precision == :bf16x3 ? Int32(2) : Int32(0)
+    ws = HIPWorkspace{T}(C_NULL, s.device, prec)
     finalizer(ws) do w
         w.handle == C_NULL || ccall((:msdp_solver_destroy, libmsdp), Cint, (Ptr{Cvoid},), w.handle)
     end
