@@ -1178,7 +1178,12 @@ struct TrtriLevel {
 template <int STAGE>
 __global__ __launch_bounds__(256) void trtri_level_kernel(TrtriLevel lv, const double* L,
                                                           int64_t ldl, double* Li, int64_t ldi,
-                                                          double* work) {
+                                                          double* work, int64_t sL, int64_t sLi,
+                                                          int64_t sW) {
+  // matrix blockIdx.y of a batch at the given strides
+  L += blockIdx.y * sL;
+  Li += blockIdx.y * sLi;
+  work += blockIdx.y * sW;
   int lo = 0, hi = lv.count - 1;
   const int w = blockIdx.x;
   while (lo < hi) {
@@ -1211,7 +1216,8 @@ static void trtri_tree(int lo, int hi, int depth, std::vector<std::vector<std::a
 
 // The recursion of trtri_rec for one matrix, depth by depth from the leaves.
 static int trtri_levels(hipStream_t s, int n, const double* L, int64_t ldl, double* Li,
-                        int64_t ldi, double* work) {
+                        int64_t ldi, double* work, int count = 1, int64_t sL = 0,
+                        int64_t sLi = 0, int64_t sW = 0) {
   std::vector<std::vector<std::array<int, 3>>> lv;
   trtri_tree(0, n, 0, lv);
   for (int dpt = (int)lv.size() - 1; dpt >= 0; --dpt) {
@@ -1223,9 +1229,9 @@ static int trtri_levels(hipStream_t s, int n, const double* L, int64_t ldl, doub
         const int lo = nd[0], mid = nd[1], hi = nd[2];
         const int r = hi - mid, c = mid - lo;
         MSDP_TRY(gemm(s, false, false, r, c, c, 1.0, L + mid + (int64_t)lo * ldl, ldl,
-                      Li + lo + (int64_t)lo * ldi, ldi, 0.0, work, r));
+                      Li + lo + (int64_t)lo * ldi, ldi, 0.0, work, r, count, sL, sLi, sW));
         MSDP_TRY(gemm(s, false, false, r, c, r, -1.0, Li + mid + (int64_t)mid * ldi, ldi, work, r,
-                      0.0, Li + mid + (int64_t)lo * ldi, ldi));
+                      0.0, Li + mid + (int64_t)lo * ldi, ldi, count, sLi, sW, sLi));
       }
       continue;
     }
@@ -1244,8 +1250,8 @@ static int trtri_levels(hipStream_t s, int n, const double* L, int64_t ldl, doub
       woff += (int64_t)r * c;
     }
     p.start[p.count] = tiles;
-    trtri_level_kernel<1><<<tiles, 256, 0, s>>>(p, L, ldl, Li, ldi, work);
-    trtri_level_kernel<2><<<tiles, 256, 0, s>>>(p, L, ldl, Li, ldi, work);
+    trtri_level_kernel<1><<<dim3(tiles, count), 256, 0, s>>>(p, L, ldl, Li, ldi, work, sL, sLi, sW);
+    trtri_level_kernel<2><<<dim3(tiles, count), 256, 0, s>>>(p, L, ldl, Li, ldi, work, sL, sLi, sW);
     MSDP_HIP(hipGetLastError());
   }
   return 0;
@@ -1304,7 +1310,8 @@ int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_
   MSDP_HIP(hipGetLastError());
   trtri_diag_kernel<<<dim3(cdiv(n, 64), count), 64, 0, s>>>(n, L, ldl, sL, Li, ldi, sLi);
   MSDP_HIP(hipGetLastError());
-  if (count == 1) return trtri_levels(s, n, L, ldl, Li, ldi, work);
+  // every depth of the recursion in one pair of launches for all matrices
+  if (count <= 65535) return trtri_levels(s, n, L, ldl, Li, ldi, work, count, sL, sLi, sW);
   return trtri_rec(s, 0, n, L, ldl, sL, Li, ldi, sLi, work, sW, count);
 }
 

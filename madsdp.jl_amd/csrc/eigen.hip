@@ -1724,6 +1724,40 @@ static DcPlan dc_plan(int n) {
 
 // workspace layout (doubles): Z (n^2), Qg (n^2), U (n^2), Qn (n^2), 12 n vectors
 static size_t dc_work_elems(int n) { return 4 * (size_t)n * n + 16 * (size_t)n + 64; }
+// The level's root maps from its per-node deflation counts: roots of node q
+// are rmap = q, rloc = 0..kp[q]-1, in node order; entries past the roots
+// (the padding to 4-wave blocks) are -1 / 0.  One workgroup.
+constexpr int DC_ROOTMAP_MAX = 1024;
+__global__ __launch_bounds__(256) void dc_rootmap_kernel(const DcNode* nodes, int nn, const int* kp,
+                                                         int padded, int* rmap, int* rloc) {
+  __shared__ int off[DC_ROOTMAP_MAX + 1];
+  (void)nodes;
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int q = 0; q < nn; ++q) {
+      off[q] = acc;
+      acc += kp[q];
+    }
+    off[nn] = acc;
+  }
+  __syncthreads();
+  const int nroots = off[nn];
+  for (int r = threadIdx.x; r < padded; r += 256) {
+    if (r >= nroots) {
+      rmap[r] = -1;
+      rloc[r] = 0;
+      continue;
+    }
+    int lo = 0, hi = nn - 1;  // last q with off[q] <= r
+    while (lo < hi) {
+      const int m = (lo + hi + 1) >> 1;
+      if (off[m] <= r) lo = m; else hi = m - 1;
+    }
+    rmap[r] = lo;
+    rloc[r] = r - off[lo];
+  }
+}
+
 static size_t dc_iwork_elems(int n) { return 24 * (size_t)n + 64; }
 
 // Eigen-decomposition of the symmetric tridiagonal (d, e): lam ascending,
@@ -1752,48 +1786,60 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
   MSDP_TRY(fill(s, (int64_t)n * n, 0.0, Z));
   const int nleaf = (int)P.leaf_lo.size();
   const int nsplit = (int)P.splits.size();
+  // the plan's static data -- leaves, splits and every level's nodes -- goes
+  // up at once; the host vectors live until the first level's read of the
+  // deflation counts has synchronised the stream, so no other wait is needed
   std::vector<int> hostbuf(2 * nleaf);
   for (int i = 0; i < nleaf; ++i) { hostbuf[i] = P.leaf_lo[i]; hostbuf[nleaf + i] = P.leaf_hi[i]; }
-  MSDP_HIP(hipMemcpyAsync(leafbuf, hostbuf.data(), sizeof(int) * 2 * nleaf, hipMemcpyHostToDevice, s));
-  if (nsplit > 0) {
-    MSDP_HIP(hipMemcpyAsync(splitbuf, P.splits.data(), sizeof(int) * nsplit, hipMemcpyHostToDevice, s));
-    dc_split_kernel<<<cdiv(nsplit, 256), 256, 0, s>>>(nsplit, splitbuf, e, d);
+  std::vector<DcNode> allnodes;
+  std::vector<int> level_off;
+  for (const auto& level : P.levels) {
+    level_off.push_back((int)allnodes.size());
+    allnodes.insert(allnodes.end(), level.begin(), level.end());
   }
+  if ((allnodes.size() * sizeof(DcNode) + sizeof(int) - 1) / sizeof(int) > 8 * (size_t)n + 64) {
+    set_error("stedc: too many merge nodes for the workspace");
+    return -2;
+  }
+  MSDP_HIP(hipMemcpyAsync(leafbuf, hostbuf.data(), sizeof(int) * 2 * nleaf, hipMemcpyHostToDevice, s));
+  if (nsplit > 0)
+    MSDP_HIP(hipMemcpyAsync(splitbuf, P.splits.data(), sizeof(int) * nsplit, hipMemcpyHostToDevice, s));
+  if (!allnodes.empty())
+    MSDP_HIP(hipMemcpyAsync(nodes_dev, allnodes.data(), sizeof(DcNode) * allnodes.size(),
+                            hipMemcpyHostToDevice, s));
+  if (nsplit > 0) dc_split_kernel<<<cdiv(nsplit, 256), 256, 0, s>>>(nsplit, splitbuf, e, d);
   dc_leaf_kernel<<<nleaf, 256, 0, s>>>(leafbuf, leafbuf + nleaf, d, e, lam, Z, n);
   MSDP_HIP(hipGetLastError());
-  // the host copies below are stream-ordered from pageable memory; keep the
-  // staging vectors alive until the stream is synchronised at each level.
-  for (const auto& level : P.levels) {
+  for (size_t li = 0; li < P.levels.size(); ++li) {
+    const auto& level = P.levels[li];
     const int nn = (int)level.size();
-    MSDP_HIP(hipMemcpyAsync(nodes_dev, level.data(), sizeof(DcNode) * nn, hipMemcpyHostToDevice, s));
-    dc_deflate_kernel<<<nn, 1024, 0, s>>>(nodes_dev, lam, e, Z, n, v);
+    const DcNode* nodes = nodes_dev + level_off[li];
+    dc_deflate_kernel<<<nn, 1024, 0, s>>>(nodes, lam, e, Z, n, v);
     int maxK = 0;
     for (const auto& nd : level) maxK = std::max(maxK, nd.hi - nd.lo);
-    dc_gather_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes_dev, Z, n, v, Qg);
-    dc_rotate_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes_dev, v, Qg);
+    dc_gather_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes, Z, n, v, Qg);
+    dc_rotate_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes, v, Qg);
     MSDP_HIP(hipGetLastError());
+    // the deflation counts decide the launch sizes below: the one host wait
+    // of the level
     std::vector<int> kp(nn);
     MSDP_HIP(hipMemcpyAsync(kp.data(), v.kp, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
     MSDP_TRY(stream_sync(s));
-    // root -> (node, local) maps
-    std::vector<int> rmap;
-    std::vector<int> rloc;
-    for (int q = 0; q < nn; ++q)
-      for (int j = 0; j < kp[q]; ++j) { rmap.push_back(q); rloc.push_back(j); }
-    const int nroots = (int)rmap.size();
+    int nroots = 0;
+    for (int q = 0; q < nn; ++q) nroots += kp[q];
     if (nroots > 0) {
-      // pad to a whole number of 4-wave blocks
+      // root -> (node, local) maps, built on the device from the counts;
+      // padded to a whole number of 4-wave blocks
       const int padded = cdiv(nroots, 4) * 4;
-      rmap.resize(padded, -1);
-      rloc.resize(padded, 0);
-      std::vector<int> both(2 * padded);
-      std::copy(rmap.begin(), rmap.end(), both.begin());
-      std::copy(rloc.begin(), rloc.end(), both.begin() + padded);
-      MSDP_HIP(hipMemcpyAsync(rootmap, both.data(), sizeof(int) * 2 * padded, hipMemcpyHostToDevice, s));
-      dc_secular_kernel<<<padded / 4, 256, 0, s>>>(nodes_dev, nn, rootmap, rootmap + padded, v);
-      dc_lowner_kernel<<<padded / 4, 256, 0, s>>>(nodes_dev, rootmap, rootmap + padded, v);
+      if (nn > DC_ROOTMAP_MAX) {
+        set_error("stedc: too many merge nodes in one level");
+        return -2;
+      }
+      dc_rootmap_kernel<<<1, 256, 0, s>>>(nodes, nn, v.kp, padded, rootmap, rootmap + padded);
+      dc_secular_kernel<<<padded / 4, 256, 0, s>>>(nodes, nn, rootmap, rootmap + padded, v);
+      dc_lowner_kernel<<<padded / 4, 256, 0, s>>>(nodes, rootmap, rootmap + padded, v);
       // reuse the maps: one workgroup per column
-      dc_vectors_kernel<<<nroots, 256, 0, s>>>(nodes_dev, rootmap, rootmap + padded, v, Ub);
+      dc_vectors_kernel<<<nroots, 256, 0, s>>>(nodes, rootmap, rootmap + padded, v, Ub);
       MSDP_HIP(hipGetLastError());
       for (int q = 0; q < nn; ++q) {
         const int K = level[q].hi - level[q].lo;
@@ -1801,13 +1847,11 @@ static int stedc(hipStream_t s, int n, double* d, const double* e, double* lam, 
         MSDP_TRY(gemm(s, false, false, K, kp[q], kp[q], 1.0, Qg + level[q].qoff, K,
                       Ub + level[q].qoff, kp[q], 0.0, Qn + level[q].qoff, K));
       }
-      MSDP_TRY(stream_sync(s));  // keep `both` alive until consumed
     }
-    dc_rank_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes_dev, v, lam, colsrc);
-    dc_scatter_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes_dev, v, Qg, Qn, colsrc,
+    dc_rank_kernel<<<dim3(cdiv(maxK, 256), nn), 256, 0, s>>>(nodes, v, lam, colsrc);
+    dc_scatter_kernel<<<dim3(cdiv(maxK, 256), maxK, nn), 256, 0, s>>>(nodes, v, Qg, Qn, colsrc,
                                                                        lam, Z, n);
     MSDP_HIP(hipGetLastError());
-    MSDP_TRY(stream_sync(s));  // `level` staging copy consumed
   }
   return 0;
 }

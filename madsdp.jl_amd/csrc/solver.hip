@@ -327,6 +327,16 @@ struct msdp_solver {
     MSDP_TRY(stream_sync(st));
     return 0;
   }
+  // the factorisation infos and the scalar slots behind one wait
+  int read_infos_slots(std::vector<int>& out) {
+    out.resize(4 * nb + 8);
+    hsc.resize(nslots);
+    MSDP_HIP(hipMemcpyAsync(out.data(), info, sizeof(int) * out.size(), hipMemcpyDeviceToHost, st));
+    MSDP_HIP(hipMemcpyAsync(hsc_pin, sc, sizeof(double) * nslots, hipMemcpyDeviceToHost, st));
+    MSDP_TRY(stream_sync(st));
+    std::memcpy(hsc.data(), hsc_pin, sizeof(double) * nslots);
+    return 0;
+  }
   int* binfo(int i, int k) { return info + 8 + 4 * i + k; }
 
   bool nt() const { return opt.direction == MSDP_DIR_NT; }
@@ -343,6 +353,7 @@ struct msdp_solver {
   int cholesky_factor2(Blk& b, const double* M0, double* L0, const double* M1, double* L1,
                        int* inf);
   int prepare_block(Blk& b, int i, bool factored = false);
+  int inverse_product(Blk& b);
   int set_dual_slacks(const double* yv);
   int slack(Blk& b, const double* yv, double* out);
   // PrimalDual
@@ -404,7 +415,8 @@ int msdp_solver::setup_aux() {
     if (b.n > JACOBI_VB_MAX) return 0;
     nmax = std::max(nmax, b.n);
   }
-  const size_t tw = std::max(trtri_work_elems(nmax), potrf_work_elems(nmax));
+  // room for the paired triangular inverses (two work areas)
+  const size_t tw = std::max(2 * trtri_work_elems(nmax), potrf_work_elems(nmax));
   for (int k = 0; k < NAUX; ++k) {
     hipStream_t q;
     MSDP_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
@@ -593,7 +605,7 @@ int msdp_solver::finalize() {
   MSDP_TRY(fill(st, 64, 0.0, red + reduce_scratch_elems(0)));  // the reduction counter
   MSDP_TRY(alloc(&ework, syevd_work_elems(nmax)));
   MSDP_TRY(alloc(&eiwork, syevd_iwork_elems(nmax)));
-  MSDP_TRY(alloc(&trwork, std::max(std::max(trtri_work_elems(nmax), potrf_work_elems(nmax)),
+  MSDP_TRY(alloc(&trwork, std::max(std::max(2 * trtri_work_elems(nmax), potrf_work_elems(nmax)),
                                     (size_t)m + 64)));
   int64_t tot = 0;
   for (auto& bb : blk) tot += bb.n;
@@ -705,10 +717,13 @@ int64_t msdp_solver::footprint() const {
   bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
   bytes += (mm / 512) * (2 * 512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdit, Hdiw
-  bytes += ((int64_t)syevd_work_elems((int)nmax) + trtri_work_elems((int)nmax) + 3 * tot) * d;
+  bytes += ((int64_t)syevd_work_elems((int)nmax) +
+            std::max<int64_t>(std::max<int64_t>(2 * (int64_t)trtri_work_elems((int)nmax),
+                                                (int64_t)potrf_work_elems((int)nmax)),
+                              mm + 64) + 3 * tot) * d;
   // the auxiliary streams' scratch of multi-block problems (setup_aux)
   if (nb >= 2 && nmax <= JACOBI_VB_MAX) {
-    const int64_t tw = (int64_t)std::max(trtri_work_elems((int)nmax), potrf_work_elems((int)nmax));
+    const int64_t tw = (int64_t)std::max(2 * trtri_work_elems((int)nmax), potrf_work_elems((int)nmax));
     bytes += NAUX * (tw + (int64_t)reduce_scratch_elems(0) + 64 + swork_b / d) * d;
   }
   // GEMM partials from the stream-ordered pool: split-K (at most 4M doubles)
@@ -805,6 +820,12 @@ int msdp_solver::prepare_block(Blk& bk, int i, bool factored) {
   Blk* bp = &bk;
   if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));  // recorded
   else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Ls, n, bp->Lsinv, n, trwork); }));
+  return inverse_product(bk);
+}
+
+// S^-1 = L_S^-T L_S^-1 (blocks.jl:128-130)
+int msdp_solver::inverse_product(Blk& bk) {
+  const int n = bk.n;
   MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Lsinv, n, bk.Lsinv, n, 0.0, bk.Si, n));
   MSDP_TRY(mirror_lower(st, n, bk.Si, n));
   return 0;
@@ -954,12 +975,22 @@ int msdp_solver::prepare(bool& ok) {
     const bool need_lx = nt() || opt.step == MSDP_STEP_EIGEN;
     // chol(S) and chol(X) of a large block share their launches
     const bool pair = need_lx && n > 64;
-    if (pair) MSDP_TRY(cholesky_factor2(bk, bk.S, bk.Ls, bk.X, bk.Lx, binfo(i, 0)));
-    MSDP_TRY(prepare_block(bk, i, pair));
-    if (need_lx) {
-      if (!pair) MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
-      if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));  // recorded
-      else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
+    if (pair) {
+      MSDP_TRY(cholesky_factor2(bk, bk.S, bk.Ls, bk.X, bk.Lx, binfo(i, 0)));
+      // L_S^-1 and L_X^-1 in the same launches (a batch of two at the
+      // matrices' distance, like the paired factorisation)
+      MSDP_TRY(opaque([=] {
+        return trtri_lower_batched(st, n, 2, bp->Ls, n, bp->Lx - bp->Ls, bp->Lsinv, n,
+                                   bp->Lxinv - bp->Lsinv, trwork, (int64_t)trtri_work_elems(n));
+      }));
+      MSDP_TRY(inverse_product(bk));
+    } else {
+      MSDP_TRY(prepare_block(bk, i, false));
+      if (need_lx) {
+        MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
+        if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));  // recorded
+        else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
+      }
     }
     if (nt()) {  // prepare_direction!(::NTWorkspace) directions.jl:45-75
       MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n));
@@ -986,8 +1017,7 @@ int msdp_solver::prepare(bool& ok) {
   }
   if (ns > 0) MSDP_TRY(scalar_op(st, SOP_ZINV, (int)ns, 0, nullptr, z, nullptr, nullptr, nullptr, zinv, nullptr, false));
   std::vector<int> inf;
-  MSDP_TRY(read_infos(inf));
-  MSDP_TRY(read_slots());
+  MSDP_TRY(read_infos_slots(inf));
   ok = true;
   for (int i = 0; i < nb && ok; ++i) {
     if (inf[8 + 4 * i] != 0 || inf[8 + 4 * i + 1] != 0) ok = false;
