@@ -223,6 +223,8 @@ int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_
 int chol_flush(hipStream_t s, int n, double* L, int64_t ld, double* scratch);
 // elementwise helpers
 int symmetrize(hipStream_t s, int n, double* A, int64_t lda);
+// two matrices of the same side (the second at A0 + sA) in one launch
+int symmetrize2(hipStream_t s, int n, double* A0, int64_t sA, int64_t lda);
 // L = chol((M + M')/2) with the flush and a zero upper triangle, n <= 64, one
 // launch; *info = 0, or the 1-based first non-positive pivot
 int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,

@@ -1503,8 +1503,10 @@ int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L
 // (J, I), I >= J, b = I (I + 1) / 2 + J.  Both tiles are read and written
 // along their columns and the transpose goes through LDS; each pair (x > y)
 // gets (A[x,y] + A[y,x]) / 2, the arithmetic of the batched form.
-__global__ __launch_bounds__(256) void sym_tiles_kernel(int n, double* A, int64_t lda, int mode) {
+__global__ __launch_bounds__(256) void sym_tiles_kernel(int n, double* A, int64_t lda, int mode,
+                                                        int64_t sA = 0) {
   __shared__ double Ut[64][65];  // Ut[yy][xx] = A[y0 + yy, x0 + xx]
+  A += blockIdx.y * sA;  // matrix blockIdx.y of a batch
   const int b = blockIdx.x;
   int I = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= b) ++I;
@@ -1542,6 +1544,16 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(int n, double* A, int64_
 static int sym_tiles(hipStream_t s, int n, double* A, int64_t lda, int mode) {
   const int nt = cdiv(n, 64);
   sym_tiles_kernel<<<nt * (nt + 1) / 2, 256, 0, s>>>(n, A, lda, mode);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// (A + A')/2 of two matrices of the same side in one launch (A1 at A0 + sA)
+int symmetrize2(hipStream_t s, int n, double* A0, int64_t sA, int64_t lda) {
+  MSDP_TRY(rec_forbid("symmetrize2"));
+  if (n <= 1) return 0;
+  const int nt = cdiv(n, 64);
+  sym_tiles_kernel<<<dim3(nt * (nt + 1) / 2, 2), 256, 0, s>>>(n, A0, lda, 0, sA);
   MSDP_HIP(hipGetLastError());
   return 0;
 }

@@ -1298,10 +1298,22 @@ int msdp_solver::step_lengths(double& a, double& bt) {
       const double* Ls[2] = {bk.Lxinv, bk.Lsinv};
       const double* Dl[2] = {bk.delX, bk.delS};
       double* T[2] = {bk.tmp2, bk.buf2};
-      for (int k = 0; k < 2; ++k) {
-        MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
-        MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n));
-        MSDP_TRY(symmetrize(st, n, T[k], n));
+      if (!rec_on && n <= 1024) {
+        // L_X^-1 dX L_X^-T and L_S^-1 dS L_S^-T in the same launches: a batch
+        // of two at the matrices' distances (intermediates tmp and buf);
+        // larger blocks keep the stream-K products of the whole chip
+        const int64_t dL = Ls[1] - Ls[0], dD = Dl[1] - Dl[0];
+        MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[0], n, Dl[0], n, 0.0, bk.tmp, n, 2, dL,
+                      dD, bk.buf - bk.tmp));
+        MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[0], n, 0.0, T[0], n, 2,
+                      bk.buf - bk.tmp, dL, T[1] - T[0]));
+        MSDP_TRY(symmetrize2(st, n, T[0], T[1] - T[0], n));
+      } else {
+        for (int k = 0; k < 2; ++k) {
+          MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
+          MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n));
+          MSDP_TRY(symmetrize(st, n, T[k], n));
+        }
       }
       if (n > JACOBI_VB_MAX) {  // the primal and dual eigenvalues in the same launches
         double *T0 = T[0], *T1 = T[1];
