@@ -197,14 +197,16 @@ size_t sytrf_bk_ctl_bytes();
 int sytrs_bk(hipStream_t s, int n, const double* A, int64_t lda, const int* ipiv, double* b);
 // Lower Cholesky in place (reads the lower triangle).  *info_dev receives 0 or
 // the 1-based index of the first non-positive pivot; kernels after a failure
-// return immediately.
-// With `inverses`, work (potrf_work_elems) receives the inverses of the
+// return immediately.  work (potrf_work_elems) holds the diagonal blocks'
+// factors on the way; with `inverses` it receives the inverses of the
 // factor's diagonal 64-blocks (block b at work + 4096 b, 64 x 64, ld 64).
 int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info_dev, double* work,
           bool inverses = false);
 // Two independent factorisations of the same side in the same launches:
-// info_dev[0] and info_dev[1] receive the two statuses.
-int potrf2(hipStream_t s, int n, double* A0, double* A1, int64_t lda, int* info_dev);
+// info_dev[0] and info_dev[1] receive the two statuses; work holds
+// 2 potrf_work_elems(n).
+int potrf2(hipStream_t s, int n, double* A0, double* A1, int64_t lda, int* info_dev,
+           double* work);
 size_t potrf_work_elems(int n);
 // Batched Cholesky of small matrices (n <= 64), one workgroup each.
 int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t stride,

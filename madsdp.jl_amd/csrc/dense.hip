@@ -944,23 +944,60 @@ __device__ __attribute__((always_inline)) inline void chol_panel_step(const doub
   if constexpr (K + 1 < 64) chol_panel_step<K + 1>(Lt, rd, nb, x);
 }
 
-__global__ __launch_bounds__(64) void chol_panel_kernel(int rows, double* A21, int64_t lda,
-                                                        const double* L11, const int* info, int jb,
-                                                        int64_t sA) {
-  A21 += blockIdx.y * sA;
-  L11 += blockIdx.y * sA;
-  info += blockIdx.y;
-  if (*info != 0) return;
+// The factor of a full 64 x 64 diagonal block into Lt (rows in lanes, kept
+// out of line so that its registers are not live beside the panel's); the
+// block is written back when `store`.  Returns nonzero on a failed pivot,
+// whose 1-based column is then left in Lt[64 * W64_LD - 1] as a double.
+__device__ __noinline__ int chol_diag_into(const double* A11, int64_t lda, double* Lt,
+                                           double* store, int n) {
+  const int lane = threadIdx.x & 63;
+  double d[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c) d[c] = c <= lane ? A11[lane + (int64_t)c * lda] : 0.0;
+  const int fail = chol64_rows(d, n, Lt);
+  if (fail) {
+    wave_lds_sync();
+    if (lane == 0) Lt[64 * W64_LD - 1] = (double)fail;
+    wave_lds_sync();
+    return 1;
+  }
+  if (store) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c) store[lane + 64 * c] = c <= lane ? d[c] : 0.0;
+  }
+  return 0;
+}
+__device__ inline int chol_diag_fail(const double* Lt) { return (int)Lt[64 * W64_LD - 1]; }
+
+// A full 64 x 64 diagonal block and the rows below it in one launch: every
+// workgroup factors the diagonal block itself (chol_diag_kernel's arithmetic,
+// so the same bits in every workgroup, the factor straight into the LDS
+// layout of the substitution) and solves its 64 rows by substitution (x_k
+// /= L_kk, then x_j -= x_k L_jk for j > k); workgroup 0 records a failure and
+// writes the block's factor to `dout` (64 x 64, ld 64) -- not over A11, which
+// the other workgroups may still be reading -- and diag_store_kernel moves
+// the factors into place after the last step.  One dependent launch per
+// column step fewer than a diagonal launch followed by a panel launch.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+chol_diag_panel_kernel(int rows, double* A11, int64_t lda, int64_t sA, int* info, int info_base,
+                       int jb, double* dout, int64_t sW) {
+  // jb (64) arrives at run time: the substitution's per-step guard then ends a
+  // basic block per step, which keeps the scheduler from hoisting later
+  // steps' LDS reads (the row would spill)
+  A11 += blockIdx.y * sA;
+  dout += blockIdx.y * sW;
+  int* inf = info + blockIdx.y;
+  if (*inf != 0) return;
   __shared__ double Lt[64 * W64_LD];  // L11[j][k] at Lt[k * W64_LD + j]
   __shared__ double rd[64];
   const int lane = threadIdx.x;
+  double* A21 = A11 + 64;
   const int r = blockIdx.x * 64 + lane;
+  if (chol_diag_into(A11, lda, Lt, blockIdx.x == 0 ? dout : nullptr, jb)) {
+    if (blockIdx.x == 0 && lane == 0) atomicCAS(inf, 0, info_base + chol_diag_fail(Lt));
+    return;
+  }
   double x[64];
-#pragma unroll
-  for (int k = 0; k < 64; ++k) x[k] = lane >= k ? L11[lane + (int64_t)k * lda] : 0.0;
-#pragma unroll
-  for (int k = 0; k < 64; ++k) Lt[k * W64_LD + lane] = x[k];
-  // the panel's loads are in flight while the reciprocals are formed
 #pragma unroll
   for (int k = 0; k < 64; ++k) x[k] = r < rows ? A21[r + (int64_t)k * lda] : 0.0;
   wave_lds_sync();
@@ -972,9 +1009,24 @@ __global__ __launch_bounds__(64) void chol_panel_kernel(int rows, double* A21, i
   for (int k = 0; k < 64; ++k) A21[r + (int64_t)k * lda] = x[k];
 }
 
+// The diagonal blocks 0..nblk-1 that chol_diag_panel_kernel factored into
+// W (block b at W + 4096 b, ld 64) copied into A's lower triangle.
+__global__ __launch_bounds__(256) void diag_store_kernel(double* A, int64_t lda, int64_t sA,
+                                                         const double* W, int64_t sW,
+                                                         const int* info) {
+  const int b = blockIdx.x;
+  A += blockIdx.y * sA;
+  W += blockIdx.y * sW + (int64_t)b * 4096;
+  if (info[blockIdx.y] != 0) return;
+  double* D = A + (int64_t)64 * b * (lda + 1);
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    if (r >= c) D[r + (int64_t)c * lda] = W[e];
+  }
+}
+
 size_t potrf_work_elems(int n) { return (size_t)cdiv(std::max(n, 1), 64) * 4096; }
 
-// count (1 or 2) matrices: the second at A + sA, its info at info[1]
 // Right-looking blocked Cholesky in 256-column panels for large n: inside a
 // panel the 64-column steps (diagonal block, rows below by substitution)
 // update only the panel's own later columns; the rest of the matrix takes one
@@ -985,19 +1037,21 @@ constexpr int POTRF_NB = 256;
 constexpr int POTRF_BLOCKED_MIN = 4096;
 
 static int potrf_blocked(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
-                         int* info) {
+                         int* info, double* work, int64_t sW) {
   for (int J0 = 0; J0 < n; J0 += POTRF_NB) {
     const int J1 = std::min(n, J0 + POTRF_NB);
     for (int j0 = J0; j0 < J1; j0 += 64) {
       const int jb = std::min(64, J1 - j0);
       double* A11 = A + j0 + (int64_t)j0 * lda;
-      chol_diag_kernel<<<count, 64, 0, s>>>(jb, A11, lda, sA, info, j0, 1, nullptr);
-      MSDP_HIP(hipGetLastError());
       const int rest = n - j0 - jb;
-      if (rest <= 0) continue;  // jb == 64 below
-      double* A21 = A11 + jb;
-      chol_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(rest, A21, lda, A11, info, jb,
-                                                                   sA);
+      if (rest <= 0) {
+        chol_diag_kernel<<<count, 64, 0, s>>>(jb, A11, lda, sA, info, j0, 1, nullptr);
+        MSDP_HIP(hipGetLastError());
+        continue;
+      }
+      double* A21 = A11 + jb;  // jb == 64 here
+      chol_diag_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(
+          rest, A11, lda, sA, info, j0, jb, work + (int64_t)(j0 / 64) * 4096, sW);
       MSDP_HIP(hipGetLastError());
       const int w = J1 - j0 - jb;  // the panel's later columns
       if (w <= 0) continue;
@@ -1018,20 +1072,42 @@ static int potrf_blocked(hipStream_t s, int n, double* A, int64_t lda, int64_t s
   return 0;
 }
 
+static int potrf_steps(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
+                       int* info, double* work, int64_t sW);
+
+// count (1 or 2) matrices: the second at A + sA, its info at info[1]; work
+// (potrf_work_elems(n) per matrix, the second at work + sW) receives the
+// factors of the diagonal blocks on the way
 static int potrf_count(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
-                       int* info) {
+                       int* info, double* work, int64_t sW) {
   MSDP_HIP(hipMemsetAsync(info, 0, sizeof(int) * count, s));
-  if (n >= POTRF_BLOCKED_MIN && !shared_device()) return potrf_blocked(s, n, A, lda, sA, count, info);
+  if (n >= POTRF_BLOCKED_MIN && !shared_device())
+    MSDP_TRY(potrf_blocked(s, n, A, lda, sA, count, info, work, sW));
+  else
+    MSDP_TRY(potrf_steps(s, n, A, lda, sA, count, info, work, sW));
+  // every block but the last went through a fused diagonal-and-panel step
+  const int nfused = cdiv(n, 64) - 1;
+  if (nfused > 0) {
+    diag_store_kernel<<<dim3(nfused, count), 256, 0, s>>>(A, lda, sA, work, sW, info);
+    MSDP_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+static int potrf_steps(hipStream_t s, int n, double* A, int64_t lda, int64_t sA, int count,
+                       int* info, double* work, int64_t sW) {
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int jb = std::min(64, n - j0);
     double* A11 = A + j0 + (int64_t)j0 * lda;
-    chol_diag_kernel<<<count, 64, 0, s>>>(jb, A11, lda, sA, info, j0, 1, nullptr);
-    MSDP_HIP(hipGetLastError());
     const int rest = n - j0 - jb;
+    if (rest == 0) {
+      chol_diag_kernel<<<count, 64, 0, s>>>(jb, A11, lda, sA, info, j0, 1, nullptr);
+      MSDP_HIP(hipGetLastError());
+    }
     if (rest > 0) {  // jb == 64 here
       double* A21 = A11 + jb;
-      chol_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(rest, A21, lda, A11, info, jb,
-                                                                   sA);
+      chol_diag_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(
+          rest, A11, lda, sA, info, j0, jb, work + (int64_t)(j0 / 64) * 4096, sW);
       MSDP_HIP(hipGetLastError());
       double* A22 = A21 + (int64_t)jb * lda;
       MSDP_TRY(gemm_guarded(s, false, true, rest, rest, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
@@ -1043,7 +1119,7 @@ static int potrf_count(hipStream_t s, int n, double* A, int64_t lda, int64_t sA,
 
 int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work, bool inverses) {
   MSDP_TRY(rec_forbid("potrf"));
-  MSDP_TRY(potrf_count(s, n, A, lda, 0, 1, info));
+  MSDP_TRY(potrf_count(s, n, A, lda, 0, 1, info, work, 0));
   if (inverses) {  // all diagonal-block inverses in one launch
     diag_inv_kernel<<<cdiv(n, 64), 64, 0, s>>>(n, A, lda, work, info);
     MSDP_HIP(hipGetLastError());
@@ -1051,14 +1127,15 @@ int potrf(hipStream_t s, int n, double* A, int64_t lda, int* info, double* work,
   return 0;
 }
 
-int potrf2(hipStream_t s, int n, double* A0, double* A1, int64_t lda, int* info) {
+int potrf2(hipStream_t s, int n, double* A0, double* A1, int64_t lda, int* info, double* work) {
   MSDP_TRY(rec_forbid("potrf2"));
   const intptr_t d = reinterpret_cast<intptr_t>(A1) - reinterpret_cast<intptr_t>(A0);
   if (d % (intptr_t)sizeof(double) != 0) {
     set_error("potrf2: the two matrices are not a whole number of doubles apart");
     return -2;
   }
-  return potrf_count(s, n, A0, lda, (int64_t)(d / (intptr_t)sizeof(double)), 2, info);
+  return potrf_count(s, n, A0, lda, (int64_t)(d / (intptr_t)sizeof(double)), 2, info, work,
+                     (int64_t)potrf_work_elems(n));
 }
 
 int potrf_small_batched(hipStream_t s, int n, double* A, int64_t lda, int64_t stride, int batch,

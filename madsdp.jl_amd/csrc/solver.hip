@@ -805,7 +805,7 @@ int msdp_solver::cholesky_factor2(Blk& bk, const double* M0, double* L0, const d
     MSDP_TRY(symmetrize(st, n, L0, n));
     MSDP_TRY(copy_mat(st, n, n, M1, n, L1, n));
     MSDP_TRY(symmetrize(st, n, L1, n));
-    MSDP_TRY(potrf2(st, n, L0, L1, n, inf));
+    MSDP_TRY(potrf2(st, n, L0, L1, n, inf, trwork));
     MSDP_TRY(chol_flush(st, n, L0, n, red));
     MSDP_TRY(chol_flush(st, n, L1, n, red));
     return 0;
