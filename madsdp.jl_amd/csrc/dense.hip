@@ -806,8 +806,9 @@ constexpr int W64_LD = 65;  // LDS leading dimension of a 64 x 64 block
 // in LAPACK's right-looking potrf.  Column j of L is left in
 // Lc[j * W64_LD + i] (zeros above the diagonal and at rows >= n).  The
 // column steps are template instances (compile-time j), so every register
-// index is static; the published column is read in chunks of 16 whose loads
-// are issued together (one LDS round trip per chunk).
+// index is static; the published column is read in chunks of 32 whose loads
+// are issued together (at most two LDS round trips per step; the reads are
+// broadcasts, every lane the same address).
 template <int J>
 __device__ __attribute__((always_inline)) inline void chol64_step(double (&r)[64], int n,
                                                                   double* Lc, int& fail) {
@@ -823,14 +824,14 @@ __device__ __attribute__((always_inline)) inline void chol64_step(double (&r)[64
       Lc[J * W64_LD + lane] = v;
       wave_lds_sync();
 #pragma unroll
-      for (int g = ((J + 1) / 16) * 16; g < 64; g += 16) {
-        double lc[16];
+      for (int g = ((J + 1) / 32) * 32; g < 64; g += 32) {
+        double lc[32];
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
+        for (int q = 0; q < 32; ++q)
           if (g + q > J) lc[q] = Lc[J * W64_LD + g + q];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
+        for (int q = 0; q < 32; ++q)
           if (g + q > J) r[g + q] -= v * lc[q];
       }
     }
@@ -921,7 +922,7 @@ __global__ void diag_inv_kernel(int n, const double* L, int64_t ldl, double* Di,
 // one wavefront per 64 rows, lane = row, the row in registers; for k = 0..63
 // x_k *= 1 / L_kk, then x_j -= x_k L_jk for j > k (BLAS dtrsm, right side,
 // which also scales by the reciprocal).  Column k of L11 is read from LDS in
-// chunks of 16 whose loads are issued together.  The step guard (always true
+// chunks of 32 whose loads are issued together.  The step guard (always true
 // for the 64-wide panels) ends a basic block per step: without it the
 // scheduler hoists the LDS reads of later steps and the row spills.
 template <int K>
@@ -931,25 +932,27 @@ __device__ __attribute__((always_inline)) inline void chol_panel_step(const doub
   if (K >= nb) return;
   x[K] *= rd[K];
 #pragma unroll
-  for (int g = ((K + 1) / 16) * 16; g < 64; g += 16) {
-    double lc[16];
+  for (int g = ((K + 1) / 32) * 32; g < 64; g += 32) {
+    double lc[32];
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
+    for (int q = 0; q < 32; ++q)
       if (g + q > K) lc[q] = Lt[K * W64_LD + g + q];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
+    for (int q = 0; q < 32; ++q)
       if (g + q > K) x[g + q] -= x[K] * lc[q];
   }
   if constexpr (K + 1 < 64) chol_panel_step<K + 1>(Lt, rd, nb, x);
 }
 
-// The factor of a full 64 x 64 diagonal block into Lt (rows in lanes, kept
-// out of line so that its registers are not live beside the panel's); the
-// block is written back when `store`.  Returns nonzero on a failed pivot,
-// whose 1-based column is then left in Lt[64 * W64_LD - 1] as a double.
-__device__ __noinline__ int chol_diag_into(const double* A11, int64_t lda, double* Lt,
-                                           double* store, int n) {
+// The factor of a full 64 x 64 diagonal block into Lt (rows in lanes; n, 64,
+// arrives at run time so that the per-step guards keep the steps apart); the
+// factor is also written to `store` (64 x 64, ld 64) when given.  Returns
+// nonzero on a failed pivot, whose 1-based column is then left in
+// Lt[64 * W64_LD - 1] as a double.
+__device__ __attribute__((always_inline)) inline int chol_diag_into(const double* A11,
+                                                                    int64_t lda, double* Lt,
+                                                                    double* store, int n) {
   const int lane = threadIdx.x & 63;
   double d[64];
 #pragma unroll
