@@ -86,7 +86,7 @@ def _bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("N,K", [(1, 1), (129, 31), (300, 1000), (700, 2500)])
+@pytest.mark.parametrize("N,K", [(1, 1), (129, 31), (300, 1000), (700, 2500), (1300, 96)])
 def test_syrk_bf16x3(env, N, K):
     """C = alpha (hi'hi + hi'lo + lo'hi) + beta C on the lower triangle, the
     upper triangle untouched.  Against the same three products formed in
