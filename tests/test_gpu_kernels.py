@@ -93,7 +93,8 @@ def test_syrk_bf16x3(env, N, K):
     FP64 from a numpy emulation of the split (hi = bf16(a), lo = bf16(a -
     hi)) the only difference is the FP32 accumulation (a transposed operand
     or a wrong lane would be off by O(1)); against A'A in FP64 the error is
-    the split's (about 2^-17 per entry)."""
+    the split's (about 2^-17 per entry).  N = 1300 spans two 8 x 8
+    super-tiles of 128-tiles (the XCD grouping of the tile order)."""
     torch, lib, st = env
     rng = np.random.default_rng(N * 3 + K)
     A = rng.standard_normal((K, N)) * np.exp(rng.uniform(-3, 3, size=(1, N)))
