@@ -261,6 +261,17 @@ int msdp_server_destroy(msdp_server* v);
 int msdp_k_gemm(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                 double beta, double* C, int64_t ldc);
+/* msdp_k_gemm with triangular operands and an optional lower-triangle output.
+ * No counterpart in the reference (BLAS has trmm for one triangular operand):
+ * it is the product the solver uses for its triangular factors (L_X V,
+ * L_X^-1 dX L_X^-T, L_S^-T L_S^-1, ...).  tri: MSDP_TRI_* bits saying that
+ * op(A) / op(B) is lower or upper triangular; every output tile then runs only
+ * the K range where both can be nonzero, so the entries on the zero side are
+ * never read.  lower = 1: only C's lower triangle is written (and computed). */
+enum { MSDP_TRI_A_LOWER = 1, MSDP_TRI_A_UPPER = 2, MSDP_TRI_B_LOWER = 4, MSDP_TRI_B_UPPER = 8 };
+int msdp_k_gemm_tri(void* stream, int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K,
+                    double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                    double beta, double* C, int64_t ldc, int32_t lower, int32_t tri);
 /* msdp_k_gemm that allocates nothing.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The

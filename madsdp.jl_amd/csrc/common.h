@@ -171,13 +171,18 @@ __device__ inline double block_reduce(double v, double* sh) {
 // --------------------------------------------------------------------------
 // dense FP64 kernels (dense.hip)
 // --------------------------------------------------------------------------
+// Triangular operands of a product (`tri`, bits): op(A) lower (its entries
+// right of the diagonal are zero) / upper, op(B) lower / upper.  Every output
+// tile then runs only the K range where both operands can be nonzero; the
+// entries skipped must be stored zeros (the result is the full product's).
+enum { TRI_AL = 1, TRI_AU = 2, TRI_BL = 4, TRI_BU = 8 };
 // C = alpha op(A) op(B) + beta C for a batch of `batch` problems separated by
 // the given strides; `lower` restricts the write (and the work) to C's lower
 // triangle, i.e. a SYRK-shaped product.
 int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
          const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
          double* C, int64_t ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0,
-         int64_t sC = 0, bool lower = false);
+         int64_t sC = 0, bool lower = false, int tri = 0);
 // the same product with its partials in the library's per-device scratch
 // array (no allocation; calls that overlap on one device race)
 int gemm_noalloc(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
@@ -301,7 +306,7 @@ enum RecKind {
   REC_CHOL64, REC_TRTRI64, REC_NKINDS
 };
 struct GemmItem {
-  int M, N, K, lower;
+  int M, N, K, lower, tri;
   double alpha, beta;
   const double* A;
   const double* B;

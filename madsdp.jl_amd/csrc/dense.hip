@@ -85,12 +85,25 @@ static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 constexpr int GT = 64;   // tile edge
 constexpr int GK = 16;   // k depth per LDS stage
 
+// K range [kb, ke) of output tile (m0, n0) (edge T) where the triangular
+// operands named by `tri` can both be nonzero; kb and ke are multiples of T
+// (or ke = K).
+__host__ __device__ inline void tri_krange(int tri, int m0, int n0, int T, int K, int& kb,
+                                           int& ke) {
+  kb = 0;
+  ke = K;
+  if (tri & TRI_AL) ke = min(ke, m0 + T);
+  if (tri & TRI_AU) kb = max(kb, m0);
+  if (tri & TRI_BL) kb = max(kb, n0);
+  if (tri & TRI_BU) ke = min(ke, n0 + T);
+}
+
 // One 64 x 64 output tile (tm, tn) of C = alpha op(A) op(B) + beta C.
 template <bool TA, bool TB>
 __device__ __attribute__((always_inline)) inline void gemm64_tile(
     int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda,
     const double* __restrict__ B, int64_t ldb, double beta, double* __restrict__ C, int64_t ldc,
-    int lower, int tm, int tn) {
+    int lower, int tm, int tn, int tri = 0) {
   if (lower && tn > tm) return;
   __shared__ double As[GK][GT + 1];
   __shared__ double Bs[GK][GT + 1];
@@ -99,7 +112,9 @@ __device__ __attribute__((always_inline)) inline void gemm64_tile(
   const int li = lane & 15, lk = lane >> 4;
   v4d acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
   const int m0 = tm * GT, n0 = tn * GT;
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  int kb, ke;
+  tri_krange(tri, m0, n0, GT, K, kb, ke);
+  for (int k0 = kb; k0 < ke; k0 += GK) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int e = tid + r * 256;
@@ -155,11 +170,11 @@ template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(
     int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
-    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
+    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard, int tri) {
   if (guard && guard[blockIdx.z] != 0) return;  // one guard per batch entry
   const int64_t bz = blockIdx.z;
   gemm64_tile<TA, TB>(M, N, K, alpha, A + bz * sA, lda, B + bz * sB, ldb, beta, C + bz * sC, ldc,
-                      lower, blockIdx.x, blockIdx.y);
+                      lower, blockIdx.x, blockIdx.y, tri);
 }
 
 // Variable-size batch of 64-tile products (recorded per-block GEMMs): the
@@ -177,7 +192,7 @@ __global__ __launch_bounds__(256) void gemm_vb_kernel(const GemmItem* __restrict
   const int t = w - start[lo];
   const int ntm = (it.M + GT - 1) / GT;
   gemm64_tile<TA, TB>(it.M, it.N, it.K, it.alpha, it.A, it.lda, it.B, it.ldb, it.beta, it.C,
-                      it.ldc, it.lower, t % ntm, t / ntm);
+                      it.ldc, it.lower, t % ntm, t / ntm, it.tri);
 }
 
 // Large-tile GEMM: 128x128 output tile per workgroup of 4 waves, each wave
@@ -281,7 +296,7 @@ template <bool TA, bool TB>
 __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
     int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
-    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard) {
+    int64_t ldc, int64_t sC, int lower, const int* __restrict__ guard, int tri) {
   if (guard && guard[blockIdx.z] != 0) return;  // one guard per batch entry
   int tm = blockIdx.x, tn = blockIdx.y;
   if (lower == 2) {  // square lower triangle on a 1-D, XCD-grouped grid
@@ -300,7 +315,11 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
-  gemm128_slabs<TA, TB>(M, N, K, A, lda, B, ldb, tm * BT, tn * BT, 0, (K + BK - 1) / BK, As, Bs, acc);
+  int kb, ke;
+  tri_krange(tri, tm * BT, tn * BT, BT, K, kb, ke);
+  if (kb < ke)
+    gemm128_slabs<TA, TB>(M, N, K, A, lda, B, ldb, tm * BT, tn * BT, kb / BK, (ke + BK - 1) / BK,
+                          As, Bs, acc);
   gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, lower, acc);
 }
 
@@ -323,6 +342,7 @@ constexpr int SK_GM = 8;
 struct SkGrid {
   int ntm, ntn, I, G;
   int64_t U;
+  int tri = 0, lower = 0;  // the variable-unit partition (gemm_skv_*)
   __host__ __device__ int64_t first(int r) const { return (int64_t)r * U / G; }
   __host__ __device__ int owner(int64_t u) const { return (int)(((u + 1) * G - 1) / U); }
   __host__ __device__ void tile(int t, int& tm, int& tn) const {
@@ -408,6 +428,146 @@ __global__ __launch_bounds__(256) void gemm_sk_fixup_kernel(int M, int N, double
   gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, 0, acc);
 }
 
+// --------------------------------------------------------------------------
+// Stream-K over a variable number of slabs per tile: triangular operands
+// (each tile runs only the K range where both can be nonzero) and lower-
+// triangle outputs (tiles above the diagonal run none).  pre[t] is the
+// prefix of the tiles' slab counts in the grouped tile order (pre[T] = U,
+// from sk_prefix_kernel); run r covers units [r U / G, (r + 1) U / G) as in
+// the uniform partition, a unit being one slab of one tile.
+// --------------------------------------------------------------------------
+__host__ __device__ inline void sk_tile_slabs(const SkGrid& sg, int tm, int tn, int& lo,
+                                              int& hi) {
+  if (sg.lower && tn > tm) {
+    lo = hi = 0;
+    return;
+  }
+  int kb, ke;
+  tri_krange(sg.tri, tm * BT, tn * BT, BT, sg.I * BK, kb, ke);
+  lo = kb / BK;
+  hi = kb < ke ? ke / BK : lo;
+}
+
+__global__ __launch_bounds__(256) void sk_prefix_kernel(SkGrid sg, long long* __restrict__ pre) {
+  __shared__ long long part[256];
+  const int T = sg.ntm * sg.ntn, tid = threadIdx.x;
+  const int per = (T + 255) / 256, t0 = tid * per, t1 = min(T, t0 + per);
+  long long sum = 0;
+  for (int t = t0; t < t1; ++t) {
+    int tm, tn, lo, hi;
+    sg.tile(t, tm, tn);
+    sk_tile_slabs(sg, tm, tn, lo, hi);
+    sum += hi - lo;
+  }
+  part[tid] = sum;
+  __syncthreads();
+  if (tid == 0) {  // exclusive scan of the 256 chunk sums
+    long long run = 0;
+    for (int q = 0; q < 256; ++q) {
+      const long long v = part[q];
+      part[q] = run;
+      run += v;
+    }
+    pre[T] = run;
+  }
+  __syncthreads();
+  long long run = part[tid];
+  for (int t = t0; t < t1; ++t) {
+    int tm, tn, lo, hi;
+    sg.tile(t, tm, tn);
+    sk_tile_slabs(sg, tm, tn, lo, hi);
+    pre[t] = run;
+    run += hi - lo;
+  }
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm_skv_f64_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda,
+    const double* __restrict__ B, int64_t ldb, double beta, double* __restrict__ C, int64_t ldc,
+    SkGrid sg, const long long* __restrict__ pre, double* __restrict__ P) {
+  const int r = sk_run(sg);
+  if (r >= sg.G) return;
+  const int T = sg.ntm * sg.ntn;
+  const long long U = pre[T];
+  long long u = (long long)r * U / sg.G;
+  const long long u1 = (long long)(r + 1) * U / sg.G;
+  if (u >= u1) return;
+  const long long ufirst = u;
+  int t = 0, hi_t = T - 1;  // the last tile with pre[t] <= u: it holds unit u
+  while (t < hi_t) {
+    const int mid = (t + hi_t + 1) >> 1;
+    if (pre[mid] <= u) t = mid; else hi_t = mid - 1;
+  }
+  __shared__ double As[2][BK][BT + 2];
+  __shared__ double Bs[2][BK][BT + 2];
+  while (u < u1) {
+    while (pre[t + 1] <= u) ++t;  // past tiles without slabs
+    int tm, tn, slo, shi;
+    sg.tile(t, tm, tn);
+    sk_tile_slabs(sg, tm, tn, slo, shi);
+    const long long pt = pre[t];
+    const int s0 = slo + (int)(u - pt);
+    const int s1 = slo + (int)min<long long>(shi - slo, u1 - pt);
+    v4d acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+    gemm128_slabs<TA, TB>(M, N, K, A, lda, B, ldb, tm * BT, tn * BT, s0, s1, As, Bs, acc);
+    if (s0 == slo && s1 == shi) {
+      gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, sg.lower, acc);
+    } else {
+      double* slot = P + (int64_t)(2 * r + (pt <= ufirst ? 0 : 1)) * (BT * BT);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) slot[((a * 4 + b) * 4 + q) * 256 + threadIdx.x] = acc[a][b][q];
+    }
+    u = pt + (s1 - slo);
+  }
+}
+
+// One workgroup per tile: the parts of a split tile added in K order into C;
+// a tile without slabs gets beta C (none above the diagonal of a lower
+// product).
+__global__ __launch_bounds__(256) void gemm_skv_fixup_kernel(int M, int N, double alpha,
+                                                             double beta, double* __restrict__ C,
+                                                             int64_t ldc, SkGrid sg,
+                                                             const long long* __restrict__ pre,
+                                                             const double* __restrict__ P) {
+  const int t = blockIdx.x;
+  int tm, tn;
+  sg.tile(t, tm, tn);
+  const long long pa = pre[t], pb = pre[t + 1];
+  v4d acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+  if (pa == pb) {
+    if (sg.lower && tn > tm) return;
+    gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, sg.lower, acc);
+    return;
+  }
+  const long long U = pre[sg.ntm * sg.ntn];
+  const int ra = (int)(((pa + 1) * sg.G - 1) / U), rb = (int)((pb * sg.G - 1) / U);
+  if (ra == rb) return;  // the whole tile was one run's: already in C
+  for (int r = ra; r <= rb; ++r) {
+    const long long fr = (long long)r * U / sg.G;
+    const double* slot = P + (int64_t)(2 * r + (fr >= pa ? 0 : 1)) * (BT * BT);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][b][q] += slot[((a * 4 + b) * 4 + q) * 256 + threadIdx.x];
+  }
+  gemm128_store(M, N, alpha, beta, C, ldc, tm * BT, tn * BT, sg.lower, acc);
+}
+
 // Split-K GEMM for skinny outputs with a long K (few output tiles): each
 // blockIdx.z computes the partial product over one K chunk into P; a second
 // kernel sums the chunks (fixed order, deterministic) into C.
@@ -417,10 +577,12 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, i
                                                           int64_t lda,
                                                           const double* __restrict__ B,
                                                           int64_t ldb, double* __restrict__ P,
-                                                          int lower) {
+                                                          int lower, int tri) {
   const int tm = blockIdx.x, tn = blockIdx.y, part = blockIdx.z;
   if (lower && tn > tm) return;  // wholly above the diagonal: never read back
-  const int kbeg = part * kc, kend = min(K, kbeg + kc);
+  int kb, ke;
+  tri_krange(tri, tm * GT, tn * GT, GT, K, kb, ke);
+  const int kbeg = max(part * kc, kb), kend = min(min(K, part * kc + kc), ke);
   __shared__ double As[GK][GT + 1];
   __shared__ double Bs[GK][GT + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -593,9 +755,39 @@ static bool stream_k_shape(int M, int N, int K, int batch, bool lower, const int
          T < (int64_t)tune(T_SK_MAX_WAVES) * G;
 }
 
+// Variable-unit stream-K (triangular operands or a lower-triangle output):
+// the tiles' slab counts decide; at least streamk_min_slabs slabs per
+// workgroup, at least half as many tiles with slabs as workgroups (fewer
+// would be split many ways), fewer than streamk_max_waves waves of tiles.
+static bool stream_kv_shape(int M, int N, int K, int batch, bool lower, int tri, const int* guard,
+                            int G, SkGrid& sg) {
+  if (batch != 1 || guard || M < BT || N < BT || (!lower && !tri)) return false;
+  sg.ntm = cdiv(M, BT);
+  sg.ntn = cdiv(N, BT);
+  sg.I = cdiv(K, BK);
+  sg.G = G;
+  sg.tri = tri;
+  sg.lower = lower ? 1 : 0;
+  const int64_t T = (int64_t)sg.ntm * sg.ntn;
+  if (T >= (int64_t)tune(T_SK_MAX_WAVES) * G) return false;
+  int64_t U = 0, live = 0;
+  for (int tm = 0; tm < sg.ntm; ++tm)
+    for (int tn = 0; tn < sg.ntn; ++tn) {
+      int lo, hi;
+      sk_tile_slabs(sg, tm, tn, lo, hi);
+      U += hi - lo;
+      live += hi > lo;
+    }
+  sg.U = U;
+  return U >= (int64_t)tune(T_SK_MIN_UNITS) * G && 2 * live >= G;
+}
+
 // stream-K partials at the largest occupancy (two of the 256-thread,
-// 67 KB-LDS workgroups fit per CU)
-size_t gemm_sk_partials_bytes() { return sizeof(double) * 2 * (size_t)2 * device_cus() * BT * BT; }
+// 67 KB-LDS workgroups fit per CU), and the variable partition's tile prefix
+size_t gemm_sk_partials_bytes() {
+  const size_t G = (size_t)2 * device_cus();
+  return sizeof(double) * (2 * G * BT * BT + (size_t)tune(T_SK_MAX_WAVES) * G + 1);
+}
 
 // Partials of the stream-K and split-K paths: the caller's workspace `ws` of
 // `ws_elems` doubles, or (ws == nullptr) a block of the stream-ordered pool.
@@ -633,11 +825,48 @@ static int gemm_stream_k(hipStream_t s, bool ta, bool tb, int M, int N, int K, d
   return 0;
 }
 
+static int gemm_stream_kv(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                          const double* A, int64_t lda, const double* B, int64_t ldb,
+                          double beta, double* C, int64_t ldc, const SkGrid& sg, double* ws,
+                          size_t ws_elems) {
+  const int T = sg.ntm * sg.ntn;
+  const size_t slots = 2 * (size_t)sg.G * BT * BT;
+  double* Pd = nullptr;
+  MSDP_TRY(partials(s, slots + T + 1, ws, ws_elems, &Pd));
+  long long* pre = reinterpret_cast<long long*>(Pd + slots);
+  sk_prefix_kernel<<<1, 256, 0, s>>>(sg, pre);
+  const dim3 g(sg.G);
+  if (!ta && !tb) gemm_skv_f64_kernel<false, false><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, pre, Pd);
+  else if (ta && !tb) gemm_skv_f64_kernel<true, false><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, pre, Pd);
+  else if (!ta && tb) gemm_skv_f64_kernel<false, true><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, pre, Pd);
+  else gemm_skv_f64_kernel<true, true><<<g, 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, pre, Pd);
+  gemm_skv_fixup_kernel<<<T, 256, 0, s>>>(M, N, alpha, beta, C, ldc, sg, pre, Pd);
+  MSDP_HIP(hipGetLastError());
+  if (!ws) MSDP_HIP(hipFreeAsync(Pd, s));
+  return 0;
+}
+
+// Multiply-adds of a product whose triangular operands (tri) and lower
+// output skip work, counted over 64 x 64 tiles (the profiler's flop count).
+static double product_flops(int M, int N, int K, bool lower, int tri) {
+  if (!tri) return 2.0 * M * N * (double)K * (lower ? 0.5 : 1.0);
+  double f = 0.0;
+  for (int tm = 0; tm < cdiv(M, GT); ++tm)
+    for (int tn = 0; tn < cdiv(N, GT); ++tn) {
+      if (lower && tn > tm) continue;
+      int kb, ke;
+      tri_krange(tri, tm * GT, tn * GT, GT, K, kb, ke);
+      if (ke > kb)
+        f += 2.0 * std::min(GT, M - tm * GT) * std::min(GT, N - tn * GT) * (double)(ke - kb);
+    }
+  return f;
+}
+
 static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                         double* C, int64_t ldc, int batch, int64_t sA, int64_t sB, int64_t sC,
                         bool lower, const int* guard, double* ws = nullptr,
-                        size_t ws_elems = 0) {
+                        size_t ws_elems = 0, int tri = 0) {
   if (M <= 0 || N <= 0 || batch <= 0) return 0;
   if (K <= 0) {
     // C = beta C
@@ -655,11 +884,19 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
   SkGrid sg;
   // a device shared with other solves: one launch per product
   const bool whole_chip = !shared_device();
-  if (whole_chip && stream_k_shape(M, N, K, batch, lower, guard, tune(T_SK_OCC) * device_cus(), sg)) {
+  if (whole_chip && !tri && stream_k_shape(M, N, K, batch, lower, guard, tune(T_SK_OCC) * device_cus(), sg)) {
     if (prof) prof_before(s);
     MSDP_TRY(gemm_stream_k(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, ws,
                            ws_elems));
     if (prof) prof_after(s, 2.0 * M * N * (double)K);
+    return 0;
+  }
+  if (whole_chip &&
+      stream_kv_shape(M, N, K, batch, lower, tri, guard, tune(T_SK_OCC) * device_cus(), sg)) {
+    if (prof) prof_before(s);
+    MSDP_TRY(gemm_stream_kv(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, sg, ws,
+                            ws_elems));
+    if (prof) prof_after(s, product_flops(M, N, K, lower, tri));
     return 0;
   }
   const int64_t tiles64 = (int64_t)cdiv(M, GT) * cdiv(N, GT) * batch;
@@ -674,13 +911,13 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
     MSDP_TRY(partials(s, (size_t)M * N * parts, ws, ws_elems, &Pd));
     if (prof) prof_before(s);
     dim3 g3(cdiv(M, GT), cdiv(N, GT), parts);
-    if (!ta && !tb) gemm_splitk_kernel<false, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
-    else if (ta && !tb) gemm_splitk_kernel<true, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
-    else if (!ta && tb) gemm_splitk_kernel<false, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
-    else gemm_splitk_kernel<true, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower);
+    if (!ta && !tb) gemm_splitk_kernel<false, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower, tri);
+    else if (ta && !tb) gemm_splitk_kernel<true, false><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower, tri);
+    else if (!ta && tb) gemm_splitk_kernel<false, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower, tri);
+    else gemm_splitk_kernel<true, true><<<g3, 256, 0, s>>>(M, N, K, kc, A, lda, B, ldb, Pd, lower, tri);
     splitk_reduce_kernel<<<cdiv((int64_t)M * N, 256), 256, 0, s>>>(M, N, parts, Pd, alpha, beta, C, ldc,
                                                                      lower);
-    if (prof) prof_after(s, 2.0 * M * N * (double)K * (lower ? 0.5 : 1.0));
+    if (prof) prof_after(s, product_flops(M, N, K, lower, tri));
     MSDP_HIP(hipGetLastError());
     if (!ws) MSDP_HIP(hipFreeAsync(Pd, s));
     return 0;
@@ -700,49 +937,49 @@ static int gemm_guarded(hipStream_t s, bool ta, bool tb, int M, int N, int K, do
       lmode = 2;
     }
     if (!ta && !tb)
-      gemm128_f64_kernel<false, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
+      gemm128_f64_kernel<false, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard, tri);
     else if (ta && !tb)
-      gemm128_f64_kernel<true, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
+      gemm128_f64_kernel<true, false><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard, tri);
     else if (!ta && tb)
-      gemm128_f64_kernel<false, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
+      gemm128_f64_kernel<false, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard, tri);
     else
-      gemm128_f64_kernel<true, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard);
-    if (prof) prof_after(s, 2.0 * M * N * (double)K * batch * (lower ? 0.5 : 1.0));
+      gemm128_f64_kernel<true, true><<<g2, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lmode, guard, tri);
+    if (prof) prof_after(s, product_flops(M, N, K, lower, tri) * batch);
     MSDP_HIP(hipGetLastError());
     return 0;
   }
   dim3 grid(cdiv(M, GT), cdiv(N, GT), batch);
   if (!ta && !tb)
-    gemm_f64_kernel<false, false><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    gemm_f64_kernel<false, false><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard, tri);
   else if (ta && !tb)
-    gemm_f64_kernel<true, false><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    gemm_f64_kernel<true, false><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard, tri);
   else if (!ta && tb)
-    gemm_f64_kernel<false, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
+    gemm_f64_kernel<false, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard, tri);
   else
-    gemm_f64_kernel<true, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard);
-  if (prof) prof_after(s, 2.0 * M * N * (double)K * batch * (lower ? 0.5 : 1.0));
+    gemm_f64_kernel<true, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, lower, guard, tri);
+  if (prof) prof_after(s, product_flops(M, N, K, lower, tri) * batch);
   MSDP_HIP(hipGetLastError());
   return 0;
 }
 
 int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, const double* A,
          int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
-         int batch, int64_t sA, int64_t sB, int64_t sC, bool lower) {
+         int batch, int64_t sA, int64_t sB, int64_t sC, bool lower, int tri) {
   if (rec_active()) {
     RecOp o;
     // batched only where a lone launch takes the 64-tile kernel as well
     if (batch == 1 && M > 0 && N > 0 && K > 0 && std::max(M, std::max(N, K)) < 512) {
       o.kind = ta ? (tb ? REC_GEMM_TT : REC_GEMM_TN) : (tb ? REC_GEMM_NT : REC_GEMM_NN);
-      o.g = GemmItem{M, N, K, lower ? 1 : 0, alpha, beta, A, B, C, lda, ldb, ldc};
+      o.g = GemmItem{M, N, K, lower ? 1 : 0, tri, alpha, beta, A, B, C, lda, ldb, ldc};
     }
     o.run = [=] {
       return gemm_guarded(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB,
-                          sC, lower, nullptr);
+                          sC, lower, nullptr, nullptr, 0, tri);
     };
     return rec_add(std::move(o));
   }
   return gemm_guarded(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC,
-                      lower, nullptr);
+                      lower, nullptr, nullptr, 0, tri);
 }
 
 // --------------------------------------------------------------------------
@@ -1278,10 +1515,11 @@ __global__ __launch_bounds__(256) void trtri_level_kernel(TrtriLevel lv, const d
   double* wk = work + lv.woff[k];
   if (STAGE == 1)
     gemm64_tile<false, false>(r, cc, cc, 1.0, L + b + (int64_t)a * ldl, ldl,
-                              Li + a + (int64_t)a * ldi, ldi, 0.0, wk, r, 0, t % ntm, t / ntm);
+                              Li + a + (int64_t)a * ldi, ldi, 0.0, wk, r, 0, t % ntm, t / ntm,
+                              TRI_BL);
   else
     gemm64_tile<false, false>(r, cc, r, -1.0, Li + b + (int64_t)b * ldi, ldi, wk, r, 0.0,
-                              Li + b + (int64_t)a * ldi, ldi, 0, t % ntm, t / ntm);
+                              Li + b + (int64_t)a * ldi, ldi, 0, t % ntm, t / ntm, TRI_AL);
 }
 
 static void trtri_tree(int lo, int hi, int depth, std::vector<std::vector<std::array<int, 3>>>& lv) {

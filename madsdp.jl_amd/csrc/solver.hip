@@ -823,10 +823,12 @@ int msdp_solver::prepare_block(Blk& bk, int i, bool factored) {
   return inverse_product(bk);
 }
 
-// S^-1 = L_S^-T L_S^-1 (blocks.jl:128-130)
+// S^-1 = L_S^-T L_S^-1 (blocks.jl:128-130): the lower triangle, each tile
+// over the K range where both triangular factors are nonzero, then mirrored
 int msdp_solver::inverse_product(Blk& bk) {
   const int n = bk.n;
-  MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Lsinv, n, bk.Lsinv, n, 0.0, bk.Si, n));
+  MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Lsinv, n, bk.Lsinv, n, 0.0, bk.Si, n, 1, 0, 0,
+                0, true, TRI_AU | TRI_BL));
   MSDP_TRY(mirror_lower(st, n, bk.Si, n));
   return 0;
 }
@@ -993,9 +995,12 @@ int msdp_solver::prepare(bool& ok) {
       }
     }
     if (nt()) {  // prepare_direction!(::NTWorkspace) directions.jl:45-75
-      MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n));
-      MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Mm, n, bk.Mm, n, 0.0, bk.Nm, n));
-      MSDP_TRY(symmetrize(st, n, bk.Nm, n));
+      // M = Ls' Lx (upper times lower), N = M'M (its lower triangle, mirrored)
+      MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Ls, n, bk.Lx, n, 0.0, bk.Mm, n, 1, 0, 0, 0,
+                    false, TRI_AU | TRI_BL));
+      MSDP_TRY(gemm(st, true, false, n, n, n, 1.0, bk.Mm, n, bk.Mm, n, 0.0, bk.Nm, n, 1, 0, 0, 0,
+                    true));
+      MSDP_TRY(mirror_lower(st, n, bk.Nm, n));
       if (n > JACOBI_VB_MAX) {
         MSDP_TRY(opaque([=]() -> int {
           MSDP_TRY(syevd(st, n, bp->Nm, n, bp->lam, bp->Vv, n, true, ework, eiwork));
@@ -1031,13 +1036,15 @@ int msdp_solver::prepare(bool& ok) {
 int msdp_solver::nt_scaling(Blk& bk, int i) {
   const int n = bk.n;
   MSDP_TRY(sqrt_min(st, n, bk.lam, bk.D, slot(bslot(i, 2))));
-  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Vv, n, 0.0, bk.G, n));
+  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Vv, n, 0.0, bk.G, n, 1, 0, 0, 0, false,
+                TRI_AL));
   MSDP_TRY(scale_cols(st, n, n, bk.G, n, bk.D, -1));
   MSDP_TRY(transpose(st, n, n, bk.Vv, n, bk.tmp, n));
   MSDP_TRY(scale_rows(st, n, n, bk.tmp, n, bk.D, +1));
-  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, bk.Lxinv, n, 0.0, bk.Gi, n));
-  MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.G, n, bk.G, n, 0.0, bk.W, n));
-  MSDP_TRY(symmetrize(st, n, bk.W, n));
+  MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.tmp, n, bk.Lxinv, n, 0.0, bk.Gi, n, 1, 0, 0, 0,
+                false, TRI_BL));
+  MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.G, n, bk.G, n, 0.0, bk.W, n, 1, 0, 0, 0, true));
+  MSDP_TRY(mirror_lower(st, n, bk.W, n));
   return 0;
 }
 
@@ -1304,14 +1311,16 @@ int msdp_solver::step_lengths(double& a, double& bt) {
         // larger blocks keep the stream-K products of the whole chip
         const int64_t dL = Ls[1] - Ls[0], dD = Dl[1] - Dl[0];
         MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[0], n, Dl[0], n, 0.0, bk.tmp, n, 2, dL,
-                      dD, bk.buf - bk.tmp));
+                      dD, bk.buf - bk.tmp, false, TRI_AL));
         MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[0], n, 0.0, T[0], n, 2,
-                      bk.buf - bk.tmp, dL, T[1] - T[0]));
+                      bk.buf - bk.tmp, dL, T[1] - T[0], false, TRI_BU));
         MSDP_TRY(symmetrize2(st, n, T[0], T[1] - T[0], n));
       } else {
         for (int k = 0; k < 2; ++k) {
-          MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n));
-          MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n));
+          MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, Ls[k], n, Dl[k], n, 0.0, bk.tmp, n, 1, 0,
+                        0, 0, false, TRI_AL));
+          MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, Ls[k], n, 0.0, T[k], n, 1, 0, 0,
+                        0, false, TRI_BU));
           MSDP_TRY(symmetrize(st, n, T[k], n));
         }
       }
@@ -1720,14 +1729,18 @@ int msdp_solver::predictor_step(bool& ok) {  // :232-264
     // proximity eigenvalues (:205-215)
     MSDP_TRY(cholesky_factor(bk, bk.Shat, bk.Lh, binfo(i, 2)));
     MSDP_TRY(trtri_lower(st, n, bk.Lh, n, bk.Lx, n, trwork));  // Lx reused for Lhat^-1
-    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Dc, n, 0.0, bk.tmp, n));
-    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lx, n, 0.0, bk.tmp2, n));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lx, n, bk.Dc, n, 0.0, bk.tmp, n, 1, 0, 0, 0,
+                  false, TRI_AL));
+    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lx, n, 0.0, bk.tmp2, n, 1, 0, 0, 0,
+                  false, TRI_BU));
     MSDP_TRY(symmetrize(st, n, bk.tmp2, n));
     if (n > JACOBI_VB_MAX)
       MSDP_TRY(syevd(st, n, bk.tmp2, n, eigall + off, bk.Vv, n, false, ework, eiwork));
     // ||Ls^-1 Shat Ls^-T||_F^2 (:219-229)
-    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lsinv, n, bk.Shat, n, 0.0, bk.tmp, n));
-    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lsinv, n, 0.0, bk.buf2, n));
+    MSDP_TRY(gemm(st, false, false, n, n, n, 1.0, bk.Lsinv, n, bk.Shat, n, 0.0, bk.tmp, n, 1, 0, 0,
+                  0, false, TRI_AL));
+    MSDP_TRY(gemm(st, false, true, n, n, n, 1.0, bk.tmp, n, bk.Lsinv, n, 0.0, bk.buf2, n, 1, 0, 0,
+                  0, false, TRI_BU));
     MSDP_TRY(dot(st, n2, bk.buf2, bk.buf2, slot(bslot(i, 6)), red));
     off += n;
   }
@@ -2528,6 +2541,16 @@ int msdp_k_gemm(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int3
                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
                 int64_t ldc) {
   return gemm((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int msdp_k_gemm_tri(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K,
+                    double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                    double beta, double* C, int64_t ldc, int32_t lower, int32_t tri) {
+  if (tri < 0 || tri > 15) {
+    set_error("msdp_k_gemm_tri: tri must be a combination of MSDP_TRI_* bits");
+    return -2;
+  }
+  return gemm((hipStream_t)stream, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+              1, 0, 0, 0, lower != 0, tri);
 }
 int msdp_k_gemm_noalloc(void* stream, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K,
                         double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,

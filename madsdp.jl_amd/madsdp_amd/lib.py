@@ -115,6 +115,9 @@ SIGNATURES = {
     "msdp_server_destroy": (C.c_int, [P]),
     "msdp_k_gemm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64]),
+    "msdp_k_gemm_tri": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_double, P, C.c_int64, P, C.c_int64, C.c_double, P, C.c_int64,
+                                  C.c_int32, C.c_int32]),
     "msdp_k_gemm_noalloc": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_double, P, C.c_int64, P, C.c_int64, C.c_double, P,
                                       C.c_int64]),

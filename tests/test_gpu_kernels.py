@@ -56,6 +56,66 @@ def test_gemm(env, ta, tb, M, N, K):
     np.testing.assert_allclose(host(dC, (M, N)), ref, rtol=0, atol=1e-12 * max(1, K))
 
 
+TRI_AL, TRI_AU, TRI_BL, TRI_BU = 1, 2, 4, 8
+
+
+def _tri_operand(rng, rows, cols, lower):
+    """A rows x cols matrix, zero above (lower) or below (upper) its diagonal."""
+    X = rng.standard_normal((rows, cols))
+    return np.tril(X) if lower else np.triu(X)
+
+
+@pytest.mark.parametrize("ta,tb,tri,lower,M,N,K", [
+    (0, 0, TRI_AL, 0, 3000, 3000, 3000),            # L_X V (stream-K, variable slabs)
+    (0, 0, TRI_BL, 0, 3000, 3000, 3000),            # tmp L_X^-1
+    (1, 0, TRI_AU | TRI_BL, 1, 3000, 3000, 3000),   # L_S^-T L_S^-1, lower triangle
+    (1, 0, TRI_AU | TRI_BL, 0, 3000, 3000, 3000),   # L_S' L_X
+    (0, 1, TRI_BU, 0, 3000, 3000, 3000),            # tmp L^-T
+    (1, 0, 0, 1, 3000, 3000, 3000),                 # M'M, lower triangle only
+    (0, 1, TRI_AL | TRI_BU, 0, 2100, 2100, 2100),
+    (0, 0, TRI_AL, 0, 300, 300, 300),               # 64 / 128-tile kernels
+    (1, 0, TRI_AU | TRI_BL, 1, 300, 300, 300),
+    (0, 1, TRI_BU, 0, 1030, 1030, 1030),
+    (0, 0, TRI_BL, 0, 64, 128, 3000),               # split-K over a lower trapezoid
+    (1, 1, TRI_AU | TRI_BU, 0, 130, 130, 130),
+])
+def test_gemm_tri(env, ta, tb, tri, lower, M, N, K):
+    """Products with triangular operands run each tile over the K range where
+    both operands can be nonzero (and a lower-triangle output over the tiles
+    on or below the diagonal only): the result is numpy's full product of the
+    same triangular matrices, and a lower product leaves the upper triangle
+    as it was."""
+    torch, lib, st = env
+    rng = np.random.default_rng(M + 3 * K + 11 * tri + 5 * lower)
+    # op(A) is M x K, op(B) is K x N; the triangle is that of op(X)
+    opA = _tri_operand(rng, M, K, True) if tri & TRI_AL else (
+        _tri_operand(rng, M, K, False) if tri & TRI_AU else rng.standard_normal((M, K)))
+    opB = _tri_operand(rng, K, N, True) if tri & TRI_BL else (
+        _tri_operand(rng, K, N, False) if tri & TRI_BU else rng.standard_normal((K, N)))
+    A = opA.T.copy() if ta else opA
+    B = opB.T.copy() if tb else opB
+    Cm = rng.standard_normal((M, N))
+    dA, dB, dC = dev(torch, A), dev(torch, B), dev(torch, Cm)
+    print(f"gemm_tri ta={ta} tb={tb} tri={tri} lower={lower} M={M} N={N} K={K}", flush=True)
+    rc = lib.msdp_k_gemm_tri(st, ta, tb, M, N, K, 1.5, P(dA), A.shape[0], P(dB), B.shape[0], -0.5,
+                             P(dC), M, lower, tri)
+    assert rc == 0
+    torch.cuda.synchronize()
+    ref = 1.5 * opA @ opB - 0.5 * Cm
+    got = host(dC, (M, N))
+    if lower:
+        iu = np.triu_indices(M, 1, N)
+        np.testing.assert_array_equal(got[iu], Cm[iu])
+        ref = np.tril(ref) + np.triu(Cm, 1)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * max(1, K))
+
+
+def test_gemm_tri_rejects_bad_flags(env):
+    torch, lib, st = env
+    d = torch.zeros(4, dtype=torch.float64, device="cuda")
+    assert lib.msdp_k_gemm_tri(st, 0, 0, 2, 2, 2, 1.0, P(d), 2, P(d), 2, 0.0, P(d), 2, 0, 16) == -2
+
+
 @pytest.mark.parametrize("trans,N,K", [(1, 300, 700), (0, 257, 64), (1, 2100, 1100),
                                        (0, 2304, 1030)])
 def test_syrk_lower(env, trans, N, K):
