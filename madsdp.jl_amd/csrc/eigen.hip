@@ -1194,10 +1194,13 @@ __device__ void syev_small(const EigBatch& it) {
           continue;
         }
         if (++iter > 60) { state = 3; break; }
-        // sqrt of the sum of squares in place of hypot, and one reciprocal per
-        // rotation in place of two quotients: the chase below is a dependent
-        // chain, and these are its longest latencies (the operands are matrix
-        // entries far inside the range where squaring overflows or underflows)
+        // The chase below is a dependent chain whose longest latencies were a
+        // square root and a quotient per rotation: each rotation instead takes
+        // 1 / sqrt(f^2 + g^2) from the hardware estimate and two Newton steps
+        // (y <- y + y (1 - x y^2) / 2, each doubling the correct bits; the
+        // last one leaves y within a few ulp), then r = x y, s = f y, c = g y.
+        // The sum of squares replaces hypot: the operands are matrix entries
+        // far inside the range where squaring overflows or underflows.
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
         double r = sqrt(fma(g, g, 1.0));
         g = d[m] - d[l] + e[l] / (g + copysign(r, g));
@@ -1208,15 +1211,18 @@ __device__ void syev_small(const EigBatch& it) {
           double en = 0.0, dn = 0.0;
           if (i > l) { en = e[i - 1]; dn = d[i - 1]; }
           const double f = s * ei, b = c * ei;
-          r = sqrt(fma(f, f, g * g));
-          e[i + 1] = r;
-          if (r == 0.0) {
+          const double x = fma(f, f, g * g);
+          if (x == 0.0) {
+            e[i + 1] = 0.0;
             d[i + 1] = dip1 - pp;
             e[m] = 0.0;
             split = true;
             break;
           }
-          const double ri = 1.0 / r;
+          double ri = __builtin_amdgcn_rsq(x);
+          ri = fma(0.5 * ri, fma(-x * ri, ri, 1.0), ri);
+          ri = fma(0.5 * ri, fma(-x * ri, ri, 1.0), ri);
+          e[i + 1] = x * ri;
           s = f * ri;
           c = g * ri;
           g = dip1 - pp;

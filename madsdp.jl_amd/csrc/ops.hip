@@ -187,7 +187,8 @@ __global__ void schur_coo_kernel(int ncon, const int* con_list, const int* con_p
 }
 
 // svec with sqrt(2) off the diagonal, column-major upper triangle order:
-// index c(c+1)/2 + r for r <= c.
+// index c(c+1)/2 + r for r <= c; entry (r, c) of the symmetric P_j is read
+// from its lower triangle, P[c, r] (the only triangle the product computes).
 __global__ void svec_pack_kernel(int n, int cb, const double* P, double* B, int64_t ldb) {
   const int64_t d = (int64_t)n * (n + 1) / 2;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,7 +199,7 @@ __global__ void svec_pack_kernel(int n, int cb, const double* P, double* B, int6
   while ((int64_t)(c + 1) * (c + 2) / 2 <= k) ++c;
   while ((int64_t)c * (c + 1) / 2 > k) --c;
   const int r = (int)(k - (int64_t)c * (c + 1) / 2);
-  const double v = P[(int64_t)j * n * n + r + (int64_t)c * n];
+  const double v = P[(int64_t)j * n * n + c + (int64_t)r * n];  // lower triangle: P[c, r]
   B[k + (int64_t)j * ldb] = r == c ? v : 1.4142135623730951 * v;
 }
 
@@ -257,8 +258,10 @@ int dense_panel(hipStream_t st, const BlockOps& b, const SchurScaling& sc, doubl
     const int c = std::min(cb, mi - j0);
     MSDP_TRY(gemm(st, false, sc.transF2, n, n, n, 1.0, b.mats + (int64_t)j0 * n2, n, sc.F2, n, 0.0,
                   T, n, c, n2, 0, n2));
+    // P_j is symmetric: svec packs its lower triangle, the only part computed
     double* dst = sc.full_vec ? Bm + (int64_t)j0 * n2 : P;
-    MSDP_TRY(gemm(st, sc.transL, false, n, n, n, 1.0, sc.Lm, n, T, n, 0.0, dst, n, c, 0, n2, n2));
+    MSDP_TRY(gemm(st, sc.transL, false, n, n, n, 1.0, sc.Lm, n, T, n, 0.0, dst, n, c, 0, n2, n2,
+                  !sc.full_vec));
     if (!sc.full_vec) {
       dim3 g(cdiv(drow, 256), c);
       svec_pack_kernel<<<g, 256, 0, st>>>(n, c, P, Bm + (int64_t)j0 * drow, drow);
