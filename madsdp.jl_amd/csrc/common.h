@@ -115,6 +115,18 @@ __device__ inline double wave_min(double v) {
   v = fmin(v, dpp_f64_or<0x143, 0xc>(v, id));
   return readlane_f64(v, 63);
 }
+// 1 / sqrt(x) for finite x > 0: the hardware estimate (v_rsq_f64) refined by
+// two Newton steps y <- y + y (1 - x y^2) / 2, each doubling the correct
+// bits, to within a few ulp.  A short dependent chain where a correctly
+// rounded square root followed by a quotient is a long one (the plane
+// rotations of the QL chase).  The Cholesky column steps keep the correctly
+// rounded operations: with these the dual-centering parity case converged
+// one iteration earlier than the FP64 reference.
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
 // Orders a wavefront's LDS stores before its later LDS loads (no workgroup
 // barrier: the wavefront's LDS accesses complete in program order).
 __device__ inline void wave_lds_sync() {

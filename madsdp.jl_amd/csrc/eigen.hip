@@ -1196,9 +1196,7 @@ __device__ void syev_small(const EigBatch& it) {
         if (++iter > 60) { state = 3; break; }
         // The chase below is a dependent chain whose longest latencies were a
         // square root and a quotient per rotation: each rotation instead takes
-        // 1 / sqrt(f^2 + g^2) from the hardware estimate and two Newton steps
-        // (y <- y + y (1 - x y^2) / 2, each doubling the correct bits; the
-        // last one leaves y within a few ulp), then r = x y, s = f y, c = g y.
+        // y = 1 / sqrt(f^2 + g^2) (rsqrt_nr), then r = x y, s = f y, c = g y.
         // The sum of squares replaces hypot: the operands are matrix entries
         // far inside the range where squaring overflows or underflows.
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
@@ -1219,9 +1217,7 @@ __device__ void syev_small(const EigBatch& it) {
             split = true;
             break;
           }
-          double ri = __builtin_amdgcn_rsq(x);
-          ri = fma(0.5 * ri, fma(-x * ri, ri, 1.0), ri);
-          ri = fma(0.5 * ri, fma(-x * ri, ri, 1.0), ri);
+          const double ri = rsqrt_nr(x);
           e[i + 1] = x * ri;
           s = f * ri;
           c = g * ri;
