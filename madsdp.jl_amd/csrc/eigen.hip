@@ -2040,21 +2040,52 @@ __global__ __launch_bounds__(1024) void lz_init_kernel(int n, double* q, int64_t
   for (int i = threadIdx.x; i < n; i += blockDim.x) q[i] *= inv;
 }
 
-// w[c] = sum_i A[i, c] q[i]  (A symmetric), one workgroup per column
+// w = A q (A symmetric: column c's dot with q gives w[c]).  LZ_GEMV_COLS
+// columns per workgroup: every thread keeps its rows of q in registers for
+// all of them and has LZ_GEMV_COLS x 4 loads of A in flight; the per-column
+// sums are added over the threads in a fixed order.
+constexpr int LZ_GEMV_COLS = 8;
+
 __global__ __launch_bounds__(256) void lz_gemv_kernel(int n, const double* A, const double* A1,
                                                       int64_t lda, const double* q, double* w,
                                                       int64_t ws, int inst0) {
-  __shared__ double red[4];
+  constexpr int C = LZ_GEMV_COLS, U = 4;
+  __shared__ double red[C][4];
   const int y = blockIdx.y + inst0;
   if (y) A = A1;
   q += y * ws;
   w += y * ws;
-  const int c = blockIdx.x;
-  const double* col = A + (int64_t)c * lda;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) v += col[i] * q[i];
-  v = block_reduce<0>(v, red);
-  if (threadIdx.x == 0) w[c] = v;
+  const int c0 = blockIdx.x * C;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* cols[C];
+#pragma unroll
+  for (int k = 0; k < C; ++k) cols[k] = A + (int64_t)min(c0 + k, n - 1) * lda;
+  double acc[C];
+#pragma unroll
+  for (int k = 0; k < C; ++k) acc[k] = 0.0;
+  for (int r0 = 0; r0 < n; r0 += 256 * U) {
+    double qv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = r0 + tid + 256 * u;
+      qv[u] = i < n ? q[i] : 0.0;
+    }
+    // rows past n read row n - 1 (a finite entry) against q = 0: every load
+    // is unconditional, so all of them are in flight together
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        acc[k] = fma(cols[k][min(r0 + tid + 256 * u, n - 1)], qv[u], acc[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const double t = wave_sum(acc[k]);
+    if (lane == 0) red[k][wv] = t;
+  }
+  __syncthreads();
+  if (tid < C && c0 + tid < n)
+    w[c0 + tid] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
 }
 
 // h[c] = Q[:, c]' w, one workgroup per column
@@ -2367,7 +2398,8 @@ static int lanczos_min(hipStream_t s, int n, const double* A0, const double* A1,
     const int inst0 = active[0] ? 0 : 1;
     const int ny = (active[0] && active[1]) ? 2 : 1;
     const double* qj = Q + (size_t)j * n;
-    lz_gemv_kernel<<<dim3(n, ny), 256, 0, s>>>(n, A0, A1, lda, qj, w, ws, inst0);
+    lz_gemv_kernel<<<dim3(cdiv(n, LZ_GEMV_COLS), ny), 256, 0, s>>>(n, A0, A1, lda, qj, w, ws,
+                                                                   inst0);
     lz_dots_kernel<<<dim3(j + 1, ny), 256, 0, s>>>(n, Q, w, h1, ws, inst0);
     lz_update_kernel<<<dim3(cdiv(n, 64), ny), 256, 0, s>>>(n, j + 1, Q, h1, w, ws, inst0);
     lz_dots_kernel<<<dim3(j + 1, ny), 256, 0, s>>>(n, Q, w, h2, ws, inst0);
