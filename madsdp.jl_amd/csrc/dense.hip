@@ -1047,8 +1047,8 @@ constexpr int W64_LD = 65;  // LDS leading dimension of a 64 x 64 block
 // are issued together (at most two LDS round trips per step; the reads are
 // broadcasts, every lane the same address).
 template <int J>
-__device__ __attribute__((always_inline)) inline void chol64_step(double (&r)[64], int n,
-                                                                  double* Lc, int& fail) {
+__device__ __attribute__((always_inline)) inline void chol64_step_one(double (&r)[64], int n,
+                                                                      double* Lc, int& fail) {
   const int lane = threadIdx.x & 63;
   if (J < n && fail == 0) {  // uniform
     const double d = readlane_f64(r[J], J);
@@ -1073,6 +1073,11 @@ __device__ __attribute__((always_inline)) inline void chol64_step(double (&r)[64
       }
     }
   }
+}
+template <int J>
+__device__ __attribute__((always_inline)) inline void chol64_step(double (&r)[64], int n,
+                                                                  double* Lc, int& fail) {
+  chol64_step_one<J>(r, n, Lc, fail);
   if constexpr (J + 1 < 64) chol64_step<J + 1>(r, n, Lc, fail);
 }
 
@@ -1163,11 +1168,11 @@ __global__ void diag_inv_kernel(int n, const double* L, int64_t ldl, double* Di,
 // for the 64-wide panels) ends a basic block per step: without it the
 // scheduler hoists the LDS reads of later steps and the row spills.
 template <int K>
-__device__ __attribute__((always_inline)) inline void chol_panel_step(const double* Lt,
-                                                                      const double* rd, int nb,
-                                                                      double (&x)[64]) {
+__device__ __attribute__((always_inline)) inline void chol_panel_step_one(const double* Lt,
+                                                                          double rdk, int nb,
+                                                                          double (&x)[64]) {
   if (K >= nb) return;
-  x[K] *= rd[K];
+  x[K] *= rdk;
 #pragma unroll
   for (int g = ((K + 1) / 32) * 32; g < 64; g += 32) {
     double lc[32];
@@ -1179,74 +1184,102 @@ __device__ __attribute__((always_inline)) inline void chol_panel_step(const doub
     for (int q = 0; q < 32; ++q)
       if (g + q > K) x[g + q] -= x[K] * lc[q];
   }
+}
+template <int K>
+__device__ __attribute__((always_inline)) inline void chol_panel_step(const double* Lt,
+                                                                      const double* rd, int nb,
+                                                                      double (&x)[64]) {
+  chol_panel_step_one<K>(Lt, rd[K], nb, x);
   if constexpr (K + 1 < 64) chol_panel_step<K + 1>(Lt, rd, nb, x);
 }
 
-// The factor of a full 64 x 64 diagonal block into Lt (rows in lanes; n, 64,
-// arrives at run time so that the per-step guards keep the steps apart); the
-// factor is also written to `store` (64 x 64, ld 64) when given.  Returns
-// nonzero on a failed pivot, whose 1-based column is then left in
-// Lt[64 * W64_LD - 1] as a double.
-__device__ __attribute__((always_inline)) inline int chol_diag_into(const double* A11,
-                                                                    int64_t lda, double* Lt,
-                                                                    double* store, int n) {
-  const int lane = threadIdx.x & 63;
-  double d[64];
-#pragma unroll
-  for (int c = 0; c < 64; ++c) d[c] = c <= lane ? A11[lane + (int64_t)c * lda] : 0.0;
-  const int fail = chol64_rows(d, n, Lt);
-  if (fail) {
-    wave_lds_sync();
-    if (lane == 0) Lt[64 * W64_LD - 1] = (double)fail;
-    wave_lds_sync();
-    return 1;
+// The factorisation of a 64 x 64 diagonal block (wavefront 0, rows in lanes)
+// and the substitution of 64 rows below it (wavefront 1) as one pipeline:
+// step J factors column J while the substitution applies column J - 1, one
+// workgroup barrier between steps.  Each wavefront does exactly the
+// arithmetic of chol64_step / chol_panel_step (the reciprocal 1 / L_KK is
+// the same quotient), so the bits are those of the two separate phases.
+template <int J>
+__device__ __attribute__((always_inline)) inline void chol_pipe_step(bool factor, double (&v)[64],
+                                                                     int jb, double* Lt,
+                                                                     int& fail) {
+  if (factor) {
+    chol64_step_one<J>(v, jb, Lt, fail);
+  } else if constexpr (J > 0) {
+    chol_panel_step_one<J - 1>(Lt, 1.0 / Lt[(J - 1) * (W64_LD + 1)], jb, v);
   }
-  if (store) {
-#pragma unroll
-    for (int c = 0; c < 64; ++c) store[lane + 64 * c] = c <= lane ? d[c] : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  if constexpr (J + 1 < 64) {
+    chol_pipe_step<J + 1>(factor, v, jb, Lt, fail);
+  } else if (!factor) {
+    chol_panel_step_one<63>(Lt, 1.0 / Lt[63 * (W64_LD + 1)], jb, v);
   }
-  return 0;
 }
-__device__ inline int chol_diag_fail(const double* Lt) { return (int)Lt[64 * W64_LD - 1]; }
+
 
 // A full 64 x 64 diagonal block and the rows below it in one launch: every
 // workgroup factors the diagonal block itself (chol_diag_kernel's arithmetic,
 // so the same bits in every workgroup, the factor straight into the LDS
-// layout of the substitution) and solves its 64 rows by substitution (x_k
-// /= L_kk, then x_j -= x_k L_jk for j > k); workgroup 0 records a failure and
-// writes the block's factor to `dout` (64 x 64, ld 64) -- not over A11, which
-// the other workgroups may still be reading -- and diag_store_kernel moves
-// the factors into place after the last step.  One dependent launch per
-// column step fewer than a diagonal launch followed by a panel launch.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+// layout of the substitution; wavefront 0) and solves its 64 rows by
+// substitution (x_k /= L_kk, then x_j -= x_k L_jk for j > k; wavefront 1),
+// the two pipelined one column apart (chol_pipe_step); workgroup 0 records a
+// failure and writes the block's factor to `dout` (64 x 64, ld 64) -- not
+// over A11, which the other workgroups may still be reading -- and
+// diag_store_kernel moves the factors into place after the last step.  One
+// dependent launch per column step fewer than a diagonal launch followed by
+// a panel launch.
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 chol_diag_panel_kernel(int rows, double* A11, int64_t lda, int64_t sA, int* info, int info_base,
                        int jb, double* dout, int64_t sW) {
-  // jb (64) arrives at run time: the substitution's per-step guard then ends a
-  // basic block per step, which keeps the scheduler from hoisting later
-  // steps' LDS reads (the row would spill)
+  // jb (64) arrives at run time: the per-step guards then end a basic block
+  // per step, which keeps the scheduler from hoisting later steps' LDS reads
+  // (the row would spill)
   A11 += blockIdx.y * sA;
   dout += blockIdx.y * sW;
   int* inf = info + blockIdx.y;
-  if (*inf != 0) return;
   __shared__ double Lt[64 * W64_LD];  // L11[j][k] at Lt[k * W64_LD + j]
-  __shared__ double rd[64];
-  const int lane = threadIdx.x;
+  __shared__ int s_skip, s_fail;
+  const int lane = threadIdx.x & 63;
+  // wavefront 0 factors, wavefront 1 substitutes (a wave-uniform, scalar branch)
+  const bool factor = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+  // an earlier step's failure ends the factorisation: one read, shared, so
+  // that both wavefronts take the same branch
+  if (threadIdx.x == 0) {
+    s_skip = *inf;
+    s_fail = 0;
+  }
+  __syncthreads();
+  if (s_skip != 0) return;
   double* A21 = A11 + 64;
   const int r = blockIdx.x * 64 + lane;
-  if (chol_diag_into(A11, lda, Lt, blockIdx.x == 0 ? dout : nullptr, jb)) {
-    if (blockIdx.x == 0 && lane == 0) atomicCAS(inf, 0, info_base + chol_diag_fail(Lt));
+  double v[64];
+  if (factor) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c) v[c] = c <= lane ? A11[lane + (int64_t)c * lda] : 0.0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 64; ++k) v[k] = r < rows ? A21[r + (int64_t)k * lda] : 0.0;
+  }
+  int fail = 0;
+  chol_pipe_step<0>(factor, v, jb, Lt, fail);
+  if (factor) {
+    if (fail) {
+      if (lane == 0) s_fail = fail;
+    } else if (blockIdx.x == 0) {
+#pragma unroll
+      for (int c = 0; c < 64; ++c) dout[lane + 64 * c] = c <= lane ? v[c] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (s_fail != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicCAS(inf, 0, info_base + s_fail);
     return;
   }
-  double x[64];
+  if (factor || r >= rows) return;
 #pragma unroll
-  for (int k = 0; k < 64; ++k) x[k] = r < rows ? A21[r + (int64_t)k * lda] : 0.0;
-  wave_lds_sync();
-  rd[lane] = 1.0 / Lt[lane * W64_LD + lane];
-  wave_lds_sync();
-  chol_panel_step<0>(Lt, rd, jb, x);
-  if (r >= rows) return;
-#pragma unroll
-  for (int k = 0; k < 64; ++k) A21[r + (int64_t)k * lda] = x[k];
+  for (int k = 0; k < 64; ++k) A21[r + (int64_t)k * lda] = v[k];
 }
 
 // The diagonal blocks 0..nblk-1 that chol_diag_panel_kernel factored into
@@ -1290,7 +1323,7 @@ static int potrf_blocked(hipStream_t s, int n, double* A, int64_t lda, int64_t s
         continue;
       }
       double* A21 = A11 + jb;  // jb == 64 here
-      chol_diag_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(
+      chol_diag_panel_kernel<<<dim3(cdiv(rest, 64), count), 128, 0, s>>>(
           rest, A11, lda, sA, info, j0, jb, work + (int64_t)(j0 / 64) * 4096, sW);
       MSDP_HIP(hipGetLastError());
       const int w = J1 - j0 - jb;  // the panel's later columns
@@ -1346,7 +1379,7 @@ static int potrf_steps(hipStream_t s, int n, double* A, int64_t lda, int64_t sA,
     }
     if (rest > 0) {  // jb == 64 here
       double* A21 = A11 + jb;
-      chol_diag_panel_kernel<<<dim3(cdiv(rest, 64), count), 64, 0, s>>>(
+      chol_diag_panel_kernel<<<dim3(cdiv(rest, 64), count), 128, 0, s>>>(
           rest, A11, lda, sA, info, j0, jb, work + (int64_t)(j0 / 64) * 4096, sW);
       MSDP_HIP(hipGetLastError());
       double* A22 = A21 + (int64_t)jb * lda;

@@ -186,20 +186,25 @@ __global__ void schur_coo_kernel(int ncon, const int* con_list, const int* con_p
   H[con_list[a] + (int64_t)con_list[b] * ldh] += s;
 }
 
-// svec with sqrt(2) off the diagonal, column-major upper triangle order:
-// index c(c+1)/2 + r for r <= c; entry (r, c) of the symmetric P_j is read
-// from its lower triangle, P[c, r] (the only triangle the product computes).
+// svec with sqrt(2) off the diagonal over the lower triangle in column-major
+// order (column c holds rows c..n-1 at c n - c (c - 1) / 2): consecutive
+// entries are consecutive rows of P_j, so the reads are coalesced, and the
+// lower triangle is the only one the product computes.  H = B'B and the
+// refinement operator B'(B v) do not depend on the order of B's rows.
 __global__ void svec_pack_kernel(int n, int cb, const double* P, double* B, int64_t ldb) {
   const int64_t d = (int64_t)n * (n + 1) / 2;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (k >= d || j >= cb) return;
-  // invert k -> (r, c)
-  int c = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
-  while ((int64_t)(c + 1) * (c + 2) / 2 <= k) ++c;
-  while ((int64_t)c * (c + 1) / 2 > k) --c;
-  const int r = (int)(k - (int64_t)c * (c + 1) / 2);
-  const double v = P[(int64_t)j * n * n + c + (int64_t)r * n];  // lower triangle: P[c, r]
+  // invert k -> (r, c): the last column c whose offset is <= k
+  auto off = [n](int64_t c) { return c * n - c * (c - 1) / 2; };
+  const double b2 = 2.0 * n + 1.0;
+  int64_t c = (int64_t)((b2 - sqrt(fmax(b2 * b2 - 8.0 * (double)k, 0.0))) * 0.5);
+  c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+  while (c + 1 < n && off(c + 1) <= k) ++c;
+  while (off(c) > k) --c;
+  const int64_t r = c + (k - off(c));
+  const double v = P[(int64_t)j * n * n + r + c * n];
   B[k + (int64_t)j * ldb] = r == c ? v : 1.4142135623730951 * v;
 }
 
