@@ -71,6 +71,31 @@ int64_t dense_panel_rows(int n, bool full_vec);
 int dense_panel(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* work,
                 size_t work_elems);
 
+// A(.) and A*(.) of every block in two launches each, for problems whose
+// blocks are all dense (two or more): one workgroup per (block, constraint)
+// pair writes its inner product to part, then each constraint adds its
+// pairs in block order (the additions of the per-block launches, in their
+// order); A*(y) runs one thread per entry of every block.
+struct DenseBlkDev {
+  const double* mats;
+  int64_t n2;
+  const int* con_list;
+  int ncon, pair0, tile0, pad;
+  const double* in[3];  // the A(.) arguments: X, buf, S^-1
+  double* out[2];       // the A*(.) results: tmp2, Dc
+};
+enum { JP_X = 0, JP_BUF = 1, JP_SI = 2 };
+enum { JT_TMP2 = 0, JT_DC = 1 };
+struct DenseBatch {
+  int nblk = 0, npairs = 0, ntiles = 0;
+  DenseBlkDev* blks = nullptr;
+  int* cptr = nullptr;    // [m + 1]: constraint c's pairs at cpairs[cptr[c]..cptr[c + 1])
+  int* cpairs = nullptr;  // [npairs], block order within a constraint
+  double* part = nullptr;  // [npairs]
+};
+int jprod_dense_batched(hipStream_t st, const DenseBatch& d, int m, int which, double* out);
+int jtprod_dense_batched(hipStream_t st, const DenseBatch& d, const double* y, int which);
+
 enum { SOP_ZINV = 0, SOP_RHS = 1, SOP_DIRS = 2, SOP_CORR = 3, SOP_XZ = 4 };
 int scalar_op(hipStream_t st, int op, int ns, double mu, const double* x, const double* z,
               const double* zinv, const double* a, const double* b, double* out, double* out2,

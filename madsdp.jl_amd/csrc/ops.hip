@@ -113,6 +113,78 @@ __global__ void jtprod_reduce_kernel(int64_t n2, int split, const double* part, 
   out[e] = s;
 }
 
+// the dense block holding key: the last one whose first pair (tile) is <= key
+__device__ inline int dense_blk_of(const DenseBlkDev* b, int nblk, int key, bool tiles) {
+  int lo = 0, hi = nblk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((tiles ? b[mid].tile0 : b[mid].pair0) <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// part[p] = <A_j, X_b> for pair p = (b, j): jprod_dense_kernel's sum
+__global__ __launch_bounds__(256) void jprod_pairs_kernel(const DenseBlkDev* blks, int nblk,
+                                                          int which, double* part) {
+  __shared__ double red[4];
+  const int p = blockIdx.x;
+  const DenseBlkDev& b = blks[dense_blk_of(blks, nblk, p, false)];
+  const int64_t n2 = b.n2;
+  const double* Aj = b.mats + (int64_t)(p - b.pair0) * n2;
+  const double* X = b.in[which];
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < n2; e += 256) s += Aj[e] * X[e];
+  s = block_reduce<0>(s, red);
+  if (threadIdx.x == 0) part[p] = s;
+}
+
+__global__ void jprod_gather_kernel(int m, const int* cptr, const int* cpairs,
+                                    const double* part, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  const int k0 = cptr[c], k1 = cptr[c + 1];
+  if (k0 == k1) return;
+  double o = out[c];
+  for (int k = k0; k < k1; ++k) o += part[cpairs[k]];
+  out[c] = o;
+}
+
+// out_b[e] = sum_j y[con_b[j]] A_j[e] for every block b (jtprod_dense_kernel
+// over the block's whole constraint range)
+__global__ __launch_bounds__(256) void jtprod_tiles_kernel(const DenseBlkDev* blks, int nblk,
+                                                           const double* y, int which) {
+  const DenseBlkDev& b = blks[dense_blk_of(blks, nblk, blockIdx.x, true)];
+  const int64_t n2 = b.n2;
+  const int64_t e = (int64_t)(blockIdx.x - b.tile0) * 256 + threadIdx.x;
+  if (e >= n2) return;
+  const int mi = b.ncon;
+  const int* dcon = b.con_list;
+  const double* mats = b.mats;
+  double s0 = 0.0, s1 = 0.0;
+  int j = 0;
+  for (; j + 1 < mi; j += 2) {
+    s0 += y[dcon[j]] * mats[(int64_t)j * n2 + e];
+    s1 += y[dcon[j + 1]] * mats[(int64_t)(j + 1) * n2 + e];
+  }
+  if (j < mi) s0 += y[dcon[j]] * mats[(int64_t)j * n2 + e];
+  b.out[which][e] = s0 + s1;
+}
+
+int jprod_dense_batched(hipStream_t st, const DenseBatch& d, int m, int which, double* out) {
+  if (d.npairs == 0) return 0;
+  jprod_pairs_kernel<<<d.npairs, 256, 0, st>>>(d.blks, d.nblk, which, d.part);
+  jprod_gather_kernel<<<cdiv(m, 256), 256, 0, st>>>(m, d.cptr, d.cpairs, d.part, out);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+int jtprod_dense_batched(hipStream_t st, const DenseBatch& d, const double* y, int which) {
+  if (d.ntiles == 0) return 0;
+  jtprod_tiles_kernel<<<d.ntiles, 256, 0, st>>>(d.blks, d.nblk, y, which);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
 int jtprod_split(int n, int ncon) {
   const int64_t n2 = (int64_t)n * n;
   const int tiles = cdiv(n2, 256);

@@ -110,6 +110,21 @@ struct msdp_solver {
   int nb = 0;
   int64_t m = 0, ns = 0;
   std::vector<Blk> blk;
+  // all blocks dense (two or more): A(.) / A*(.) of every block in one go
+  DenseBatch dbatch;
+  std::vector<DenseBlkDev> dbh_blks;
+  std::vector<int> dbh_cptr, dbh_cpairs;
+  int build_dense_batch();
+  // A(.) of every block accumulated into out (which: JP_X, JP_BUF, JP_SI),
+  // in block order
+  int jprod_blocks(int which, double* out) {
+    if (dbatch.nblk > 0) return jprod_dense_batched(st, dbatch, (int)m, which, out);
+    for (auto& bk : blk) {
+      const double* in = which == JP_X ? bk.X : which == JP_BUF ? bk.buf : bk.Si;
+      MSDP_TRY(jprod(st, bk.ops, in, out));
+    }
+    return 0;
+  }
   msdp_options opt;
   bool finalized = false;
   // device vectors
@@ -648,6 +663,7 @@ int msdp_solver::finalize() {
   swork_elems = want;
   if (want) MSDP_TRY(alloc(&swork, want));
   MSDP_TRY(setup_aux());
+  MSDP_TRY(build_dense_batch());
   // norms of the data (src/solver.jl:119-120, 217-221, 322, 330)
   for (int i = 0; i < nb; ++i) MSDP_TRY(max_abs(st, (int64_t)blk[i].n * blk[i].n, blk[i].C, slot(bslot(i, 0)), red));
   if (m > 0) MSDP_TRY(max_abs(st, m, b, slot(0), red));
@@ -678,6 +694,9 @@ int msdp_solver::finalize() {
   std::vector<double>().swap(h_b);
   std::vector<double>().swap(h_dlin);
   std::vector<double>().swap(h_Clin);
+  std::vector<DenseBlkDev>().swap(dbh_blks);
+  std::vector<int>().swap(dbh_cptr);
+  std::vector<int>().swap(dbh_cpairs);
   finalized = true;
   return 0;
 }
@@ -714,6 +733,15 @@ int64_t msdp_solver::footprint() const {
     }
   }
   bytes += swork_b;
+  {  // the batched dense A(.) / A*(.) tables (build_dense_batch)
+    bool all_dense = nb >= 2 && m > 0;
+    int64_t pairs = 0;
+    for (const auto& b : blk) {
+      all_dense = all_dense && b.has_dense && !b.h_dcon.empty();
+      pairs += (int64_t)b.h_dcon.size();
+    }
+    if (all_dense) bytes += nb * (int64_t)sizeof(DenseBlkDev) + (mm + 1) * 4 + pairs * 12 + 64;
+  }
   bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
   bytes += (mm / 512) * (2 * 512 * 512 + (int64_t)trtri_work_elems(512)) * d;  // Hdi, Hdit, Hdiw
@@ -740,6 +768,58 @@ int64_t msdp_solver::footprint() const {
   }
   // the recorder's device argument buffer (grown on demand, 1 MiB at least)
   return bytes + ((int64_t)4 << 20);
+}
+
+// The batched dense A(.) / A*(.) tables (DenseBatch): every block dense, two
+// or more of them.  Pairs (block, constraint) in block order; each
+// constraint's pairs listed in block order.
+int msdp_solver::build_dense_batch() {
+  dbatch = DenseBatch{};
+  if (nb < 2 || m <= 0) return 0;
+  for (auto& b : blk)
+    if (b.ops.kind != BlockOps::DENSE) return 0;
+  dbh_blks.assign(nb, DenseBlkDev{});
+  std::vector<std::vector<int>> lists((size_t)m);
+  int P = 0, T = 0;
+  for (int i = 0; i < nb; ++i) {
+    Blk& b = blk[i];
+    DenseBlkDev& x = dbh_blks[i];
+    x.mats = b.ops.mats;
+    x.n2 = (int64_t)b.n * b.n;
+    x.con_list = b.ops.con_list;
+    x.ncon = b.ops.ncon;
+    x.pair0 = P;
+    x.tile0 = T;
+    x.in[0] = b.X;
+    x.in[1] = b.buf;
+    x.in[2] = b.Si;
+    x.out[0] = b.tmp2;
+    x.out[1] = b.Dc;
+    for (int j = 0; j < x.ncon; ++j) lists[b.h_dcon[j]].push_back(P + j);
+    P += x.ncon;
+    T += cdiv(x.n2, 256);
+  }
+  dbh_cptr.assign((size_t)m + 1, 0);
+  dbh_cpairs.clear();
+  dbh_cpairs.reserve(P);
+  for (int64_t c = 0; c < m; ++c) {
+    dbh_cpairs.insert(dbh_cpairs.end(), lists[c].begin(), lists[c].end());
+    dbh_cptr[c + 1] = (int)dbh_cpairs.size();
+  }
+  DenseBatch d;
+  d.nblk = nb;
+  d.npairs = P;
+  d.ntiles = T;
+  MSDP_TRY(alloc(&d.blks, nb));
+  MSDP_TRY(alloc(&d.cptr, (size_t)m + 1));
+  MSDP_TRY(alloc(&d.cpairs, (size_t)P));
+  MSDP_TRY(alloc(&d.part, (size_t)P));
+  MSDP_HIP(hipMemcpyAsync(d.blks, dbh_blks.data(), sizeof(DenseBlkDev) * nb, hipMemcpyHostToDevice, st));
+  MSDP_HIP(hipMemcpyAsync(d.cptr, dbh_cptr.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
+  if (P > 0)
+    MSDP_HIP(hipMemcpyAsync(d.cpairs, dbh_cpairs.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+  dbatch = d;
+  return 0;
 }
 
 // Copies the solution (X, S, y, x, z) to the host, frees the device state and
@@ -935,17 +1015,19 @@ int msdp_solver::update_residuals() {
   Range range_("madsdp:update_residuals");
   if (m > 0) {
     MSDP_TRY(fill(st, m, 0.0, Rp));
-    for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.X, Rp));
+    MSDP_TRY(jprod_blocks(JP_X, Rp));
     if (ns > 0) MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, x, ns, 1.0, Rp, m));
     MSDP_TRY(axpby(st, m, 1.0, b, -1.0, Rp, Rp));
   }
+  const bool jt_batched = dbatch.nblk > 0;
+  if (jt_batched) MSDP_TRY(jtprod_dense_batched(st, dbatch, y, JT_TMP2));
   MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
     Blk& bk = blk[i];
     const int64_t n2 = (int64_t)bk.n * bk.n;
     Blk* bp = &bk;
-    MSDP_TRY(opaque([=] { return jtprod(st, bp->ops, y, bp->tmp2); }));
+    if (!jt_batched) MSDP_TRY(opaque([=] { return jtprod(st, bp->ops, y, bp->tmp2); }));
     MSDP_TRY(axpbypcz(st, n2, 1.0, bk.C, -1.0, bk.S, -1.0, bk.tmp2, bk.Rd));
     MSDP_TRY(max_abs(st, n2, bk.Rd, slot(bslot(i, 1)), red));
   }
@@ -1230,7 +1312,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     }
     MSDP_TRY(rec_end());
     // A(.) accumulates every block into dely: in block order on the main stream
-    for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.buf, dely));
+    MSDP_TRY(jprod_blocks(JP_BUF, dely));
     if (ns > 0) {
       MSDP_TRY(scalar_op(st, SOP_RHS, (int)ns, mu_, x, z, zinv, Rdlin, corrlin, ws1, nullptr, corrected));
       MSDP_TRY(gemm(st, false, false, (int)m, 1, (int)ns, 1.0, Clin, m, ws1, ns, 1.0, dely, m));
@@ -1238,6 +1320,8 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     MSDP_TRY(schur_solve(dely));
   }
   // directions
+  const bool jt_batched = dbatch.nblk > 0;
+  if (jt_batched) MSDP_TRY(jtprod_dense_batched(st, dbatch, dely, JT_TMP2));
   MSDP_TRY(rec_begin());
   for (int i = 0; i < nb; ++i) {
     OnBlock on(this, i);
@@ -1245,7 +1329,7 @@ int msdp_solver::solve_directions(double mu_, bool corrected) {
     const int n = bk.n;
     const int64_t n2 = (int64_t)n * n;
     Blk* bp = &bk;
-    MSDP_TRY(opaque([=] { return jtprod(st, bp->ops, dely, bp->tmp2); }));
+    if (!jt_batched) MSDP_TRY(opaque([=] { return jtprod(st, bp->ops, dely, bp->tmp2); }));
     MSDP_TRY(axpby(st, n2, 1.0, bk.Rd, -1.0, bk.tmp2, bk.delS));
     MSDP_TRY(symmetrize(st, n, bk.delS, n));
     if (bk.ops.kind == BlockOps::EMPTY) MSDP_TRY(fill(st, n2, 0.0, bk.buf2));
@@ -1662,7 +1746,7 @@ int msdp_solver::refresh_slacks(bool& ok) {  // :83-86
 
 int msdp_solver::newton_direction(bool& ok) {  // :89-110
   MSDP_TRY(fill(st, m, 0.0, wm));
-  for (auto& bk : blk) MSDP_TRY(jprod(st, bk.ops, bk.Si, wm));
+  MSDP_TRY(jprod_blocks(JP_SI, wm));
   MSDP_TRY(axpby(st, m, 1.0, wm, -dc_t, b, dc_g));
   MSDP_TRY(factorize_schur(ok));
   if (!ok) return 0;
