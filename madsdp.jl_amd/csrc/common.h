@@ -248,6 +248,10 @@ int symmetrize2(hipStream_t s, int n, double* A0, int64_t sA, int64_t lda);
 // launch; *info = 0, or the 1-based first non-positive pivot
 int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
                    int* info);
+// the same for 64 < n <= 128 in one launch of one workgroup (recorded like
+// cholesky_small); info = the 1-based failing column, 0 on success
+int cholesky_mid(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
+                 int* info);
 int mirror_lower(hipStream_t s, int n, double* A, int64_t lda);  // upper := lower'
 int copy_mat(hipStream_t s, int rows, int cols, const double* A, int64_t lda, double* B,
              int64_t ldb);
@@ -315,7 +319,7 @@ constexpr int JACOBI_VB_MAX = 128;
 // --------------------------------------------------------------------------
 enum RecKind {
   REC_OPAQUE = 0, REC_GEMM_NN, REC_GEMM_TN, REC_GEMM_NT, REC_GEMM_TT, REC_EW, REC_RED,
-  REC_CHOL64, REC_TRTRI64, REC_NKINDS
+  REC_CHOL64, REC_TRTRI64, REC_CHOL128, REC_NKINDS
 };
 struct GemmItem {
   int M, N, K, lower, tri;

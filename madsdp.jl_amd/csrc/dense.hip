@@ -1828,6 +1828,122 @@ __global__ __launch_bounds__(64) void chol_small_vb_kernel(const CholItem* __res
   chol_small_body(it.n, it.M, it.ldm, it.L, it.ldl, it.info);
 }
 
+// _cholesky_factor! for 64 < n <= 128 in one workgroup of two wavefronts:
+// the symmetrised copy; the first 64 columns factored (wavefront 0) with the
+// rows below substituted one column behind (wavefront 1), as in
+// chol_diag_panel_kernel; the trailing block updated (A22 -= L21 L21',
+// rank-1 steps in k order) and factored by wavefront 1; then the flush of
+// entries below eps^2 max|L_ii| and the zeroed upper triangle.  One launch
+// where the blocked path takes nine.
+__device__ __attribute__((always_inline)) inline void chol_mid_body(int n, const double* M,
+                                                                    int64_t ldm, double* L,
+                                                                    int64_t ldl, int* info) {
+  __shared__ double Lt1[64 * W64_LD];  // L11 by columns: Lt1[k * W64_LD + j] = L[j][k]
+  __shared__ double Lt2[64 * W64_LD];  // L22, the same layout
+  __shared__ double R21[64 * W64_LD];  // L21 by rows: R21[i * W64_LD + k] = L[64 + i][k]
+  __shared__ double s_dm[2];
+  __shared__ int s_fail;
+  const int lane = threadIdx.x & 63;
+  const bool w0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+  const int n2 = n - 64;     // order of the trailing block, 1..64
+  const int i2 = 64 + lane;  // wavefront 1's row
+  double v[64];
+  if (w0) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      v[c] = c <= lane ? (M[lane + (int64_t)c * ldm] + M[c + (int64_t)lane * ldm]) / 2 : 0.0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+      v[k] = lane < n2 ? (M[i2 + (int64_t)k * ldm] + M[k + (int64_t)i2 * ldm]) / 2 : 0.0;
+  }
+  if (threadIdx.x == 0) s_fail = 0;
+  int fail = 0;
+  chol_pipe_step<0>(w0, v, 64, Lt1, fail);
+  if (w0) {
+    if (fail && lane == 0) s_fail = fail;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 64; ++k) R21[lane * W64_LD + k] = v[k];
+  }
+  __syncthreads();
+  if (s_fail) {
+    if (threadIdx.x == 0) *info = s_fail;
+    return;
+  }
+  if (!w0) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      v[c] = (lane < n2 && c <= lane)
+                 ? (M[i2 + (int64_t)(64 + c) * ldm] + M[(64 + c) + (int64_t)i2 * ldm]) / 2
+                 : 0.0;
+    for (int k = 0; k < 64; ++k) {
+      const double x = R21[lane * W64_LD + k];
+#pragma unroll
+      for (int c = 0; c < 64; ++c) v[c] -= x * R21[c * W64_LD + k];
+    }
+    fail = chol64_rows(v, n2, Lt2);
+    if (fail && lane == 0) s_fail = 64 + fail;
+  }
+  __syncthreads();
+  if (s_fail) {
+    if (threadIdx.x == 0) *info = s_fail;
+    return;
+  }
+  if (threadIdx.x == 0) *info = 0;
+  double dm = w0 ? fabs(Lt1[lane * W64_LD + lane]) : (lane < n2 ? fabs(Lt2[lane * W64_LD + lane]) : 0.0);
+  dm = wave_max(dm);
+  if (lane == 0) s_dm[threadIdx.x >> 6] = dm;
+  __syncthreads();
+  const double eps = 2.220446049250313e-16;
+  const double tol = eps * eps * fmax(s_dm[0], s_dm[1]);
+  auto flushed = [tol](double x, bool diag) { return diag ? x : (fabs(x) < tol ? 0.0 : x); };
+  if (w0) {
+    for (int c = 0; c < 64; ++c) {
+      const double x = Lt1[c * W64_LD + lane];
+      L[lane + (int64_t)c * ldl] = lane >= c ? flushed(x, lane == c) : 0.0;
+    }
+    for (int c = 64; c < n; ++c) L[lane + (int64_t)c * ldl] = 0.0;
+  } else if (lane < n2) {
+    for (int c = 0; c < 64; ++c)
+      L[i2 + (int64_t)c * ldl] = flushed(R21[lane * W64_LD + c], false);
+    for (int c = 0; c < n2; ++c) {
+      const double x = Lt2[c * W64_LD + lane];
+      L[i2 + (int64_t)(64 + c) * ldl] = lane >= c ? flushed(x, lane == c) : 0.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) void chol_mid_kernel(
+    int n, const double* M, int64_t ldm, double* L, int64_t ldl, int* info) {
+  chol_mid_body(n, M, ldm, L, ldl, info);
+}
+
+// a batch of blocks of different sizes, one workgroup each
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+chol_mid_vb_kernel(const CholItem* __restrict__ items) {
+  const CholItem it = items[blockIdx.x];
+  chol_mid_body(it.n, it.M, it.ldm, it.L, it.ldl, it.info);
+}
+
+int cholesky_mid(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
+                 int* info) {
+  if (n <= 64 || n > 128) {
+    set_error("cholesky_mid: n must be in 65..128");
+    return -2;
+  }
+  if (rec_active()) {
+    RecOp o;
+    o.kind = REC_CHOL128;
+    o.ch = CholItem{n, 0, M, L, info, ldm, ldl};
+    o.run = [=] { return cholesky_mid(s, n, M, ldm, L, ldl, info); };
+    return rec_add(std::move(o));
+  }
+  chol_mid_kernel<<<1, 128, 0, s>>>(n, M, ldm, L, ldl, info);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
 int cholesky_small(hipStream_t s, int n, const double* M, int64_t ldm, double* L, int64_t ldl,
                    int* info) {
   if (n <= 0 || n > 64) {
@@ -2245,6 +2361,7 @@ static size_t rec_item_bytes(int kind) {
     case REC_EW: return sizeof(EwItem);
     case REC_RED: return sizeof(RedItem);
     case REC_CHOL64: return sizeof(CholItem);
+    case REC_CHOL128: return sizeof(CholItem);
     case REC_TRTRI64: return sizeof(TrtriItem);
     default: return sizeof(GemmItem);
   }
@@ -2274,7 +2391,8 @@ size_t rec_pack(int kind, const std::vector<const RecOp*>& items, char* host, in
         break;
       }
       case REC_RED: std::memcpy(host + k * ib, &o.r, ib); break;
-      case REC_CHOL64: std::memcpy(host + k * ib, &o.ch, ib); break;
+      case REC_CHOL64:
+      case REC_CHOL128: std::memcpy(host + k * ib, &o.ch, ib); break;
       case REC_TRTRI64: std::memcpy(host + k * ib, &o.tr, ib); break;
       default:
         std::memcpy(host + k * ib, &o.g, ib);
@@ -2302,6 +2420,9 @@ int rec_launch(hipStream_t s, int kind, int count, int grid, const char* dev) {
       break;
     case REC_CHOL64:
       chol_small_vb_kernel<<<count, 64, 0, s>>>(reinterpret_cast<const CholItem*>(dev));
+      break;
+    case REC_CHOL128:
+      chol_mid_vb_kernel<<<count, 128, 0, s>>>(reinterpret_cast<const CholItem*>(dev));
       break;
     case REC_TRTRI64:
       trtri64_vb_kernel<<<count, 64, 0, s>>>(reinterpret_cast<const TrtriItem*>(dev));

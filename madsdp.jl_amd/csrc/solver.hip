@@ -866,6 +866,7 @@ int msdp_solver::keep_solution() {
 int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) {
   const int n = bk.n;
   if (n <= 64) return cholesky_small(st, n, M, n, L, n, inf);
+  if (n <= 128) return cholesky_mid(st, n, M, n, L, n, inf);
   return opaque([=]() -> int {
     MSDP_TRY(copy_mat(st, n, n, M, n, L, n));
     MSDP_TRY(symmetrize(st, n, L, n));
@@ -880,6 +881,10 @@ int msdp_solver::cholesky_factor(Blk& bk, const double* M, double* L, int* inf) 
 int msdp_solver::cholesky_factor2(Blk& bk, const double* M0, double* L0, const double* M1,
                                   double* L1, int* inf) {
   const int n = bk.n;
+  if (n > 64 && n <= 128) {
+    MSDP_TRY(cholesky_mid(st, n, M0, n, L0, n, inf));
+    return cholesky_mid(st, n, M1, n, L1, n, inf + 1);
+  }
   return opaque([=]() -> int {
     MSDP_TRY(copy_mat(st, n, n, M0, n, L0, n));
     MSDP_TRY(symmetrize(st, n, L0, n));
@@ -2660,6 +2665,7 @@ int msdp_k_cholesky_factor(void* stream, int32_t n, const double* M, int64_t ldm
                            int32_t* info, double* scratch) {
   hipStream_t st = (hipStream_t)stream;
   if (n >= 1 && n <= 64) return cholesky_small(st, n, M, ldm, L, ldl, info);
+  if (n > 64 && n <= 128) return cholesky_mid(st, n, M, ldm, L, ldl, info);
   MSDP_TRY(copy_mat(st, n, n, M, ldm, L, ldl));
   MSDP_TRY(symmetrize(st, n, L, ldl));
   MSDP_TRY(potrf(st, n, L, ldl, info, scratch + reduce_scratch_elems(0) + 64));

@@ -201,7 +201,7 @@ def test_panel_apply(env, rows, cols, acc):
     assert np.all(np.abs(do.cpu().numpy() - ref) <= 1e-13 * (rows + cols) * scale)
 
 
-@pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 300])
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 100, 128, 200, 300])
 def test_cholesky_factor(env, n):
     torch, lib, st = env
     rng = np.random.default_rng(n)
@@ -230,6 +230,22 @@ def test_cholesky_not_pd(env):
         assert lib.msdp_k_cholesky_factor(st, 2, P(dM), 2, P(dL), 2, P(info), P(scratch)) == 0
         torch.cuda.synchronize()
         assert int(info.item()) == pivot
+
+
+@pytest.mark.parametrize("n,bad", [(100, 10), (100, 70), (128, 127), (65, 64)])
+def test_cholesky_mid_not_pd(env, n, bad):
+    """65 <= n <= 128 (one workgroup): a negative pivot in the first 64
+    columns or in the trailing block is reported by its 1-based column."""
+    torch, lib, st = env
+    M = np.eye(n)
+    M[bad, bad] = -1.0
+    dM, dL = dev(torch, M), dev(torch, np.zeros((n, n)))
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    scratch = torch.zeros(int(lib.msdp_k_cholesky_factor_scratch(n)), dtype=torch.float64, device="cuda")
+    print(f"cholesky not pd n={n} bad={bad}", flush=True)
+    assert lib.msdp_k_cholesky_factor(st, n, P(dM), n, P(dL), n, P(info), P(scratch)) == 0
+    torch.cuda.synchronize()
+    assert int(info.item()) == bad + 1
 
 
 def test_cholesky_flush(env):
