@@ -319,7 +319,7 @@ constexpr int JACOBI_VB_MAX = 128;
 // --------------------------------------------------------------------------
 enum RecKind {
   REC_OPAQUE = 0, REC_GEMM_NN, REC_GEMM_TN, REC_GEMM_NT, REC_GEMM_TT, REC_EW, REC_RED,
-  REC_CHOL64, REC_TRTRI64, REC_CHOL128, REC_NKINDS
+  REC_CHOL64, REC_TRTRI64, REC_CHOL128, REC_TRTRI128, REC_NKINDS
 };
 struct GemmItem {
   int M, N, K, lower, tri;

@@ -1507,6 +1507,93 @@ __global__ __launch_bounds__(64) void trtri64_vb_kernel(const TrtriItem* __restr
   trtri64_body(it.n, it.L, it.ldl, it.Li, it.ldi, 0);
 }
 
+// L^-1 for 64 < n <= 128 in one workgroup of two wavefronts: the inverses
+// of the two diagonal blocks (wavefront 0: L11, wavefront 1: L22, each by
+// trtri64_body's column substitution), then W = L21 L11^-1 and
+// X21 = -L22^-1 W (each wavefront half of the 64 columns, a row per lane),
+// and the whole factor written with a zero upper triangle.  One launch
+// where the recursive path takes four.
+__device__ __attribute__((always_inline)) inline void trtri_mid_body(int n, const double* L,
+                                                                     int64_t ldl, double* Li,
+                                                                     int64_t ldi) {
+  __shared__ double T1[64 * W64_LD];  // L11, then L11^-1 by rows: T1[r * W64_LD + c]
+  __shared__ double T2[64 * W64_LD];  // L22, then L22^-1 by rows
+  __shared__ double Wl[64 * W64_LD];  // W by columns: Wl[k * W64_LD + i] = W[i][k]
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n2 = n - 64;
+  const int b0 = w == 0 ? 0 : 64, nb = w == 0 ? 64 : n2;
+  double* T = w == 0 ? T1 : T2;
+  {
+    double x[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+      x[j] = (lane < nb && j < nb && lane >= j) ? L[(b0 + lane) + (int64_t)(b0 + j) * ldl] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) T[j * W64_LD + lane] = x[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 64; ++i) x[i] = 0.0;
+    trtri64_step<0>(T, nb, x);
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 64; ++i) T[i * W64_LD + lane] = x[i];
+  }
+  __syncthreads();
+  // W[i][k] = sum_j L21[i][j] L11^-1[j][k] for this wavefront's columns k
+  const int k0 = 32 * w;
+  {
+    double l[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) l[j] = lane < n2 ? L[(64 + lane) + (int64_t)j * ldl] : 0.0;
+    double acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+#pragma unroll
+      for (int q = 0; q < 32; ++q) acc[q] = fma(l[j], T1[j * W64_LD + k0 + q], acc[q]);
+#pragma unroll
+    for (int q = 0; q < 32; ++q) Wl[(k0 + q) * W64_LD + lane] = acc[q];
+  }
+  __syncthreads();
+  // X21[i][k] = -sum_j L22^-1[i][j] W[j][k]
+  if (lane < n2) {
+    double acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+      const double a = T2[lane * W64_LD + j];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) acc[q] = fma(a, Wl[(k0 + q) * W64_LD + j], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 32; ++q) Li[(64 + lane) + (int64_t)(k0 + q) * ldi] = -acc[q];
+  }
+  // the diagonal blocks and the zero upper triangle: wavefront 0 rows 0..63,
+  // wavefront 1 rows 64..n-1
+  if (w == 0) {
+    for (int c = 0; c < n; ++c)
+      Li[lane + (int64_t)c * ldi] = (c < 64 && c <= lane) ? T1[lane * W64_LD + c] : 0.0;
+  } else if (lane < n2) {
+    for (int c = 0; c < n2; ++c)
+      Li[(64 + lane) + (int64_t)(64 + c) * ldi] = c <= lane ? T2[lane * W64_LD + c] : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) void trtri_mid_kernel(
+    int n, const double* L, int64_t ldl, int64_t sL, double* Li, int64_t ldi, int64_t sLi) {
+  trtri_mid_body(n, L + (int64_t)blockIdx.x * sL, ldl, Li + (int64_t)blockIdx.x * sLi, ldi);
+}
+
+// recorded 64 < n <= 128 inverses of different blocks, one workgroup each
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+trtri_mid_vb_kernel(const TrtriItem* __restrict__ items) {
+  const TrtriItem it = items[blockIdx.x];
+  trtri_mid_body(it.n, it.L, it.ldl, it.Li, it.ldi);
+}
+
 __global__ void zero_upper_kernel(int n, double* A, int64_t lda, int64_t sA) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   A += (int64_t)blockIdx.z * sA;
@@ -1653,6 +1740,24 @@ int trtri_lower_batched(hipStream_t s, int n, int count, const double* L, int64_
       return 0;
     };
     return rec_add(std::move(t));
+  }
+  if (n > 64 && n <= 128) {  // one workgroup per matrix
+    if (rec_active() && count == 1) {
+      RecOp t;
+      t.kind = REC_TRTRI128;
+      t.tr = TrtriItem{n, 0, L, Li, ldl, ldi};
+      t.run = [=]() -> int {
+        trtri_mid_kernel<<<1, 128, 0, s>>>(n, L, ldl, 0, Li, ldi, 0);
+        MSDP_HIP(hipGetLastError());
+        return 0;
+      };
+      return rec_add(std::move(t));
+    }
+    MSDP_TRY(rec_forbid("trtri"));
+    if (count <= 0) return 0;
+    trtri_mid_kernel<<<count, 128, 0, s>>>(n, L, ldl, sL, Li, ldi, sLi);
+    MSDP_HIP(hipGetLastError());
+    return 0;
   }
   MSDP_TRY(rec_forbid("trtri"));
   if (n <= 0 || count <= 0) return 0;
@@ -2363,6 +2468,7 @@ static size_t rec_item_bytes(int kind) {
     case REC_CHOL64: return sizeof(CholItem);
     case REC_CHOL128: return sizeof(CholItem);
     case REC_TRTRI64: return sizeof(TrtriItem);
+    case REC_TRTRI128: return sizeof(TrtriItem);
     default: return sizeof(GemmItem);
   }
 }
@@ -2393,7 +2499,8 @@ size_t rec_pack(int kind, const std::vector<const RecOp*>& items, char* host, in
       case REC_RED: std::memcpy(host + k * ib, &o.r, ib); break;
       case REC_CHOL64:
       case REC_CHOL128: std::memcpy(host + k * ib, &o.ch, ib); break;
-      case REC_TRTRI64: std::memcpy(host + k * ib, &o.tr, ib); break;
+      case REC_TRTRI64:
+      case REC_TRTRI128: std::memcpy(host + k * ib, &o.tr, ib); break;
       default:
         std::memcpy(host + k * ib, &o.g, ib);
         wgs = cdiv(o.g.M, GT) * cdiv(o.g.N, GT);
@@ -2426,6 +2533,9 @@ int rec_launch(hipStream_t s, int kind, int count, int grid, const char* dev) {
       break;
     case REC_TRTRI64:
       trtri64_vb_kernel<<<count, 64, 0, s>>>(reinterpret_cast<const TrtriItem*>(dev));
+      break;
+    case REC_TRTRI128:
+      trtri_mid_vb_kernel<<<count, 128, 0, s>>>(reinterpret_cast<const TrtriItem*>(dev));
       break;
     case REC_GEMM_NN:
       gemm_vb_kernel<false, false><<<grid, 256, 0, s>>>(reinterpret_cast<const GemmItem*>(dev), start, count);

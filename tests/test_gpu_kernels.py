@@ -265,7 +265,7 @@ def test_cholesky_flush(env):
     np.testing.assert_allclose(Lh @ Lh.T, M, atol=1e-15)
 
 
-@pytest.mark.parametrize("n", [1, 7, 64, 65, 130, 333])
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 100, 128, 130, 333])
 def test_trtri(env, n):
     torch, lib, st = env
     rng = np.random.default_rng(n + 11)
