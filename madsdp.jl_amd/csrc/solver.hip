@@ -903,7 +903,7 @@ int msdp_solver::prepare_block(Blk& bk, int i, bool factored) {
   const int n = bk.n;
   if (!factored) MSDP_TRY(cholesky_factor(bk, bk.S, bk.Ls, binfo(i, 0)));
   Blk* bp = &bk;
-  if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));  // recorded
+  if (n <= 128) MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));  // recorded
   else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Ls, n, bp->Lsinv, n, trwork); }));
   return inverse_product(bk);
 }
@@ -1077,7 +1077,7 @@ int msdp_solver::prepare(bool& ok) {
       MSDP_TRY(prepare_block(bk, i, false));
       if (need_lx) {
         MSDP_TRY(cholesky_factor(bk, bk.X, bk.Lx, binfo(i, 1)));
-        if (n <= 64) MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));  // recorded
+        if (n <= 128) MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));  // recorded
         else MSDP_TRY(opaque([=] { return trtri_lower(st, n, bp->Lx, n, bp->Lxinv, n, trwork); }));
       }
     }
