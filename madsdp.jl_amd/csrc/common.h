@@ -195,6 +195,19 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
          const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
          double* C, int64_t ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0,
          int64_t sC = 0, bool lower = false, int tri = 0);
+// One recorded op (the recorder must be active) of `batch` products at the
+// given strides, whatever their sizes: one 64-tile item of the
+// variable-size batched launch per position (the dense Schur assembly of
+// many blocks).
+int gemm_vb_record(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                   double* C, int64_t ldc, int batch, int64_t sA, int64_t sB, int64_t sC,
+                   bool lower);
+// One recorded op: B[:, j] = svec(lower triangle of P_j) for j < count
+// (P_j n x n at j n^2; the order of svec_pack_kernel)
+int svec_record(hipStream_t s, int n, int count, const double* P, double* B, int64_t ldb);
+// the same as one launch (ops.hip)
+int svec_pack(hipStream_t s, int n, int count, const double* P, double* B, int64_t ldb);
 // the same product with its partials in the library's per-device scratch
 // array (no allocation; calls that overlap on one device race)
 int gemm_noalloc(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
@@ -328,10 +341,13 @@ struct GemmItem {
   const double* B;
   double* C;
   int64_t lda, ldb, ldc;
+  int batch = 1;  // `batch` products at the strides below (gemm_vb_record)
+  int64_t sA = 0, sB = 0, sC = 0;
 };
 enum EwOp {
   EW_AXPBY = 0, EW_AXPBYPCZ, EW_FILL, EW_COPY, EW_SYM, EW_MIRROR, EW_TRANSPOSE, EW_SCALE_COLS,
-  EW_SCALE_ROWS, EW_RNT, EW_SCALE_BY_DIAG, EW_ZERO_UPPER
+  EW_SCALE_ROWS, EW_RNT, EW_SCALE_BY_DIAG, EW_ZERO_UPPER,
+  EW_SVEC  // Z[:, j] = svec of the lower triangle of X_j (n = p, X_j at j n^2, ld ldb)
 };
 struct EwItem {
   int op, rows, cols, p;

@@ -189,10 +189,14 @@ __global__ __launch_bounds__(256) void gemm_vb_kernel(const GemmItem* __restrict
     if (start[mid] <= w) lo = mid; else hi = mid - 1;
   }
   const GemmItem it = items[lo];
-  const int t = w - start[lo];
+  int t = w - start[lo];
   const int ntm = (it.M + GT - 1) / GT;
-  gemm64_tile<TA, TB>(it.M, it.N, it.K, it.alpha, it.A, it.lda, it.B, it.ldb, it.beta, it.C,
-                      it.ldc, it.lower, t % ntm, t / ntm, it.tri);
+  const int per = ntm * ((it.N + GT - 1) / GT);
+  const int64_t bz = t / per;  // the item's product (batched items)
+  t -= (int)bz * per;
+  gemm64_tile<TA, TB>(it.M, it.N, it.K, it.alpha, it.A + bz * it.sA, it.lda, it.B + bz * it.sB,
+                      it.ldb, it.beta, it.C + bz * it.sC, it.ldc, it.lower, t % ntm, t / ntm,
+                      it.tri);
 }
 
 // Large-tile GEMM: 128x128 output tile per workgroup of 4 waves, each wave
@@ -2274,6 +2278,39 @@ __global__ void fill_kernel(int64_t count, double v, double* X) {
     X[i] = v;
 }
 
+int gemm_vb_record(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                   double* C, int64_t ldc, int batch, int64_t sA, int64_t sB, int64_t sC,
+                   bool lower) {
+  if (!rec_active()) {
+    set_error("gemm_vb_record: no recorded region");
+    return -2;
+  }
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
+  RecOp o;
+  o.kind = ta ? (tb ? REC_GEMM_TT : REC_GEMM_TN) : (tb ? REC_GEMM_NT : REC_GEMM_NN);
+  o.g = GemmItem{M, N, K, lower ? 1 : 0, 0, alpha, beta, A, B, C, lda, ldb, ldc, batch, sA, sB,
+                 sC};
+  o.run = [=] {
+    return gemm(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC, lower);
+  };
+  return rec_add(std::move(o));
+}
+
+int svec_record(hipStream_t s, int n, int count, const double* P, double* B, int64_t ldb) {
+  if (!rec_active()) {
+    set_error("svec_record: no recorded region");
+    return -2;
+  }
+  const int64_t d = (int64_t)n * (n + 1) / 2;
+  if (d <= 0 || count <= 0) return 0;
+  RecOp o;
+  o.kind = REC_EW;
+  o.e = EwItem{EW_SVEC, (int)d, count, n, 0, 0, ldb, 0, 0, 0, P, nullptr, nullptr, B};
+  o.run = [=] { return svec_pack(s, n, count, P, B, ldb); };
+  return rec_add(std::move(o));
+}
+
 int fill(hipStream_t s, int64_t count, double v, double* X) {
   if (count <= 0) return 0;
   if (rec_active()) {
@@ -2420,6 +2457,19 @@ __global__ __launch_bounds__(256) void ew_vb_kernel(const EwItem* __restrict__ i
           it.Z[i + (int64_t)j * n] = it.X[i + (int64_t)j * n] * it.Y[j + (int64_t)j * n];
           break;
         }
+        case EW_SVEC: {  // entry i of svec(X_j): column c, row r >= c of the lower triangle
+          const int64_t n = it.p;
+          auto off = [n](int64_t c) { return c * n - c * (c - 1) / 2; };
+          const double b2 = 2.0 * n + 1.0;
+          int64_t c = (int64_t)((b2 - sqrt(fmax(b2 * b2 - 8.0 * (double)i, 0.0))) * 0.5);
+          c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+          while (c + 1 < n && off(c + 1) <= i) ++c;
+          while (off(c) > i) --c;
+          const int64_t r = c + (i - off(c));
+          const double v = it.X[(int64_t)j * n * n + r + c * n];
+          it.Z[i + (int64_t)j * it.ldb] = r == c ? v : 1.4142135623730951 * v;
+          break;
+        }
         default: break;
       }
     }
@@ -2503,7 +2553,7 @@ size_t rec_pack(int kind, const std::vector<const RecOp*>& items, char* host, in
       case REC_TRTRI128: std::memcpy(host + k * ib, &o.tr, ib); break;
       default:
         std::memcpy(host + k * ib, &o.g, ib);
-        wgs = cdiv(o.g.M, GT) * cdiv(o.g.N, GT);
+        wgs = cdiv(o.g.M, GT) * cdiv(o.g.N, GT) * std::max(1, o.g.batch);
         break;
     }
     start[k] = total;

@@ -65,6 +65,9 @@ int scatter_lower_add(hipStream_t st, int mi, const int* con, const double* Hb, 
                       int64_t ldh);
 int schur_block(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* H, int64_t m,
                 double* work, size_t work_elems);
+// the block's contribution as recorded ops into its Hbuf (T, P: mi n^2 each)
+int dense_schur_record(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* T,
+                       double* P);
 // rows of a dense block's scaled constraint panel: n(n+1)/2 (svec) or n^2 (vec)
 int64_t dense_panel_rows(int n, bool full_vec);
 // b.Bbuf[:, j] = svec(P_j) (vec for HKM) in FP64, P_j = op(Lm) A_j op(F2)

@@ -280,6 +280,37 @@ __global__ void svec_pack_kernel(int n, int cb, const double* P, double* B, int6
   B[k + (int64_t)j * ldb] = r == c ? v : 1.4142135623730951 * v;
 }
 
+int svec_pack(hipStream_t s, int n, int count, const double* P, double* B, int64_t ldb) {
+  const int64_t d = (int64_t)n * (n + 1) / 2;
+  if (d <= 0 || count <= 0) return 0;
+  svec_pack_kernel<<<dim3(cdiv(d, 256), count), 256, 0, s>>>(n, count, P, B, ldb);
+  MSDP_HIP(hipGetLastError());
+  return 0;
+}
+
+// The dense Schur contribution of one block as recorded ops (the recorder
+// batches each across the blocks): T_j = A_j op(F2) and P_j = op(Lm) T_j
+// (its lower triangle) for all the block's constraints at once, B[:, j] =
+// svec(P_j), Hbuf = B'B (lower triangle), T and P the block's own
+// mi n^2 buffers.  NT and dual-centering scalings (svec panels).
+int dense_schur_record(hipStream_t st, const BlockOps& b, const SchurScaling& sc, double* T,
+                       double* P) {
+  const int n = b.n, mi = b.ncon;
+  const int64_t n2 = (int64_t)n * n;
+  const int64_t drow = dense_panel_rows(n, false);
+  if (sc.full_vec || b.kind != BlockOps::DENSE || mi <= 0 || b.dcon_identity) {
+    set_error("dense_schur_record: a dense block of constraints with svec panels only");
+    return -2;
+  }
+  MSDP_TRY(gemm_vb_record(st, false, sc.transF2, n, n, n, 1.0, b.mats, n, sc.F2, n, 0.0, T, n, mi,
+                          n2, 0, n2, false));
+  MSDP_TRY(gemm_vb_record(st, sc.transL, false, n, n, n, 1.0, sc.Lm, n, T, n, 0.0, P, n, mi, 0, n2,
+                          n2, true));
+  MSDP_TRY(svec_record(st, n, mi, P, b.Bbuf, drow));
+  return gemm_vb_record(st, true, false, mi, mi, (int)drow, 1.0, b.Bbuf, drow, b.Bbuf, drow, 0.0,
+                        b.Hbuf, mi, 1, 0, 0, 0, true);
+}
+
 // Low-rank: H[j,k] += sum_{r in j, s in k} lam_r lam_s PL[r,s] PR[s,r] with
 // PL = F' L F and PR = F' R F (R x R, ld R)
 __global__ void schur_lowrank_kernel(int ncon, const int* con_list, const int* con_ptr,

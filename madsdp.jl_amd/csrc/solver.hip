@@ -49,6 +49,9 @@ struct Blk {
   double cnorm = 0.0;
   double *X, *S, *Rd, *Ls, *Si, *Lsinv, *delX, *delS, *Lx, *Lxinv, *corr, *buf, *buf2, *tmp,
       *tmp2, *W, *G, *Gi, *D, *RNT, *Mm, *Nm, *Vv, *lam, *Shat, *Dc, *Lh;
+  // T_j and P_j of all the block's constraints (the recorded Schur assembly
+  // of many dense blocks), mi n^2 each
+  double *asmT = nullptr, *asmP = nullptr;
   bool rd_nonzero = true;
   // mixed precision: FP32 copies of the dense constraints, the scaled
   // constraint panel and the scaling factors
@@ -112,6 +115,7 @@ struct msdp_solver {
   std::vector<Blk> blk;
   // all blocks dense (two or more): A(.) / A*(.) of every block in one go
   DenseBatch dbatch;
+  static constexpr size_t ASM_BATCH_MAX_BYTES = (size_t)2 << 30;
   std::vector<DenseBlkDev> dbh_blks;
   std::vector<int> dbh_cptr, dbh_cpairs;
   int build_dense_batch();
@@ -740,7 +744,16 @@ int64_t msdp_solver::footprint() const {
       all_dense = all_dense && b.has_dense && !b.h_dcon.empty();
       pairs += (int64_t)b.h_dcon.size();
     }
-    if (all_dense) bytes += nb * (int64_t)sizeof(DenseBlkDev) + (mm + 1) * 4 + pairs * 12 + 64;
+    if (all_dense) {
+      bytes += nb * (int64_t)sizeof(DenseBlkDev) + (mm + 1) * 4 + pairs * 12 + 64;
+      int64_t asm_b = 0;
+      bool small = true;
+      for (const auto& b : blk) {
+        asm_b += 2 * (int64_t)b.h_dcon.size() * b.n * b.n * d;
+        small = small && b.n <= JACOBI_VB_MAX;
+      }
+      if (small && asm_b <= (int64_t)ASM_BATCH_MAX_BYTES) bytes += asm_b;
+    }
   }
   bytes += (3 * mm * mm + 2 * mm * nss + 12 * mm + 9 * nss) * d;
   bytes += (int64_t)potrf_work_elems((int)mm) * d;
@@ -819,6 +832,18 @@ int msdp_solver::build_dense_batch() {
   if (P > 0)
     MSDP_HIP(hipMemcpyAsync(d.cpairs, dbh_cpairs.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
   dbatch = d;
+  // the recorded Schur assembly's per-block products, when the blocks are
+  // recorded (multi) and they fit a modest budget
+  if (multi) {
+    size_t tot = 0;
+    for (auto& b : blk) tot += 2 * (size_t)b.ops.ncon * b.n * b.n;
+    if (tot * sizeof(double) <= ASM_BATCH_MAX_BYTES)
+      for (auto& b : blk) {
+        const size_t e = (size_t)b.ops.ncon * b.n * b.n;
+        MSDP_TRY(alloc(&b.asmT, e));
+        MSDP_TRY(alloc(&b.asmP, e));
+      }
+  }
   return 0;
 }
 
@@ -1203,16 +1228,32 @@ int msdp_solver::factorize_schur(bool& ok) {
   auto deferred = [&](const Blk& bk) {
     return split && bk.ops.kind == BlockOps::DENSE && !bk.ops.dcon_identity && bk.ops.ncon > 0;
   };
-  MSDP_TRY(fork());
-  for (int i = 0; i < nb; ++i) {
-    Blk& bk = blk[i];
-    if (!deferred(bk)) continue;
-    OnBlock on(this, i);
-    SchurScaling sc;
-    MSDP_TRY(schur_scaling(bk, sc));
-    MSDP_TRY(schur_block(st, bk.ops, sc, nullptr, m, swork, swork_elems));
+  // every block's products as recorded ops: one batched launch per product
+  // for all the blocks (NT and dual-centering scalings: svec panels)
+  const bool recorded = split && multi && blk[0].asmT && (nt() || dual_centering());
+  if (recorded) {
+    MSDP_TRY(rec_begin());
+    for (int i = 0; i < nb; ++i) {
+      Blk& bk = blk[i];
+      if (!deferred(bk)) continue;
+      OnBlock on(this, i);
+      SchurScaling sc;
+      MSDP_TRY(schur_scaling(bk, sc));
+      MSDP_TRY(dense_schur_record(st, bk.ops, sc, bk.asmT, bk.asmP));
+    }
+    MSDP_TRY(rec_end());
+  } else {
+    MSDP_TRY(fork());
+    for (int i = 0; i < nb; ++i) {
+      Blk& bk = blk[i];
+      if (!deferred(bk)) continue;
+      OnBlock on(this, i);
+      SchurScaling sc;
+      MSDP_TRY(schur_scaling(bk, sc));
+      MSDP_TRY(schur_block(st, bk.ops, sc, nullptr, m, swork, swork_elems));
+    }
+    MSDP_TRY(join());
   }
-  MSDP_TRY(join());
   for (auto& bk : blk) {
     if (deferred(bk)) {
       MSDP_TRY(scatter_lower_add(st, bk.ops.ncon, bk.ops.con_list, bk.ops.Hbuf, Hs, m));
