@@ -1091,12 +1091,18 @@ int msdp_solver::prepare(bool& ok) {
     const bool pair = need_lx && n > 64;
     if (pair) {
       MSDP_TRY(cholesky_factor2(bk, bk.S, bk.Ls, bk.X, bk.Lx, binfo(i, 0)));
-      // L_S^-1 and L_X^-1 in the same launches (a batch of two at the
-      // matrices' distance, like the paired factorisation)
-      MSDP_TRY(opaque([=] {
-        return trtri_lower_batched(st, n, 2, bp->Ls, n, bp->Lx - bp->Ls, bp->Lsinv, n,
-                                   bp->Lxinv - bp->Lsinv, trwork, (int64_t)trtri_work_elems(n));
-      }));
+      if (n <= 128) {
+        // recorded: batched with the other blocks' inverses at this position
+        MSDP_TRY(trtri_lower(st, n, bk.Ls, n, bk.Lsinv, n, trwork));
+        MSDP_TRY(trtri_lower(st, n, bk.Lx, n, bk.Lxinv, n, trwork));
+      } else {
+        // L_S^-1 and L_X^-1 in the same launches (a batch of two at the
+        // matrices' distance, like the paired factorisation)
+        MSDP_TRY(opaque([=] {
+          return trtri_lower_batched(st, n, 2, bp->Ls, n, bp->Lx - bp->Ls, bp->Lsinv, n,
+                                     bp->Lxinv - bp->Lsinv, trwork, (int64_t)trtri_work_elems(n));
+        }));
+      }
       MSDP_TRY(inverse_product(bk));
     } else {
       MSDP_TRY(prepare_block(bk, i, false));
