@@ -1206,8 +1206,10 @@ __device__ void syev_small(const EigBatch& it) {
         bool split = false;
         double dip1 = d[m], ei = e[m - 1], di = d[m - 1];
         for (int i = m - 1; i >= l; --i) {
-          double en = 0.0, dn = 0.0;
-          if (i > l) { en = e[i - 1]; dn = d[i - 1]; }
+          // the next pair, loaded without a branch (a guarded load would
+          // be waited for at once); at i == l it is read and not used
+          const int ip = i > 0 ? i - 1 : 0;
+          const double en = e[ip], dn = d[ip];
           const double f = s * ei, b = c * ei;
           const double x = fma(f, f, g * g);
           if (x == 0.0) {
@@ -1250,13 +1252,36 @@ __device__ void syev_small(const EigBatch& it) {
     __syncthreads();
     const int cnt = s_cnt, top = s_top;
     if (tid < n) {
+      // rotation q maps (z_i, z_i+1) of the row, i = top - q, downwards: the
+      // entry it leaves at i is the next rotation's z_i+1, carried in f
       double* zr = As[tid];
-      for (int q = 0; q < cnt; ++q) {
-        const int i = top - q;
-        const double c = cs[q], s = sn[q];
-        const double f = zr[i + 1];
-        zr[i + 1] = s * zr[i] + c * f;
-        zr[i] = c * zr[i] - s * f;
+      if (cnt > 0) {
+        double f = zr[top + 1];
+        int q = 0;
+        // four rotations per group, every operand loaded at the group's start
+        for (; q + 4 <= cnt; q += 4) {
+          const int i = top - q;
+          double c[4], s[4], a[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            c[u] = cs[q + u];
+            s[u] = sn[q + u];
+            a[u] = zr[i - u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            zr[i - u + 1] = s[u] * a[u] + c[u] * f;
+            f = c[u] * a[u] - s[u] * f;
+          }
+        }
+        for (; q < cnt; ++q) {
+          const int i = top - q;
+          const double c = cs[q], s = sn[q];
+          const double a = zr[i];
+          zr[i + 1] = s * a + c * f;
+          f = c * a - s * f;
+        }
+        zr[top - cnt + 1] = f;
       }
     }
     if (s_state != 0) break;
