@@ -1333,12 +1333,10 @@ static int potrf_blocked(hipStream_t s, int n, double* A, int64_t lda, int64_t s
       const int w = J1 - j0 - jb;  // the panel's later columns
       if (w <= 0) continue;
       double* A22 = A21 + (int64_t)jb * lda;
-      // the panel's square part (lower) and the rows below it
-      MSDP_TRY(gemm_guarded(s, false, true, w, w, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
+      // the panel's later columns, all rows below the step: one launch (the
+      // lower mask skips the square part's upper triangle)
+      MSDP_TRY(gemm_guarded(s, false, true, rest, w, jb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
                             count, sA, sA, sA, true, info));
-      if (n > J1)
-        MSDP_TRY(gemm_guarded(s, false, true, n - J1, w, jb, -1.0, A21 + w, lda, A21, lda, 1.0,
-                              A22 + w, lda, count, sA, sA, sA, false, info));
     }
     if (n > J1) {
       double* P = A + J1 + (int64_t)J0 * lda;
