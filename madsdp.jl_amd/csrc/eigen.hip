@@ -956,25 +956,25 @@ __global__ __launch_bounds__(256) void tridiag_min_kernel(int n, const double* d
 // ==========================================================================
 constexpr int EV_MAX = 128;
 
-__global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) {
-  const EigBatch it = items[blockIdx.x];
-  const int n = it.n;
-  __shared__ double As[EV_MAX][EV_MAX + 1];
-  __shared__ double v[EV_MAX], p[EV_MAX], w[EV_MAX], d[EV_MAX], e[EV_MAX];
-  __shared__ double red[16];
-  __shared__ double s_tau, s_scale, lo_s, hi_s;
-  __shared__ int first;
-  const int tid = threadIdx.x, NT = 512;
-  for (int x = tid; x < n * n; x += NT) {
-    const int i = x % n, j = x / n;
-    As[i][j] = it.A[i + (int64_t)j * it.lda];
-  }
-  __syncthreads();
+// Householder reduction of the n x n (n <= 128) symmetric matrix in As to
+// tridiagonal form (d[0..n-1], e[0..n-2]) by 512 threads, four barriers per
+// step: wavefront 0 forms the reflector, all threads p = tau A v, wavefront 0
+// w = p - (tau/2)(p'v) v, all threads the rank-2 update.  The two sums (the
+// reflector's norm, p'v) are wave sums over lanes holding rows k + c + lane
+// and k + c + 64 + lane, added in that order: the sums of a block reduction
+// over one row per thread.  With Hv, column k receives the reflector (v
+// below the diagonal, tau on it).
+__device__ __attribute__((always_inline)) inline void trd_lds(double (*As)[EV_MAX + 1], int n,
+                                                              double* v, double* p, double* w,
+                                                              double* d, double* e, double* Hv,
+                                                              double* s_tau) {
+  const int tid = threadIdx.x, NT = blockDim.x;
   for (int k = 0; k < n - 2; ++k) {
-    double ss = 0.0;
-    for (int i = k + 2 + tid; i < n; i += NT) ss += As[i][k] * As[i][k];
-    ss = block_reduce<0>(ss, red);
-    if (tid == 0) {
+    if (tid < 64) {
+      const int lane = tid, i0 = k + 2 + lane, i1 = i0 + 64;
+      const double a0 = i0 < n ? As[i0][k] : 0.0, a1 = i1 < n ? As[i1][k] : 0.0;
+      // each term as fma(x, x, 0) (a product, not contracted into the sum)
+      const double ss = wave_sum(fma(a0, a0, 0.0)) + wave_sum(fma(a1, a1, 0.0));
       const double alpha = As[k + 1][k];
       double beta, t, sc;
       if (ss == 0.0) {
@@ -984,17 +984,21 @@ __global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) 
         t = (beta - alpha) / beta;
         sc = 1.0 / (alpha - beta);
       }
-      d[k] = As[k][k];
-      e[k] = beta;
-      s_tau = t;
-      s_scale = sc;
+      if (lane == 0) {
+        d[k] = As[k][k];
+        e[k] = beta;
+        *s_tau = t;
+        if (Hv) Hv[k + (int64_t)k * n] = t;
+      }
+      for (int i = k + 1 + lane; i < n; i += 64) {
+        const double vi = i == k + 1 ? 1.0 : As[i][k] * sc;
+        v[i] = vi;
+        if (Hv) Hv[i + (int64_t)k * n] = vi;
+      }
     }
     __syncthreads();
-    const double t = s_tau;
+    const double t = *s_tau;
     if (t == 0.0) continue;  // uniform: H_k = I
-    const double sc = s_scale;
-    for (int i = k + 1 + tid; i < n; i += NT) v[i] = i == k + 1 ? 1.0 : As[i][k] * sc;
-    __syncthreads();
     const int L = n - k - 1;
     {  // p = t A22 v, 4 lanes per row
       const int r = tid >> 2, q = tid & 3, i = k + 1 + r;
@@ -1006,11 +1010,15 @@ __global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) 
       if (q == 0 && r < L) p[i] = t * acc;
     }
     __syncthreads();
-    double pv = 0.0;
-    for (int i = k + 1 + tid; i < n; i += NT) pv += p[i] * v[i];
-    pv = block_reduce<0>(pv, red);
-    const double c2 = 0.5 * t * pv;
-    for (int i = k + 1 + tid; i < n; i += NT) w[i] = p[i] - c2 * v[i];
+    if (tid < 64) {
+      const int lane = tid, i0 = k + 1 + lane, i1 = i0 + 64;
+      const double q0 = i0 < n ? fma(p[i0], v[i0], 0.0) : 0.0;
+      const double q1 = i1 < n ? fma(p[i1], v[i1], 0.0) : 0.0;
+      const double pv = wave_sum(q0) + wave_sum(q1);
+      const double c2 = 0.5 * t * pv;
+      if (i0 < n) w[i0] = p[i0] - c2 * v[i0];
+      if (i1 < n) w[i1] = p[i1] - c2 * v[i1];
+    }
     __syncthreads();
     for (int x = tid; x < L * L; x += NT) {
       const int i = k + 1 + x % L, j = k + 1 + x / L;
@@ -1018,6 +1026,23 @@ __global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) 
     }
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(512) void eigvals_vb_kernel(const EigBatch* items) {
+  const EigBatch it = items[blockIdx.x];
+  const int n = it.n;
+  __shared__ double As[EV_MAX][EV_MAX + 1];
+  __shared__ double v[EV_MAX], p[EV_MAX], w[EV_MAX], d[EV_MAX], e[EV_MAX];
+  __shared__ double red[16];
+  __shared__ double s_tau, lo_s, hi_s;
+  __shared__ int first;
+  const int tid = threadIdx.x, NT = 512;
+  for (int x = tid; x < n * n; x += NT) {
+    const int i = x % n, j = x / n;
+    As[i][j] = it.A[i + (int64_t)j * it.lda];
+  }
+  __syncthreads();
+  trd_lds(As, n, v, p, w, d, e, nullptr, &s_tau);
   if (tid == 0) {
     if (n >= 2) {
       d[n - 2] = As[n - 2][n - 2];
@@ -1089,8 +1114,7 @@ __device__ void syev_small(const EigBatch& it) {
   __shared__ double As[EV_MAX][EV_MAX + 1];
   __shared__ double v[EV_MAX], p[EV_MAX], w[EV_MAX], d[EV_MAX], e[EV_MAX];
   __shared__ double cs[EV_MAX], sn[EV_MAX];
-  __shared__ double red[16];
-  __shared__ double s_tau, s_scale;
+  __shared__ double s_tau;
   __shared__ int s_cnt, s_top, s_state;
   const int tid = threadIdx.x, NT = blockDim.x;
   double* Hv = it.Vtmp;
@@ -1099,58 +1123,7 @@ __device__ void syev_small(const EigBatch& it) {
     As[i][j] = it.A[i + (int64_t)j * it.lda];
   }
   __syncthreads();
-  for (int k = 0; k < n - 2; ++k) {
-    double ss = 0.0;
-    for (int i = k + 2 + tid; i < n; i += NT) ss += As[i][k] * As[i][k];
-    ss = block_reduce<0>(ss, red);
-    if (tid == 0) {
-      const double alpha = As[k + 1][k];
-      double beta, t, sc;
-      if (ss == 0.0) {
-        beta = alpha; t = 0.0; sc = 0.0;
-      } else {
-        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-        t = (beta - alpha) / beta;
-        sc = 1.0 / (alpha - beta);
-      }
-      d[k] = As[k][k];
-      e[k] = beta;
-      s_tau = t;
-      s_scale = sc;
-      Hv[k + (int64_t)k * n] = t;
-    }
-    __syncthreads();
-    const double t = s_tau, sc = s_scale;
-    for (int i = k + 1 + tid; i < n; i += NT) {
-      const double vi = i == k + 1 ? 1.0 : As[i][k] * sc;
-      v[i] = vi;
-      Hv[i + (int64_t)k * n] = vi;
-    }
-    __syncthreads();
-    if (t == 0.0) continue;  // uniform: H_k = I
-    const int L = n - k - 1;
-    {
-      const int r = tid >> 2, q = tid & 3, i = k + 1 + r;
-      double acc = 0.0;
-      if (r < L)
-        for (int j = k + 1 + q; j < n; j += 4) acc += As[i][j] * v[j];
-      acc += __shfl_xor(acc, 1);
-      acc += __shfl_xor(acc, 2);
-      if (q == 0 && r < L) p[i] = t * acc;
-    }
-    __syncthreads();
-    double pv = 0.0;
-    for (int i = k + 1 + tid; i < n; i += NT) pv += p[i] * v[i];
-    pv = block_reduce<0>(pv, red);
-    const double c2 = 0.5 * t * pv;
-    for (int i = k + 1 + tid; i < n; i += NT) w[i] = p[i] - c2 * v[i];
-    __syncthreads();
-    for (int x = tid; x < L * L; x += NT) {
-      const int i = k + 1 + x % L, j = k + 1 + x / L;
-      As[i][j] -= v[i] * w[j] + w[i] * v[j];
-    }
-    __syncthreads();
-  }
+  trd_lds(As, n, v, p, w, d, e, Hv, &s_tau);
   if (tid == 0) {
     if (n >= 2) {
       d[n - 2] = As[n - 2][n - 2];
