@@ -1260,28 +1260,45 @@ __device__ void syev_small(const EigBatch& it) {
     if (s_state != 0) break;
   }
   __syncthreads();
-  // V = H_0 ... H_{n-3} Z
-  for (int k = n - 3; k >= 0; --k) {
-    const double t = Hv[k + (int64_t)k * n];
-    if (t == 0.0) continue;  // uniform
-    for (int i = k + 1 + tid; i < n; i += NT) v[i] = Hv[i + (int64_t)k * n];
-    __syncthreads();
-    {
-      const int c = tid >> 2, q = tid & 3;
-      double acc = 0.0;
-      if (c < n)
-        for (int i = k + 1 + q; i < n; i += 4) acc += v[i] * As[i][c];
-      acc += __shfl_xor(acc, 1);
-      acc += __shfl_xor(acc, 2);
-      if (q == 0 && c < n) w[c] = t * acc;
+  // V = H_0 ... H_{n-3} Z.  Two barriers per reflector: its vector is
+  // staged in LDS during the previous reflector's update (v and p alternate).
+  {
+    int cur = 0;
+    bool staged = false;
+    for (int k = n - 3; k >= 0; --k) {
+      const double t = Hv[k + (int64_t)k * n];
+      if (t == 0.0) {  // uniform
+        staged = false;
+        continue;
+      }
+      double* vk = cur ? p : v;
+      if (!staged) {
+        for (int i = k + 1 + tid; i < n; i += NT) vk[i] = Hv[i + (int64_t)k * n];
+        __syncthreads();
+      }
+      {
+        const int c = tid >> 2, q = tid & 3;
+        double acc = 0.0;
+        if (c < n)
+          for (int i = k + 1 + q; i < n; i += 4) acc += vk[i] * As[i][c];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        if (q == 0 && c < n) w[c] = t * acc;
+      }
+      __syncthreads();
+      if (k > 0) {
+        double* vn = cur ? v : p;
+        for (int i = k + tid; i < n; i += NT) vn[i] = Hv[i + (int64_t)(k - 1) * n];
+      }
+      const int L = n - k - 1;
+      for (int x = tid; x < L * n; x += NT) {
+        const int i = k + 1 + x % L, c = x / L;
+        As[i][c] -= vk[i] * w[c];
+      }
+      __syncthreads();
+      cur ^= 1;
+      staged = k > 0;
     }
-    __syncthreads();
-    const int L = n - k - 1;
-    for (int x = tid; x < L * n; x += NT) {
-      const int i = k + 1 + x % L, c = x / L;
-      As[i][c] -= v[i] * w[c];
-    }
-    __syncthreads();
   }
   // sort by eigenvalue (ties by index) and write
   for (int i = tid; i < n; i += NT) {
