@@ -150,6 +150,21 @@ __device__ __attribute__((always_inline)) inline void gemm64_tile(
     __syncthreads();
   }
   const v4d* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+  // C's old values first, all loads issued together at clamped (in-bounds)
+  // addresses: loads under the store guards were each waited for before the
+  // next one was issued
+  double old[4][4];
+  if (beta != 0.0) {  // uniform
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int gn = min(n0 + wn + 16 * (q & 1) + li, N - 1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = min(m0 + wm + 16 * (q >> 1) + lk + 4 * r, M - 1);
+        old[q][r] = C[gm + (int64_t)gn * ldc];
+      }
+    }
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int i0 = m0 + wm + 16 * (q >> 1) + lk;
@@ -159,9 +174,8 @@ __device__ __attribute__((always_inline)) inline void gemm64_tile(
     for (int r = 0; r < 4; ++r) {
       const int gm = i0 + 4 * r;
       if (gm >= M || (lower && gm < gn)) continue;
-      double* c = C + gm + (int64_t)gn * ldc;
       const double v = alpha * (*accs[q])[r];
-      *c = beta == 0.0 ? v : v + beta * (*c);
+      C[gm + (int64_t)gn * ldc] = beta == 0.0 ? v : v + beta * old[q][r];
     }
   }
 }
@@ -280,6 +294,21 @@ __device__ __attribute__((always_inline)) inline void gemm128_store(
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
+    // this row group's old C values first, the 16 loads issued together at
+    // clamped (in-bounds) addresses: loads under the store guards were each
+    // waited for before the next one was issued
+    double old[4][4];
+    if (beta != 0.0) {  // uniform
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int gn = min(n0 + wn + 16 * b + li, N - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = min(m0 + wm + 16 * a + lk + 4 * r, M - 1);
+          old[b][r] = C[gm + (int64_t)gn * ldc];
+        }
+      }
+    }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int gn = n0 + wn + 16 * b + li;
@@ -288,9 +317,8 @@ __device__ __attribute__((always_inline)) inline void gemm128_store(
       for (int r = 0; r < 4; ++r) {
         const int gm = m0 + wm + 16 * a + lk + 4 * r;
         if (gm >= M || (lower && gm < gn)) continue;
-        double* c = C + gm + (int64_t)gn * ldc;
         const double v = alpha * acc[a][b][r];
-        *c = beta == 0.0 ? v : v + beta * (*c);
+        C[gm + (int64_t)gn * ldc] = beta == 0.0 ? v : v + beta * old[b][r];
       }
     }
   }
