@@ -1113,9 +1113,9 @@ __device__ void syev_small(const EigBatch& it) {
   const int n = it.n;
   __shared__ double As[EV_MAX][EV_MAX + 1];
   __shared__ double v[EV_MAX], p[EV_MAX], w[EV_MAX], d[EV_MAX], e[EV_MAX];
-  __shared__ double cs[EV_MAX], sn[EV_MAX];
+  __shared__ double cs[2 * EV_MAX], sn[2 * EV_MAX];  // two sweeps' rotations
   __shared__ double s_tau;
-  __shared__ int s_cnt, s_top, s_state;
+  __shared__ int s_cnt[2], s_top[2], s_state;
   const int tid = threadIdx.x, NT = blockDim.x;
   double* Hv = it.Vtmp;
   for (int x = tid; x < n * n; x += NT) {
@@ -1142,7 +1142,7 @@ __device__ void syev_small(const EigBatch& it) {
   // control in every lane (same values): the split search is one ballot per
   // 64 entries, and the chase carries d[i], e[i] in registers, loading the
   // next pair one rotation ahead (d[i] is not written by rotation i).
-  int l = 0, iter = 0;
+  int l = 0, iter = 0, sweep = 0;
   for (;;) {
     __syncthreads();
     if (tid < 64) {
@@ -1201,8 +1201,8 @@ __device__ void syev_small(const EigBatch& it) {
           pp = s * r;
           d[i + 1] = g + pp;
           g = c * r - b;
-          cs[cnt] = c;
-          sn[cnt] = s;
+          cs[EV_MAX * (sweep & 1) + cnt] = c;
+          sn[EV_MAX * (sweep & 1) + cnt] = s;
           ++cnt;
           dip1 = di;
           ei = en;
@@ -1218,16 +1218,20 @@ __device__ void syev_small(const EigBatch& it) {
       }
       if (lane == 0) {
         s_state = state;
-        s_cnt = cnt;
-        s_top = top;
+        s_cnt[sweep & 1] = cnt;
+        s_top[sweep & 1] = top;
       }
-    }
-    __syncthreads();
-    const int cnt = s_cnt, top = s_top;
-    if (tid < n) {
+    } else if (sweep > 0 && tid - 64 < n) {
+      // wavefronts 1.. apply the previous sweep's rotations to the rows of Z
+      // while wavefront 0 forms this sweep's (they touch d, e only).  A sweep
+      // that sets the state forms no rotations, so none are left after it.
+      const int pb = (sweep - 1) & 1;
+      const int cnt = s_cnt[pb], top = s_top[pb];
+      const double* csp = cs + EV_MAX * pb;
+      const double* snp = sn + EV_MAX * pb;
       // rotation q maps (z_i, z_i+1) of the row, i = top - q, downwards: the
       // entry it leaves at i is the next rotation's z_i+1, carried in f
-      double* zr = As[tid];
+      double* zr = As[tid - 64];
       if (cnt > 0) {
         double f = zr[top + 1];
         int q = 0;
@@ -1237,8 +1241,8 @@ __device__ void syev_small(const EigBatch& it) {
           double c[4], s[4], a[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            c[u] = cs[q + u];
-            s[u] = sn[q + u];
+            c[u] = csp[q + u];
+            s[u] = snp[q + u];
             a[u] = zr[i - u];
           }
 #pragma unroll
@@ -1249,7 +1253,7 @@ __device__ void syev_small(const EigBatch& it) {
         }
         for (; q < cnt; ++q) {
           const int i = top - q;
-          const double c = cs[q], s = sn[q];
+          const double c = csp[q], s = snp[q];
           const double a = zr[i];
           zr[i + 1] = s * a + c * f;
           f = c * a - s * f;
@@ -1257,7 +1261,9 @@ __device__ void syev_small(const EigBatch& it) {
         zr[top - cnt + 1] = f;
       }
     }
+    __syncthreads();
     if (s_state != 0) break;
+    ++sweep;
   }
   __syncthreads();
   // V = H_0 ... H_{n-3} Z.  Two barriers per reflector: its vector is
